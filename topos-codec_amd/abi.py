@@ -1,0 +1,187 @@
+"""ctypes mirrors of the public frame ABI structs and status codes.
+
+The layouts follow the frame section of the Topos C ABI
+(native/include/topos_codec.h:33-56 status codes, :175-224 topos_frame_config,
+:230-237 topos_frame_input, :240-254 topos_frame_stats, :262-285
+topos_frame_output, :374-380 topos_batch_packet, :426-431 topos_plane_view,
+:487-495 topos_decoder_config), so one set of Python structs drives both this
+library and any other implementation of the same ABI.
+"""
+import ctypes
+
+TOPOS_CODEC_ABI_VERSION = 2
+TC_FRAME_MAX_PLANES = 4
+TC_MAX_SLICES = 512
+
+TC_OK = 0
+TC_WARN_CONCEALED = 1
+TC_ERR_INVALID_ARGUMENT = -1
+TC_ERR_OUT_OF_MEMORY = -2
+TC_ERR_UNSUPPORTED_VERSION = -3
+TC_ERR_UNSUPPORTED_PROFILE = -4
+TC_ERR_UNSUPPORTED_PIXEL_FORMAT = -5
+TC_ERR_UNSUPPORTED_MATRIX = -6
+TC_ERR_UNSUPPORTED_ALPHA_MODE = -7
+TC_ERR_LIMIT_EXCEEDED = -8
+TC_ERR_MALFORMED = -9
+TC_ERR_TRUNCATED = -10
+TC_ERR_CHECKSUM_MISMATCH = -11
+TC_ERR_STATE = -12
+TC_ERR_CANCELLED = -13
+TC_ERR_IO = -14
+TC_ERR_BUFFER_TOO_SMALL = -15
+TC_ERR_NOT_IMPLEMENTED = -16
+
+
+class FrameConfig(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("visible_width", ctypes.c_uint16),
+        ("visible_height", ctypes.c_uint16),
+        ("profile", ctypes.c_uint8),
+        ("pixel_format", ctypes.c_uint8),
+        ("bit_depth", ctypes.c_uint8),
+        ("qmatrix_id", ctypes.c_uint8),
+        ("qp_base", ctypes.c_uint8),
+        ("qp_delta_luma", ctypes.c_int8),
+        ("qp_delta_chroma", ctypes.c_int8),
+        ("slice_rows", ctypes.c_uint8),
+        ("alpha_mode", ctypes.c_uint8),
+        ("alpha_bit_depth", ctypes.c_uint8),
+        ("alpha_premultiplied", ctypes.c_uint8),
+        ("color_range", ctypes.c_uint8),
+        ("color_primaries", ctypes.c_uint8),
+        ("color_transfer", ctypes.c_uint8),
+        ("color_matrix", ctypes.c_uint8),
+        ("chroma_siting", ctypes.c_uint8),
+        ("sar_num", ctypes.c_uint16),
+        ("sar_den", ctypes.c_uint16),
+        ("reserved", ctypes.c_uint32 * 8),
+    ]
+
+
+class FrameInput(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("planes", ctypes.c_void_p * TC_FRAME_MAX_PLANES),
+        ("strides", ctypes.c_size_t * TC_FRAME_MAX_PLANES),
+        ("reserved", ctypes.c_uint32 * 8),
+    ]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("packet_size", ctypes.c_uint32),
+        ("color_payload_bytes", ctypes.c_uint32),
+        ("alpha_payload_bytes", ctypes.c_uint32),
+        ("color_header_bytes", ctypes.c_uint32),
+        ("alpha_header_bytes", ctypes.c_uint32),
+        ("slice_count", ctypes.c_uint16),
+        ("qp_base", ctypes.c_uint8),
+        ("reserved8", ctypes.c_uint8),
+        ("alpha_max_abs_error", ctypes.c_uint16),
+        ("reserved16", ctypes.c_uint16),
+        ("reserved", ctypes.c_uint32 * 8),
+    ]
+
+
+class FrameOutput(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("visible_width", ctypes.c_uint16),
+        ("visible_height", ctypes.c_uint16),
+        ("coded_width", ctypes.c_uint16),
+        ("coded_height", ctypes.c_uint16),
+        ("plane_count", ctypes.c_uint8),
+        ("bit_depth", ctypes.c_uint8),
+        ("profile", ctypes.c_uint8),
+        ("pixel_format", ctypes.c_uint8),
+        ("alpha_mode", ctypes.c_uint8),
+        ("alpha_bit_depth", ctypes.c_uint8),
+        ("color_range", ctypes.c_uint8),
+        ("color_primaries", ctypes.c_uint8),
+        ("color_transfer", ctypes.c_uint8),
+        ("color_matrix", ctypes.c_uint8),
+        ("chroma_siting", ctypes.c_uint8),
+        ("concealed_slices", ctypes.c_uint16),
+        ("slice_count", ctypes.c_uint16),
+        ("sar_num", ctypes.c_uint16),
+        ("sar_den", ctypes.c_uint16),
+        ("slice_status", ctypes.c_uint8 * TC_MAX_SLICES),
+        ("reserved", ctypes.c_uint32 * 8),
+    ]
+
+
+class BatchPacket(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("size", ctypes.c_size_t)]
+
+
+class PlaneView(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("pixels", ctypes.c_void_p),
+        ("stride", ctypes.c_size_t),
+    ]
+
+
+class DecoderConfig(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("max_slice_workers", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 7),
+    ]
+
+
+def make_config(width, height, qp, *, profile=0, pixel_format=0, bit_depth=10,
+                qmatrix_id=0, slice_rows=0, entropy_mode=8, features=0,
+                alpha_mode=0, alpha_bit_depth=0, qp_delta_luma=0,
+                qp_delta_chroma=0, **extra):
+    """Build a zero-initialised FrameConfig with the given fields set."""
+    c = FrameConfig()
+    c.struct_size = ctypes.sizeof(FrameConfig)
+    c.abi_version = TOPOS_CODEC_ABI_VERSION
+    c.visible_width = width
+    c.visible_height = height
+    c.profile = profile
+    c.pixel_format = pixel_format
+    c.bit_depth = bit_depth
+    c.qmatrix_id = qmatrix_id
+    c.qp_base = qp
+    c.slice_rows = slice_rows
+    c.alpha_mode = alpha_mode
+    c.alpha_bit_depth = alpha_bit_depth
+    c.qp_delta_luma = qp_delta_luma
+    c.qp_delta_chroma = qp_delta_chroma
+    c.reserved[0] = entropy_mode
+    c.reserved[5] = features
+    for k, v in extra.items():
+        setattr(c, k, v)
+    return c
+
+
+def config_fields(cfg):
+    """Plain-dict copy of a FrameConfig (for logging / fixtures)."""
+    out = {}
+    for name, _ in FrameConfig._fields_:
+        v = getattr(cfg, name)
+        out[name] = list(v) if name == "reserved" else int(v)
+    return out
+
+
+def plane_dims(width, height, pixel_format, alpha_mode):
+    """Visible (w, h) of each input plane, in the ABI's plane order."""
+    if pixel_format == 3:
+        pw, ph = (width + 1) // 2, (height + 1) // 2
+        return [(pw, ph)] * 4
+    cw = (width + 1) // 2 if pixel_format == 0 else width
+    dims = [(width, height), (cw, height), (cw, height)]
+    if alpha_mode:
+        dims.append((width, height))
+    return dims
