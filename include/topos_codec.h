@@ -1,0 +1,200 @@
+/* topos_codec.h — frame encode/decode C ABI of the MI355X intra backend.
+ *
+ * Drop-in for the frame section of the Topos C ABI: the structs below have the
+ * same layout (ABI version 2) and the functions the same names, argument
+ * meaning, status codes and error behaviour as the reference declarations they
+ * replace (reference file native/include/topos_codec.h, line numbers cited per
+ * item). Every call runs on the first visible AMD GPU; the calls are
+ * thread-safe (device work is serialised per process) and a decoder context is
+ * not re-entrant.
+ */
+#ifndef TOPOS_CODEC_H
+#define TOPOS_CODEC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI generation of the public structs (reference topos_codec.h:31) */
+#define TOPOS_CODEC_ABI_VERSION 2
+
+/* status codes, frozen values (reference topos_codec.h:33-56) */
+enum {
+    TC_OK = 0,
+    TC_WARN_CONCEALED = 1,
+    TC_ERR_INVALID_ARGUMENT = -1,
+    TC_ERR_OUT_OF_MEMORY = -2,
+    TC_ERR_UNSUPPORTED_VERSION = -3,
+    TC_ERR_UNSUPPORTED_PROFILE = -4,
+    TC_ERR_UNSUPPORTED_PIXEL_FORMAT = -5,
+    TC_ERR_UNSUPPORTED_MATRIX = -6,
+    TC_ERR_UNSUPPORTED_ALPHA_MODE = -7,
+    TC_ERR_LIMIT_EXCEEDED = -8,
+    TC_ERR_MALFORMED = -9,
+    TC_ERR_TRUNCATED = -10,
+    TC_ERR_CHECKSUM_MISMATCH = -11,
+    TC_ERR_STATE = -12,
+    TC_ERR_CANCELLED = -13,
+    TC_ERR_IO = -14,
+    TC_ERR_BUFFER_TOO_SMALL = -15,
+    TC_ERR_NOT_IMPLEMENTED = -16,
+    TC_ERR_REFERENCE_INVALID = -17
+};
+
+#define TC_FRAME_MAX_PLANES 4
+#define TC_FRAME_DEFAULT_SLICE_ROWS 16u
+#define TC_FRAME_SLICE_OK 0u
+#define TC_FRAME_SLICE_CONCEALED 1u
+#define TC_MAX_SLICES 512
+
+/* encoder configuration; all-zero (plus struct_size/abi_version) is a valid
+ * minimal config (reference topos_codec.h:175-224). reserved[0] low byte
+ * selects the entropy mode (8 = rANS2 intra product mode), reserved[5] the
+ * output features of mode 8 (bit0 deblock, bit1 refined qp table). */
+typedef struct topos_frame_config {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t visible_width;
+    uint16_t visible_height;
+    uint8_t profile;
+    uint8_t pixel_format;      /* 0 YUV 4:2:2, 1 YUV 4:4:4, 2 GBR, 3 CFA (4 phase planes) */
+    uint8_t bit_depth;
+    uint8_t qmatrix_id;
+    uint8_t qp_base;
+    int8_t qp_delta_luma;
+    int8_t qp_delta_chroma;
+    uint8_t slice_rows;        /* block rows per slice; 0 = 16 */
+    uint8_t alpha_mode;        /* 0 none, 1 lossless, 2 limited precision */
+    uint8_t alpha_bit_depth;
+    uint8_t alpha_premultiplied;
+    uint8_t color_range;
+    uint8_t color_primaries;
+    uint8_t color_transfer;
+    uint8_t color_matrix;
+    uint8_t chroma_siting;
+    uint16_t sar_num;
+    uint16_t sar_den;
+    uint32_t reserved[8];
+} topos_frame_config;
+
+/* planar uint16 input; strides in uint16 elements, 0 = tight
+ * (reference topos_codec.h:226-237) */
+typedef struct topos_frame_input {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    const uint16_t* planes[TC_FRAME_MAX_PLANES];
+    size_t strides[TC_FRAME_MAX_PLANES];
+    uint32_t reserved[8];
+} topos_frame_input;
+
+/* encode statistics (reference topos_codec.h:239-254) */
+typedef struct topos_frame_stats {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t packet_size;
+    uint32_t color_payload_bytes;
+    uint32_t alpha_payload_bytes;
+    uint32_t color_header_bytes;
+    uint32_t alpha_header_bytes;
+    uint16_t slice_count;
+    uint8_t qp_base;
+    uint8_t reserved8;
+    uint16_t alpha_max_abs_error;
+    uint16_t reserved16;
+    uint32_t reserved[8];
+} topos_frame_stats;
+
+/* decode description + per-slice status (reference topos_codec.h:256-285) */
+typedef struct topos_frame_output {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t visible_width;
+    uint16_t visible_height;
+    uint16_t coded_width;
+    uint16_t coded_height;
+    uint8_t plane_count;
+    uint8_t bit_depth;
+    uint8_t profile;
+    uint8_t pixel_format;
+    uint8_t alpha_mode;
+    uint8_t alpha_bit_depth;
+    uint8_t color_range;
+    uint8_t color_primaries;
+    uint8_t color_transfer;
+    uint8_t color_matrix;
+    uint8_t chroma_siting;
+    uint16_t concealed_slices;
+    uint16_t slice_count;
+    uint16_t sar_num;
+    uint16_t sar_den;
+    uint8_t slice_status[TC_MAX_SLICES];
+    uint32_t reserved[8];
+} topos_frame_output;
+
+/* batch packet reference (reference topos_codec.h:387-390) */
+typedef struct topos_batch_packet {
+    const uint8_t* data;
+    size_t size;
+} topos_batch_packet;
+
+/* output plane view for decoder contexts (reference topos_codec.h:427-432) */
+typedef struct topos_plane_view {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t* pixels;
+    size_t stride;
+} topos_plane_view;
+
+/* decoder context config (reference topos_codec.h:502-510) */
+typedef struct topos_decoder_config {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t max_slice_workers;
+    uint32_t reserved[7];
+} topos_decoder_config;
+
+typedef struct tc_decoder tc_decoder;
+
+/* reference topos_codec.h:125 */
+int32_t tc_abi_version(void);
+/* reference topos_codec.h:146 */
+const char* tc_status_message(int32_t status);
+/* reference topos_codec.h:150 */
+const char* tc_last_error(void);
+
+/* reference topos_codec.h:288 */
+int32_t tc_frame_config_validate(const topos_frame_config* cfg);
+/* reference topos_codec.h:292 */
+size_t tc_frame_packet_bound(const topos_frame_config* cfg);
+/* reference topos_codec.h:296-297: out untouched unless TC_OK */
+int32_t tc_frame_encode(const topos_frame_config* cfg, const topos_frame_input* input,
+                        uint8_t* out, size_t out_cap, topos_frame_stats* stats);
+/* reference topos_codec.h:336-339: planes_out == NULL is query mode */
+int32_t tc_frame_decode(const uint8_t* data, size_t size,
+                        uint16_t* const planes_out[TC_FRAME_MAX_PLANES],
+                        const size_t strides[TC_FRAME_MAX_PLANES], topos_frame_output* out_info);
+/* reference topos_codec.h:398-400: flat count x 4 plane pointers */
+int32_t tc_frame_decode_batch(const topos_batch_packet* packets, uint32_t count,
+                              uint16_t* const* planes_out, const size_t* strides,
+                              topos_frame_output* infos);
+/* reference topos_codec.h:410-411 */
+int32_t tc_frame_plane_geometry(const topos_frame_output* info, uint32_t plane,
+                                uint32_t* width, uint32_t* height);
+
+/* decoder contexts (reference topos_codec.h:515-552) */
+int32_t tc_decoder_create(const topos_decoder_config* cfg, tc_decoder** out);
+void tc_decoder_destroy(tc_decoder* dec);
+int32_t tc_decoder_prepare(tc_decoder* dec, const uint8_t* packet, size_t size,
+                           topos_frame_output* info);
+int32_t tc_decoder_decode(tc_decoder* dec, const uint8_t* packet, size_t size,
+                          const topos_plane_view out[TC_FRAME_MAX_PLANES], topos_frame_output* info);
+int32_t tc_decoder_decode_batch(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
+                                const topos_plane_view* out, topos_frame_output* infos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

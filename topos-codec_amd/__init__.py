@@ -1,0 +1,292 @@
+"""Python binding of the MI355X intra backend.
+
+Mirrors the frame part of the reference binding (python/topos_codec/
+topos_binding.py: ToposCodec.encode :3097, .decode :3308, .decode_batch :3462)
+over the C ABI in include/topos_codec.h, plus the device-resident batch calls
+of include/topos_gpu.h. The library is loaded from this directory; there is no
+fallback: without libtopos_codec_amd.so (or without a GPU) calls fail loudly.
+"""
+import ctypes
+import importlib.util
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load_abi():
+    spec = importlib.util.spec_from_file_location("topos_codec_amd_abi", os.path.join(_HERE, "abi.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+abi = _load_abi()
+LIB_PATH = os.path.join(_HERE, "libtopos_codec_amd.so")
+SYNTH_PATH = os.path.join(_HERE, "libtopos_synth.so")
+
+
+class ToposCodecError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ToposCodecError(abi.TC_ERR_NOT_IMPLEMENTED,
+                              f"{path} is not built (run topos-codec_amd/build.py)")
+    lib = ctypes.CDLL(path)
+    u8p, vp = ctypes.POINTER(ctypes.c_uint8), ctypes.c_void_p
+    lib.tc_last_error.restype = ctypes.c_char_p
+    lib.tc_status_message.restype = ctypes.c_char_p
+    lib.tc_frame_packet_bound.restype = ctypes.c_size_t
+    lib.tc_frame_packet_bound.argtypes = [vp]
+    lib.tc_frame_config_validate.argtypes = [vp]
+    lib.tc_frame_encode.argtypes = [vp, vp, vp, ctypes.c_size_t, vp]
+    lib.tc_frame_decode.argtypes = [vp, ctypes.c_size_t, vp, vp, vp]
+    lib.tc_frame_decode_batch.argtypes = [vp, ctypes.c_uint32, vp, vp, vp]
+    lib.tc_frame_plane_geometry.argtypes = [vp, ctypes.c_uint32, vp, vp]
+    lib.tcx_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t]
+    lib.tcx_dev_free.argtypes = [ctypes.c_uint64]
+    lib.tcx_h2d.argtypes = [ctypes.c_uint64, vp, ctypes.c_size_t]
+    lib.tcx_d2h.argtypes = [vp, ctypes.c_uint64, ctypes.c_size_t]
+    lib.tcx_encode_batch_dev.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
+    lib.tcx_decode_batch_dev.argtypes = [ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
+    lib.tcx_set_timing.argtypes = [ctypes.c_int]
+    lib.tcx_get_timing.argtypes = [vp]
+    del u8p
+    return lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class ToposCodec:
+    """Frame encode / decode through the GPU backend (host buffers in, host buffers out)."""
+
+    def __init__(self, path=LIB_PATH):
+        self._lib = load_library(path)
+
+    def last_error(self):
+        return self._lib.tc_last_error().decode(errors="replace")
+
+    def _check(self, status, what):
+        if status < 0:
+            raise ToposCodecError(status, f"{what}: {self.last_error()}")
+        return status
+
+    def validate(self, config):
+        return int(self._lib.tc_frame_config_validate(ctypes.byref(config)))
+
+    def packet_bound(self, config):
+        return int(self._lib.tc_frame_packet_bound(ctypes.byref(config)))
+
+    def encode(self, config, planes, stats_out=None):
+        """Encode one frame; planes are C-contiguous uint16 arrays in ABI plane order."""
+        inp = abi.FrameInput()
+        inp.struct_size = ctypes.sizeof(abi.FrameInput)
+        inp.abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        keep = []
+        for i, p in enumerate(planes):
+            a = np.ascontiguousarray(p, dtype=np.uint16)
+            keep.append(a)
+            inp.planes[i] = a.ctypes.data
+        need = self.packet_bound(config)
+        out = np.empty(max(need, 1), np.uint8)
+        st = stats_out if stats_out is not None else abi.FrameStats()
+        status = int(self._lib.tc_frame_encode(ctypes.byref(config), ctypes.byref(inp), _ptr(out),
+                                               ctypes.c_size_t(need), ctypes.byref(st)))
+        self._check(status, "tc_frame_encode")
+        return out[: st.packet_size].tobytes()
+
+    def encode_status(self, config, planes, out_cap=None):
+        """Raw call: returns (status, packet bytes or b'', stats)."""
+        inp = abi.FrameInput()
+        inp.struct_size = ctypes.sizeof(abi.FrameInput)
+        inp.abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        keep = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+        for i, a in enumerate(keep):
+            inp.planes[i] = a.ctypes.data
+        cap = self.packet_bound(config) if out_cap is None else out_cap
+        out = np.empty(max(cap, 1), np.uint8)
+        st = abi.FrameStats()
+        status = int(self._lib.tc_frame_encode(ctypes.byref(config), ctypes.byref(inp), _ptr(out),
+                                               ctypes.c_size_t(cap), ctypes.byref(st)))
+        pkt = out[: st.packet_size].tobytes() if status == 0 else b""
+        return status, pkt, st
+
+    def query(self, packet):
+        info = abi.FrameOutput()
+        status = int(self._lib.tc_frame_decode(packet, ctypes.c_size_t(len(packet)), None, None,
+                                               ctypes.byref(info)))
+        return status, info
+
+    def decode(self, packet, strict=True):
+        """Decode one frame; returns (status, [planes], info). Concealment is a warning."""
+        status, info = self.query(packet)
+        if status != 0:
+            if strict:
+                self._check(status, "tc_frame_decode(query)")
+            return status, [], info
+        dims = abi.plane_dims(info.visible_width, info.visible_height, info.pixel_format, info.alpha_mode)
+        outs = [np.zeros((h, w), np.uint16) for (w, h) in dims]
+        ptrs = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None] * (4 - len(outs))))
+        status = int(self._lib.tc_frame_decode(packet, ctypes.c_size_t(len(packet)), ptrs, None,
+                                               ctypes.byref(info)))
+        if strict:
+            self._check(status, "tc_frame_decode")
+        return status, outs, info
+
+    def decode_batch(self, packets):
+        n = len(packets)
+        infos = (abi.FrameOutput * n)()
+        pk = (abi.BatchPacket * n)()
+        bufs = [np.frombuffer(p, np.uint8) for p in packets]
+        for i, b in enumerate(bufs):
+            pk[i].data = b.ctypes.data
+            pk[i].size = len(packets[i])
+        status = int(self._lib.tc_frame_decode_batch(pk, n, None, None, infos))
+        self._check(status, "tc_frame_decode_batch(query)")
+        outs, ptrs = [], (ctypes.c_void_p * (4 * n))()
+        for i in range(n):
+            inf = infos[i]
+            dims = abi.plane_dims(inf.visible_width, inf.visible_height, inf.pixel_format, inf.alpha_mode)
+            o = [np.zeros((h, w), np.uint16) for (w, h) in dims]
+            outs.append(o)
+            for p, a in enumerate(o):
+                ptrs[4 * i + p] = a.ctypes.data
+        status = int(self._lib.tc_frame_decode_batch(pk, n, ptrs, None, infos))
+        return status, outs, infos
+
+
+class DeviceBuffer:
+    """A block of HBM owned by the backend (freed on close / garbage collection)."""
+
+    def __init__(self, lib, nbytes):
+        self._lib = lib
+        self.ptr = ctypes.c_uint64(0)
+        self.nbytes = int(nbytes)
+        st = lib.tcx_dev_alloc(ctypes.byref(self.ptr), ctypes.c_size_t(self.nbytes))
+        if st != 0:
+            raise ToposCodecError(st, lib.tc_last_error().decode(errors="replace"))
+
+    @property
+    def addr(self):
+        return int(self.ptr.value)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        st = self._lib.tcx_h2d(ctypes.c_uint64(self.addr), _ptr(a), ctypes.c_size_t(a.nbytes))
+        if st != 0:
+            raise ToposCodecError(st, self._lib.tc_last_error().decode(errors="replace"))
+
+    def download(self, nbytes=None, dtype=np.uint8):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        out = np.empty(n, np.uint8)
+        st = self._lib.tcx_d2h(_ptr(out), ctypes.c_uint64(self.addr), ctypes.c_size_t(n))
+        if st != 0:
+            raise ToposCodecError(st, self._lib.tc_last_error().decode(errors="replace"))
+        return out.view(dtype)
+
+    def close(self):
+        if self.ptr.value:
+            self._lib.tcx_dev_free(ctypes.c_uint64(self.ptr.value))
+            self.ptr = ctypes.c_uint64(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceCodec(ToposCodec):
+    """Batched encode / decode with inputs and outputs resident in HBM."""
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self._lib, nbytes)
+
+    def set_timing(self, on):
+        self._lib.tcx_set_timing(1 if on else 0)
+
+    def timing(self):
+        t = (ctypes.c_float * 7)()
+        self._lib.tcx_get_timing(t)
+        return dict(zip(["fdct", "slice_enc", "scan", "emit", "slice_dec", "idct", "deblock"], list(t)))
+
+    def encode_batch_dev(self, cfgs, plane_addrs, out_bufs, strides=None):
+        """cfgs: list of FrameConfig; plane_addrs: per frame list of 3-4 device addresses."""
+        n = len(cfgs)
+        carr = (abi.FrameConfig * n)(*cfgs)
+        planes = (ctypes.c_uint64 * (4 * n))()
+        strd = (ctypes.c_size_t * (4 * n))()
+        for i, pa in enumerate(plane_addrs):
+            for p, a in enumerate(pa):
+                planes[4 * i + p] = a
+                if strides is not None:
+                    strd[4 * i + p] = strides[i][p]
+        outs = (ctypes.c_uint64 * n)(*[b.addr for b in out_bufs])
+        caps = (ctypes.c_size_t * n)(*[b.nbytes for b in out_bufs])
+        sizes = (ctypes.c_uint32 * n)()
+        rcs = (ctypes.c_int32 * n)()
+        stats = (abi.FrameStats * n)()
+        st = int(self._lib.tcx_encode_batch_dev(carr, n, planes, strd, outs, caps, sizes, rcs, stats))
+        return st, list(sizes), list(rcs), stats
+
+    def decode_batch_dev(self, host_packets, dev_packets, plane_addrs, strides=None):
+        n = len(host_packets)
+        hp = (ctypes.c_void_p * n)()
+        keep = [np.frombuffer(p, np.uint8) for p in host_packets]
+        for i, k in enumerate(keep):
+            hp[i] = k.ctypes.data
+        sizes = (ctypes.c_size_t * n)(*[len(p) for p in host_packets])
+        dp = (ctypes.c_uint64 * n)(*[b.addr for b in dev_packets])
+        planes = (ctypes.c_uint64 * (4 * n))()
+        strd = (ctypes.c_size_t * (4 * n))()
+        for i, pa in enumerate(plane_addrs):
+            for p, a in enumerate(pa):
+                planes[4 * i + p] = a
+                if strides is not None:
+                    strd[4 * i + p] = strides[i][p]
+        infos = (abi.FrameOutput * n)()
+        rcs = (ctypes.c_int32 * n)()
+        st = int(self._lib.tcx_decode_batch_dev(n, hp, sizes, dp, planes, strd, infos, rcs))
+        return st, list(rcs), infos
+
+
+# ---------------------------------------------------------------- workloads
+
+def load_synth(path=SYNTH_PATH):
+    lib = ctypes.CDLL(path)
+    for name in ("tsyn_frame422", "tsyn_frame4444", "tsyn_raw12", "tsyn_luma10", "tsyn_chroma10",
+                 "tsyn_color12", "tsyn_alpha16"):
+        getattr(lib, name).restype = ctypes.c_int
+    return lib
+
+
+def synth_frame422(syn, w, h, seed):
+    cw = (w + 1) // 2
+    y = np.zeros((h, w), np.uint16)
+    u = np.zeros((h, cw), np.uint16)
+    v = np.zeros((h, cw), np.uint16)
+    syn.tsyn_frame422(ctypes.c_uint64(seed), w, h, _ptr(y), _ptr(u), _ptr(v))
+    return [y, u, v]
+
+
+def synth_frame4444(syn, w, h, seed, alpha=True):
+    ps = [np.zeros((h, w), np.uint16) for _ in range(4 if alpha else 3)]
+    syn.tsyn_frame4444(ctypes.c_uint64(seed), w, h, _ptr(ps[0]), _ptr(ps[1]), _ptr(ps[2]),
+                       _ptr(ps[3]) if alpha else None)
+    return ps
+
+
+def synth_raw12(syn, w, h, seed):
+    pw, ph = (w + 1) // 2, (h + 1) // 2
+    ps = [np.zeros((ph, pw), np.uint16) for _ in range(4)]
+    arr = (ctypes.c_void_p * 4)(*[p.ctypes.data for p in ps])
+    syn.tsyn_raw12(ctypes.c_uint64(seed), 0, pw, ph, arr)
+    return ps

@@ -1,0 +1,70 @@
+"""Build recipe for the MI355X backend (gfx950).
+
+Produces, in-tree:
+  topos-codec_amd/libtopos_codec_amd.so   C11 host layer (gcc) + HIP kernels (hipcc)
+  topos-codec_amd/libtopos_synth.so       frozen synthetic frame generators
+Run:  python topos-codec_amd/build.py [--jobs N]
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("TOPOS_GPU_ARCH", "gfx950")
+
+HOST_C = ["tc_status.c", "tc_format.c", "tc_engine.c", "tc_api.c", "tc_gpu_api.c"]
+HIP_SRC = ["tc_runtime.hip", "tc_encode.hip", "tc_decode.hip", "tc_alpha.hip"]
+
+CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
+          "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "host")]
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "kernels")]
+
+
+def run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def build(jobs=8, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    steps = []
+    for f in HOST_C:
+        src = os.path.join(CSRC, "host", f)
+        steps.append(["gcc", *CFLAGS, "-c", src, "-o", os.path.join(BUILD, f + ".o")])
+    for f in HIP_SRC:
+        src = os.path.join(CSRC, "kernels", f)
+        steps.append([HIPCC, *HIPFLAGS, "-c", src, "-o", os.path.join(BUILD, f + ".o")])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        outs = list(ex.map(run, steps))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    objs = [os.path.join(BUILD, f + ".o") for f in HOST_C + HIP_SRC]
+    lib = os.path.join(PKG, "libtopos_codec_amd.so")
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib + ".tmp", *objs, "-lpthread"])
+    os.replace(lib + ".tmp", lib)
+    synth = os.path.join(PKG, "libtopos_synth.so")
+    run(["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-shared", "-o", synth + ".tmp",
+         os.path.join(CSRC, "synth", "synth.c"), "-lm"])
+    os.replace(synth + ".tmp", synth)
+    return lib
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=8)
+    ap.add_argument("-v", action="store_true")
+    a = ap.parse_args()
+    print(build(a.jobs, a.v))

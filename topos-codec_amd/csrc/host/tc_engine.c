@@ -1,0 +1,501 @@
+/* Device engine: grow-only HBM buffers, descriptor tables and the encode /
+ * decode launch sequences behind the public calls. One engine per process,
+ * serialised by a mutex; all device work runs on one default-priority stream.
+ *
+ * Batch model: n frames of one call share one flat block index space, one
+ * descriptor upload and one launch of every kernel, so a batch costs the same
+ * number of launches as a single frame.
+ */
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "tc_host.h"
+
+typedef struct dbuf { uint64_t p; size_t cap; } dbuf;
+
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
+static int g_state = 0;       /* 0 uninitialised, 1 ready, -1 failed */
+static int32_t g_init_rc = 0;
+static char g_init_msg[256];
+
+static dbuf g_in, g_q, g_dcm, g_scratch, g_pkt, g_desc, g_res, g_out, g_tmp;
+static uint8_t* g_hdesc = NULL;   /* pinned staging for descriptor uploads */
+static size_t g_hdesc_cap = 0;
+static uint8_t* g_hres = NULL;    /* pinned staging for result downloads */
+static size_t g_hres_cap = 0;
+
+static size_t al256(size_t v) { return (v + 255u) & ~(size_t)255u; }
+
+static int32_t dbuf_need(dbuf* b, size_t bytes)
+{
+    if (bytes <= b->cap) { return TC_OK; }
+    if (b->p) { (void)tcg_sync(); (void)tcg_free(b->p); b->p = 0; b->cap = 0; }
+    size_t cap = bytes + bytes / 4u + 4096u;
+    if (tcg_malloc(&b->p, cap) != 0) {
+        b->p = 0;
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "device allocation of %zu bytes failed: %s", cap, tcg_last_error());
+    }
+    b->cap = cap;
+    return TC_OK;
+}
+
+static int32_t host_need(uint8_t** p, size_t* cap, size_t bytes)
+{
+    if (bytes <= *cap) { return TC_OK; }
+    if (*p) { (void)tcg_sync(); (void)tcg_host_free(*p); *p = NULL; *cap = 0; }
+    size_t c = bytes + bytes / 4u + 4096u;
+    if (tcg_host_alloc((void**)p, c) != 0) {
+        *p = NULL;
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "pinned host allocation of %zu bytes failed", c);
+    }
+    *cap = c;
+    return TC_OK;
+}
+
+int32_t tc_engine_acquire(void)
+{
+    pthread_mutex_lock(&g_mu);
+    if (g_state == 1) { return TC_OK; }
+    if (g_state == 0) {
+        int32_t rc = tcg_init();
+        if (rc == 0) {
+            static double cost[4097];
+            uint32_t crc[256], x2n[32];
+            cost[0] = 0.0;
+            for (uint32_t f = 1; f <= 4096u; ++f) { cost[f] = 12.0 - tc_ctx_log2((double)f); }
+            tc_crc_tables(crc, x2n);
+            rc = tcg_set_tables(cost, crc, x2n);
+        }
+        if (rc != 0) {
+            g_state = -1;
+            g_init_rc = rc == -16 ? TC_ERR_NOT_IMPLEMENTED : TC_ERR_OUT_OF_MEMORY;
+            strncpy(g_init_msg, tcg_last_error(), sizeof(g_init_msg) - 1u);
+        } else {
+            g_state = 1;
+        }
+    }
+    if (g_state != 1) {
+        pthread_mutex_unlock(&g_mu);
+        return tc_fail(g_init_rc, "GPU backend unavailable: %s", g_init_msg);
+    }
+    return TC_OK;
+}
+
+void tc_engine_release(void) { pthread_mutex_unlock(&g_mu); }
+
+/* ------------------------------------------------------------------ encode */
+
+typedef struct enc_frame_plan {
+    tc_fhdr fh;
+    uint32_t slice_rows;
+    uint32_t nslices;
+    int32_t qp_eff[4];
+} enc_frame_plan;
+
+int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
+{
+    enc_frame_plan* plan = (enc_frame_plan*)calloc(n ? n : 1u, sizeof(enc_frame_plan));
+    if (plan == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode plan"); }
+    uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, nqctx = 0;
+    size_t in_bytes = 0, scratch_bytes = 0, pkt_bytes = 0;
+    int32_t rc = TC_OK;
+    for (uint32_t f = 0; f < n && rc == TC_OK; ++f) {
+        tc_enc_req* r = &reqs[f];
+        enc_frame_plan* P = &plan[f];
+        tc_cfg_to_fhdr(r->cfg, &P->fh);
+        (void)tc_fhdr_geometry(&P->fh);
+        P->slice_rows = tc_slice_rows_of(r->cfg);
+        uint32_t bands[4];
+        rc = tc_band_plan(&P->fh, P->slice_rows, bands, &P->nslices);
+        if (rc != TC_OK) { break; }
+        P->fh.slice_count = (uint16_t)P->nslices;
+        for (uint32_t p = 0; p < P->fh.plane_count; ++p) {
+            const tc_fhdr* fh = &P->fh;
+            nplanes++;
+            if (!tc_is_alpha(fh, p)) {
+                P->qp_eff[p] = tc_qp_eff(fh, r->cfg, p);
+                nqctx++;
+                ntiles += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+            }
+            nblocks += (uint32_t)fh->bcols[p] * fh->brows[p];
+            if (r->dev_planes == NULL) {
+                const size_t stride = ((size_t)fh->pvw[p] + 7u) & ~(size_t)7u;
+                in_bytes += al256(stride * fh->pvh[p] * 2u);
+            }
+            for (uint32_t y0 = 0; y0 < fh->brows[p]; y0 += P->slice_rows) {
+                const uint32_t h = fh->brows[p] - y0 < P->slice_rows ? fh->brows[p] - y0 : P->slice_rows;
+                const size_t cap = (size_t)h * fh->bcols[p] * TCG_SLICE_BYTES_PER_BLOCK + TCG_SLICE_BYTES_EXTRA;
+                scratch_bytes += al256(cap);
+                pkt_bytes += TC_SLICE_HDR_SIZE + cap;
+            }
+        }
+        nslices += P->nslices;
+        pkt_bytes = al256(pkt_bytes + TC_HDR_SIZE + 1024u);
+    }
+    if (rc != TC_OK) { free(plan); return rc; }
+
+    /* descriptor blob: qctx | planes | slices | frames ; results | frame_out */
+    const size_t o_q = 0;
+    const size_t o_pl = al256(o_q + (size_t)nqctx * sizeof(tcg_qctx));
+    const size_t o_sl = al256(o_pl + (size_t)nplanes * sizeof(tcg_eplane));
+    const size_t o_fr = al256(o_sl + (size_t)nslices * sizeof(tcg_eslice));
+    const size_t desc_bytes = al256(o_fr + (size_t)n * sizeof(tcg_eframe));
+    const size_t o_res = 0;
+    const size_t o_fo = al256((size_t)nslices * sizeof(tcg_eresult));
+    const size_t res_bytes = al256(o_fo + (size_t)n * sizeof(tcg_eframe_out));
+
+    if ((rc = dbuf_need(&g_q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
+        (rc = dbuf_need(&g_dcm, (size_t)nblocks * 4u + 256u)) != TC_OK ||
+        (rc = dbuf_need(&g_scratch, scratch_bytes)) != TC_OK ||
+        (rc = dbuf_need(&g_desc, desc_bytes)) != TC_OK ||
+        (rc = dbuf_need(&g_res, res_bytes)) != TC_OK ||
+        (rc = host_need(&g_hdesc, &g_hdesc_cap, desc_bytes)) != TC_OK ||
+        (rc = host_need(&g_hres, &g_hres_cap, res_bytes)) != TC_OK) {
+        free(plan);
+        return rc;
+    }
+    int need_pkt = 0;
+    for (uint32_t f = 0; f < n; ++f) { if (reqs[f].dev_out == 0u) { need_pkt = 1; } }
+    if (need_pkt && (rc = dbuf_need(&g_pkt, pkt_bytes)) != TC_OK) { free(plan); return rc; }
+    if (in_bytes && (rc = dbuf_need(&g_in, in_bytes)) != TC_OK) { free(plan); return rc; }
+
+    memset(g_hdesc, 0, desc_bytes);
+    tcg_qctx* hq = (tcg_qctx*)(g_hdesc + o_q);
+    tcg_eplane* hp = (tcg_eplane*)(g_hdesc + o_pl);
+    tcg_eslice* hs = (tcg_eslice*)(g_hdesc + o_sl);
+    tcg_eframe* hf = (tcg_eframe*)(g_hdesc + o_fr);
+    uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0;
+    size_t in_off = 0, scr_off = 0, pkt_off = 0;
+    for (uint32_t f = 0; f < n && rc == TC_OK; ++f) {
+        tc_enc_req* r = &reqs[f];
+        enc_frame_plan* P = &plan[f];
+        const tc_fhdr* fh = &P->fh;
+        const uint32_t tbl = (fh->flags & 8u) ? 1u : 0u;
+        const uint32_t s0 = is;
+        size_t frame_cap = TC_HDR_SIZE;
+        for (uint32_t p = 0; p < fh->plane_count; ++p) {
+            tcg_eplane* E = &hp[ip];
+            const uint32_t vw = fh->pvw[p], vh = fh->pvh[p];
+            if (r->dev_planes != NULL) {
+                E->src = r->dev_planes[p];
+                E->stride = (uint32_t)(r->dev_strides && r->dev_strides[p] ? r->dev_strides[p] : vw);
+            } else {
+                const size_t stride = ((size_t)vw + 7u) & ~(size_t)7u;
+                const size_t sst = r->input->strides[p] ? r->input->strides[p] : vw;
+                E->src = g_in.p + in_off;
+                E->stride = (uint32_t)stride;
+                rc = tcg_h2d_2d(E->src, stride * 2u, r->input->planes[p], sst * 2u, (size_t)vw * 2u, vh);
+                if (rc != 0) { rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "input upload: %s", tcg_last_error()); break; }
+                in_off += al256(stride * vh * 2u);
+            }
+            E->vis_w = vw; E->vis_h = vh;
+            E->bcols = fh->bcols[p]; E->brows = fh->brows[p];
+            E->blk0 = blk;
+            E->coded_w = fh->pcw[p];
+            E->tile0 = tile;
+            if (tc_is_alpha(fh, p)) {
+                E->kind = 1u;
+                E->qctx = 0u;
+                E->alpha_shift = fh->alpha_mode == 2u ? 16u - fh->alpha_bit_depth : 0u;
+            } else {
+                E->kind = 0u;
+                E->qctx = iq;
+                tc_qctx_build(&hq[iq], tc_qm_tab[fh->qmatrix_id][tc_is_chroma(fh, p) ? 1 : 0],
+                              (uint32_t)P->qp_eff[p], fh->bit_depth, tbl);
+                iq++;
+                tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+            }
+            for (uint32_t y0 = 0; y0 < fh->brows[p]; y0 += P->slice_rows) {
+                const uint32_t h = fh->brows[p] - y0 < P->slice_rows ? fh->brows[p] - y0 : P->slice_rows;
+                tcg_eslice* S = &hs[is++];
+                S->plane = ip; S->frame = f; S->y0 = y0; S->h = h;
+                S->blk0 = blk + y0 * fh->bcols[p];
+                S->nblk = h * fh->bcols[p];
+                S->scratch = scr_off;
+                S->scratch_cap = (uint32_t)(S->nblk * TCG_SLICE_BYTES_PER_BLOCK + TCG_SLICE_BYTES_EXTRA);
+                S->qpd = tc_is_alpha(fh, p) ? 64u : (uint32_t)(64 + P->qp_eff[p] - (int32_t)fh->qp_base);
+                S->hdr_plane = p;
+                scr_off += al256(S->scratch_cap);
+                frame_cap += TC_SLICE_HDR_SIZE + S->scratch_cap;
+            }
+            blk += (uint32_t)fh->bcols[p] * fh->brows[p];
+            ip++;
+        }
+        if (rc != TC_OK) { break; }
+        tcg_eframe* F = &hf[f];
+        if (r->dev_out != 0u) {
+            F->packet = r->dev_out;
+            F->cap = (uint32_t)(r->dev_out_cap < TC_MAX_PACKET ? r->dev_out_cap : TC_MAX_PACKET);
+        } else {
+            F->packet = g_pkt.p + pkt_off;
+            F->cap = (uint32_t)(frame_cap < TC_MAX_PACKET ? frame_cap : TC_MAX_PACKET);
+            pkt_off = al256(pkt_off + frame_cap + 1024u);
+        }
+        F->s0 = s0; F->s1 = is;
+        tc_fhdr tmp = *fh;
+        tmp.packet_size = 0u;
+        tc_fhdr_pack(&tmp, F->hdr);
+    }
+    if (rc != TC_OK) { free(plan); return rc; }
+    if (tcg_h2d(g_desc.p, g_hdesc, desc_bytes) != 0) {
+        free(plan);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
+    }
+    if (tcg_encode((const tcg_qctx*)(uintptr_t)(g_desc.p + o_q), nplanes,
+                   (const tcg_eplane*)(uintptr_t)(g_desc.p + o_pl), nslices,
+                   (const tcg_eslice*)(uintptr_t)(g_desc.p + o_sl), n,
+                   (const tcg_eframe*)(uintptr_t)(g_desc.p + o_fr), g_q.p, g_dcm.p, tile, g_scratch.p,
+                   (tcg_eresult*)(uintptr_t)(g_res.p + o_res),
+                   (tcg_eframe_out*)(uintptr_t)(g_res.p + o_fo)) != 0) {
+        free(plan);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode launch failed: %s", tcg_last_error());
+    }
+    if (tcg_d2h(g_hres + o_fo, g_res.p + o_fo, (size_t)n * sizeof(tcg_eframe_out)) != 0) {
+        free(plan);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode result download: %s", tcg_last_error());
+    }
+    const tcg_eframe_out* fo = (const tcg_eframe_out*)(g_hres + o_fo);
+    int32_t first = TC_OK;
+    for (uint32_t f = 0; f < n; ++f) {
+        tc_enc_req* r = &reqs[f];
+        const tcg_eframe_out* o = &fo[f];
+        r->packet_size = o->size;
+        if (o->status != 0) {
+            r->rc = o->status == -1 ? tc_fail(TC_ERR_INVALID_ARGUMENT,
+                                              "token domain violated (bit_depth %u qp too low for full-scale content)",
+                                              (unsigned)plan[f].fh.bit_depth)
+                                    : tc_fail(o->status, "encode failed on device (status %d)", (int)o->status);
+        } else {
+            if (r->stats) {
+                topos_frame_stats* st = r->stats;
+                st->packet_size = o->size;
+                st->color_payload_bytes = o->color_payload;
+                st->alpha_payload_bytes = o->alpha_payload;
+                st->color_header_bytes = TC_HDR_SIZE + TC_SLICE_HDR_SIZE * o->color_slices;
+                st->alpha_header_bytes = TC_SLICE_HDR_SIZE * o->alpha_slices;
+                st->slice_count = (uint16_t)plan[f].nslices;
+                st->qp_base = plan[f].fh.qp_base;
+                st->alpha_max_abs_error = 0u;
+            }
+            r->rc = TC_OK;
+            if (r->out != NULL || r->dev_out == 0u) {
+                if (o->size > r->out_cap) {
+                    r->rc = tc_fail(TC_ERR_BUFFER_TOO_SMALL, "out_cap %zu < %u bytes needed", r->out_cap,
+                                    (unsigned)o->size);
+                } else if (r->out != NULL) {
+                    if (tcg_d2h(r->out, hf[f].packet, o->size) != 0) {
+                        r->rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "packet download: %s", tcg_last_error());
+                    }
+                }
+            }
+        }
+        if (first == TC_OK && r->rc != TC_OK) { first = r->rc; }
+    }
+    free(plan);
+    return first;
+}
+
+/* ------------------------------------------------------------------ decode */
+
+int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
+{
+    uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, nrows = 0, ndb = 0;
+    size_t pkt_bytes = 0, out_bytes = 0, tmp_bytes = 0;
+    for (uint32_t f = 0; f < n; ++f) {
+        const tc_fhdr* fh = &reqs[f].view->fh;
+        nslices += reqs[f].view->nslices;
+        if (reqs[f].dev_data == 0u) { pkt_bytes += al256(reqs[f].size + 1024u); }
+        const int db = (fh->flags & 4u) != 0u;
+        for (uint32_t p = 0; p < fh->plane_count; ++p) {
+            nplanes++;
+            nblocks += (uint32_t)fh->bcols[p] * fh->brows[p];
+            nrows += fh->brows[p];
+            if (!tc_is_alpha(fh, p)) { ntiles += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u); }
+            const size_t pb = al256((size_t)fh->pvw[p] * fh->pvh[p] * 2u);
+            if (reqs[f].dev_planes == NULL) { out_bytes += pb; }
+            if (db && p < 3u) { tmp_bytes += pb; }
+        }
+    }
+    const size_t o_q = 0;
+    const size_t o_pl = al256(o_q + (size_t)nslices * sizeof(tcg_qctx));
+    const size_t o_sl = al256(o_pl + (size_t)nplanes * sizeof(tcg_dplane));
+    const size_t o_rs = al256(o_sl + (size_t)nslices * sizeof(tcg_dslice));
+    const size_t o_db = al256(o_rs + (size_t)nrows * sizeof(uint16_t));
+    const size_t desc_bytes = al256(o_db + (size_t)(nplanes + 1u) * sizeof(uint32_t));
+    const size_t res_bytes = al256((size_t)nslices * sizeof(int32_t));
+    int32_t rc;
+    if ((rc = dbuf_need(&g_q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
+        (rc = dbuf_need(&g_desc, desc_bytes)) != TC_OK ||
+        (rc = dbuf_need(&g_res, res_bytes)) != TC_OK ||
+        (rc = host_need(&g_hdesc, &g_hdesc_cap, desc_bytes)) != TC_OK ||
+        (rc = host_need(&g_hres, &g_hres_cap, res_bytes)) != TC_OK) {
+        return rc;
+    }
+    if (pkt_bytes && (rc = dbuf_need(&g_pkt, pkt_bytes)) != TC_OK) { return rc; }
+    if (out_bytes && (rc = dbuf_need(&g_out, out_bytes)) != TC_OK) { return rc; }
+    if (tmp_bytes && (rc = dbuf_need(&g_tmp, tmp_bytes)) != TC_OK) { return rc; }
+
+    memset(g_hdesc, 0, desc_bytes);
+    tcg_qctx* hq = (tcg_qctx*)(g_hdesc + o_q);
+    tcg_dplane* hp = (tcg_dplane*)(g_hdesc + o_pl);
+    tcg_dslice* hs = (tcg_dslice*)(g_hdesc + o_sl);
+    uint16_t* hrs = (uint16_t*)(g_hdesc + o_rs);
+    uint32_t* hdb = (uint32_t*)(g_hdesc + o_db);
+    uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0;
+    size_t pkt_off = 0, out_off = 0, tmp_off = 0;
+    uint64_t* out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
+    if (out_dev == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode plan"); }
+    for (uint32_t f = 0; f < n; ++f) {
+        tc_dec_req* r = &reqs[f];
+        const tc_view* v = r->view;
+        const tc_fhdr* fh = &v->fh;
+        uint64_t pk = r->dev_data;
+        if (pk == 0u) {
+            pk = g_pkt.p + pkt_off;
+            if (tcg_h2d(pk, r->data, r->size) != 0) {
+                free(out_dev);
+                return tc_fail(TC_ERR_OUT_OF_MEMORY, "packet upload: %s", tcg_last_error());
+            }
+            pkt_off += al256(r->size + 1024u);
+        }
+        const uint32_t tbl = (fh->flags & 8u) ? 1u : 0u;
+        const int db = (fh->flags & 4u) != 0u;
+        const uint32_t ip0 = ip;
+        for (uint32_t p = 0; p < fh->plane_count; ++p) {
+            tcg_dplane* D = &hp[ip + p];
+            const uint32_t vw = fh->pvw[p], vh = fh->pvh[p];
+            if (r->dev_planes != NULL) {
+                D->dst = r->dev_planes[p];
+                D->stride = (uint32_t)(r->dev_strides && r->dev_strides[p] ? r->dev_strides[p] : vw);
+            } else {
+                D->dst = g_out.p + out_off;
+                D->stride = vw;
+                out_off += al256((size_t)vw * vh * 2u);
+            }
+            out_dev[(size_t)f * 4u + p] = D->dst;
+            D->vis_w = vw; D->vis_h = vh;
+            D->bcols = fh->bcols[p]; D->brows = fh->brows[p];
+            D->blk0 = blk;
+            D->row_slice = row;
+            D->coded_w = fh->pcw[p];
+            D->tile0 = tile;
+            if (tc_is_alpha(fh, p)) {
+                D->kind = 1u;
+                D->fill = 0xFFFFu;
+                D->maxv = 0xFFFFu;
+                D->alpha_mask = fh->alpha_mode == 2u ? (1u << (16u - fh->alpha_bit_depth)) - 1u : 0u;
+            } else {
+                D->kind = 0u;
+                D->fill = 1u << (fh->bit_depth - 1u);
+                D->maxv = (1u << fh->bit_depth) - 1u;
+                tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                if (db && p < 3u) {
+                    D->tmp = g_tmp.p + tmp_off;
+                    tmp_off += al256((size_t)vw * vh * 2u);
+                    if (vw % 8u == 0u && vh % 8u == 0u && vw >= 16u && vh >= 16u && fh->qp_base <= 95u &&
+                        fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
+                        D->deblock = 1u;
+                        D->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
+                    } else {
+                        D->tmp = 0u;
+                    }
+                }
+            }
+            blk += (uint32_t)fh->bcols[p] * fh->brows[p];
+            row += fh->brows[p];
+        }
+        /* slices; quantiser contexts deduplicated per (plane class, qp) within the frame */
+        int32_t qkey[16];
+        uint32_t qidx[16], nk = 0;
+        for (uint32_t i = 0; i < v->nslices; ++i) {
+            const tc_shdr* h = &v->sl[i];
+            tcg_dslice* S = &hs[is];
+            const uint32_t p = h->plane;
+            const tcg_dplane* D = &hp[ip0 + p];
+            S->payload = pk + v->off[i];
+            S->size = h->size;
+            S->crc = h->crc;
+            S->plane = ip0 + p;
+            S->y0 = h->y0; S->h = h->h;
+            S->blk0 = D->blk0 + (uint32_t)h->y0 * fh->bcols[p];
+            S->k1 = h->k1; S->k2 = h->k2;
+            S->dc_max = fh->bit_depth > 12u ? 0x0FFFFFFFu : 0x08000000u;
+            S->lv_max = fh->bit_depth > 12u ? 0x07FFFFFFu : 0x04000000u;
+            if (tc_is_alpha(fh, p)) {
+                S->kind = 1u;
+                S->qctx = 0u;
+            } else {
+                S->kind = 0u;
+                const int32_t qp = (int32_t)fh->qp_base + (int32_t)h->qpd - 64;
+                const int32_t key = qp * 2 + (tc_is_chroma(fh, p) ? 1 : 0);
+                uint32_t k = 0;
+                while (k < nk && qkey[k] != key) { ++k; }
+                if (k == nk) {
+                    if (nk < 16u) { qkey[nk] = key; qidx[nk] = iq; nk++; }
+                    tc_qctx_build(&hq[iq], tc_qm_tab[fh->qmatrix_id][tc_is_chroma(fh, p) ? 1 : 0], (uint32_t)qp,
+                                  fh->bit_depth, tbl);
+                    S->qctx = iq++;
+                } else {
+                    S->qctx = qidx[k];
+                }
+            }
+            for (uint32_t y = 0; y < h->h; ++y) { hrs[D->row_slice + h->y0 + y] = (uint16_t)is; }
+            is++;
+        }
+        ip += fh->plane_count;
+    }
+    /* deblock tile table: strips x 256-column tiles per filtered plane */
+    uint32_t dbt = 0;
+    for (uint32_t i = 0; i < ip; ++i) {
+        hdb[i] = dbt;
+        if (hp[i].deblock) { dbt += (hp[i].vis_h / 8u + 1u) * ((hp[i].vis_w + 255u) / 256u); }
+    }
+    hdb[ip] = dbt;
+    (void)ndb;
+    if (tcg_h2d(g_desc.p, g_hdesc, desc_bytes) != 0) {
+        free(out_dev);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
+    }
+    if (tcg_decode((const tcg_qctx*)(uintptr_t)(g_desc.p + o_q), nplanes,
+                   (const tcg_dplane*)(uintptr_t)(g_desc.p + o_pl), nslices,
+                   (const tcg_dslice*)(uintptr_t)(g_desc.p + o_sl),
+                   (const uint16_t*)(uintptr_t)(g_desc.p + o_rs), g_q.p, tile,
+                   (int32_t*)(uintptr_t)g_res.p, dbt, (const uint32_t*)(uintptr_t)(g_desc.p + o_db)) != 0) {
+        free(out_dev);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode launch failed: %s", tcg_last_error());
+    }
+    if (tcg_d2h(g_hres, g_res.p, (size_t)nslices * sizeof(int32_t)) != 0) {
+        free(out_dev);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "status download: %s", tcg_last_error());
+    }
+    const int32_t* st = (const int32_t*)g_hres;
+    int32_t first = TC_OK;
+    uint32_t sbase = 0;
+    for (uint32_t f = 0; f < n; ++f) {
+        tc_dec_req* r = &reqs[f];
+        const tc_view* v = r->view;
+        uint32_t conc = 0;
+        for (uint32_t i = 0; i < v->nslices; ++i) {
+            const uint8_t s = st[sbase + i] != 0 ? TC_FRAME_SLICE_CONCEALED : TC_FRAME_SLICE_OK;
+            if (r->info) { r->info->slice_status[i] = s; }
+            conc += s;
+        }
+        sbase += v->nslices;
+        if (r->info) { r->info->concealed_slices = (uint16_t)conc; }
+        r->rc = conc ? tc_fail(TC_WARN_CONCEALED, "%u of %u slices concealed", conc, (unsigned)v->nslices) : TC_OK;
+        if (r->dev_planes == NULL && r->planes != NULL) {
+            for (uint32_t p = 0; p < v->fh.plane_count; ++p) {
+                const size_t vw = v->fh.pvw[p], vh = v->fh.pvh[p];
+                const size_t dst_stride = r->strides && r->strides[p] ? r->strides[p] : vw;
+                if (tcg_d2h_2d(r->planes[p], dst_stride * 2u, out_dev[(size_t)f * 4u + p], vw * 2u, vw * 2u, vh) != 0) {
+                    r->rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "plane download: %s", tcg_last_error());
+                    break;
+                }
+            }
+        }
+        if (first == TC_OK && r->rc != TC_OK) { first = r->rc; }
+    }
+    free(out_dev);
+    return first;
+}

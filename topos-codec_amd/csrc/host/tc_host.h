@@ -1,0 +1,120 @@
+/* Internal declarations of the C11 host layer (framing rules, error state,
+ * device engine). Not installed; the public surface is include/topos_codec.h.
+ */
+#ifndef TC_HOST_H
+#define TC_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "topos_codec.h"
+#include "../kernels/tcg.h"
+
+#define TC_HDR_SIZE 53u
+#define TC_SLICE_HDR_SIZE 17u
+#define TC_MAX_DIM 16384u
+#define TC_MAX_SLICE_COUNT 512u
+#define TC_MAX_PACKET 268435456u
+#define TC_QP_MAX 95u
+#define TC_TRANSFER_TRAW_LOG 20u
+
+/* ---- error state (tc_status.c) ---- */
+int32_t tc_fail(int32_t code, const char* fmt, ...);
+void tc_clear_error(void);
+
+/* ---- frame header model (tc_format.c) ---- */
+typedef struct tc_fhdr {
+    uint8_t major, minor;
+    uint16_t flags;
+    uint8_t profile, pixel_format, bit_depth, alpha_mode, alpha_bit_depth;
+    uint8_t frame_type;
+    uint16_t gop_id;
+    uint8_t ref_distance;
+    uint16_t coded_w, coded_h, visible_w, visible_h;
+    uint8_t plane_count, qmatrix_id, qp_base;
+    uint16_t slice_count;
+    uint8_t color_range, color_primaries, color_transfer, color_matrix, chroma_siting;
+    uint16_t sar_num, sar_den;
+    uint32_t packet_size;
+    uint8_t entropy_mode, codebook_version, coding_mode;
+    /* derived per-plane geometry */
+    uint16_t pcw[4], pch[4], pvw[4], pvh[4], bcols[4], brows[4];
+} tc_fhdr;
+
+typedef struct tc_shdr {
+    uint32_t size;
+    uint8_t plane;
+    uint16_t y0, h;
+    uint8_t qpd, k1, k2, k3;
+    uint32_t crc;
+} tc_shdr;
+
+typedef struct tc_view {
+    tc_fhdr fh;
+    uint16_t nslices;
+    tc_shdr sl[TC_MAX_SLICE_COUNT];
+    uint32_t off[TC_MAX_SLICE_COUNT]; /* payload offset inside the packet */
+} tc_view;
+
+extern const uint32_t tc_qp_scale_tab[2][96];
+extern const uint16_t tc_qm_tab[5][2][64];
+extern const uint8_t tc_zigzag_inv[64];
+
+uint32_t tc_crc32(const uint8_t* p, size_t n);
+void tc_crc_tables(uint32_t crc256[256], uint32_t x2n32[32]);
+double tc_ctx_log2(double x);
+
+int32_t tc_cfg_validate(const topos_frame_config* cfg);
+void tc_cfg_to_fhdr(const topos_frame_config* cfg, tc_fhdr* fh);
+int32_t tc_fhdr_geometry(tc_fhdr* fh);
+int32_t tc_fhdr_validate(const tc_fhdr* fh);
+void tc_fhdr_pack(const tc_fhdr* fh, uint8_t out[TC_HDR_SIZE]);
+int32_t tc_fhdr_parse(const uint8_t* data, size_t size, tc_fhdr* fh);
+int32_t tc_band_plan(const tc_fhdr* fh, uint32_t slice_rows, uint32_t bands[4], uint32_t* total);
+int32_t tc_parse_structure(const uint8_t* data, size_t size, tc_view* v);
+uint32_t tc_slice_rows_of(const topos_frame_config* cfg);
+int32_t tc_qp_eff(const tc_fhdr* fh, const topos_frame_config* cfg, uint32_t plane);
+int tc_is_chroma(const tc_fhdr* fh, uint32_t p);
+int tc_is_alpha(const tc_fhdr* fh, uint32_t p);
+void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bit_depth, uint32_t tbl);
+uint32_t tc_deblock_alpha(uint32_t tbl, uint32_t qp);
+void tc_fill_output_info(const tc_fhdr* fh, topos_frame_output* info);
+
+/* ---- device engine (tc_engine.c) ---- */
+int32_t tc_engine_acquire(void);   /* lazily initialises the GPU; locks the engine */
+void tc_engine_release(void);
+
+/* encode n frames (all with the same cfg-derived header but separate inputs)
+ * into device packets; host_out[i] receives frame i's packet when non-NULL */
+typedef struct tc_enc_req {
+    const topos_frame_config* cfg;
+    const topos_frame_input* input;   /* host planes (API path) */
+    const uint64_t* dev_planes;       /* device planes (resident path), or NULL */
+    const size_t* dev_strides;        /* elements */
+    uint8_t* out;                     /* host output (may be NULL) */
+    size_t out_cap;
+    uint64_t dev_out;                 /* device output packet (resident path) */
+    size_t dev_out_cap;
+    topos_frame_stats* stats;
+    int32_t rc;
+    uint32_t packet_size;
+} tc_enc_req;
+
+int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n);
+
+typedef struct tc_dec_req {
+    const uint8_t* data;              /* host packet (API path) */
+    size_t size;
+    uint64_t dev_data;                /* device packet (resident path), or 0 */
+    uint16_t* const* planes;          /* host planes (4), or NULL */
+    const size_t* strides;            /* host strides (4), or NULL */
+    const uint64_t* dev_planes;       /* device planes (resident path), or NULL */
+    const size_t* dev_strides;
+    topos_frame_output* info;
+    const tc_view* view;              /* parsed structure */
+    int32_t rc;
+} tc_dec_req;
+
+int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n);
+
+#endif

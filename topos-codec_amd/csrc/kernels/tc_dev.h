@@ -1,0 +1,98 @@
+// Device-side helpers shared by the gfx950 kernel translation units.
+#ifndef TC_DEV_H
+#define TC_DEV_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tcg.h"
+#include "tc_math.h"
+
+// Frozen transform matrix M[u][x] and weights W[u][v] = round(2^32/(E_u E_v)),
+// and the zigzag scan (native/src/transform/transform_tables.h:10-36,
+// native/src/entropy/scan.h:10-32): bitstream constants.
+static __device__ const int8_t kTM[64] = {
+    13, 13, 13, 13, 13, 13, 13, 13,   13, 11, 7, 1, -1, -7, -11, -13,
+    12, 5, -5, -12, -12, -5, 5, 12,   11, -4, -13, -8, 8, 13, 4, -11,
+    9, -9, -9, 9, 9, -9, -9, 9,       8, -13, 4, 11, -11, -4, 13, -8,
+    5, -12, 12, -5, -5, 12, -12, 5,   1, -7, 11, -13, 13, -11, 7, -1,
+};
+static __device__ const uint32_t kTE[8] = { 1352, 680, 676, 740, 648, 740, 676, 680 };
+static __device__ const uint32_t kTW[64] = {
+    2350, 4672, 4699, 4293, 4902, 4293, 4699, 4672,  4672, 9288, 9343, 8535, 9747, 8535, 9343, 9288,
+    4699, 9343, 9399, 8586, 9805, 8586, 9399, 9343,  4293, 8535, 8586, 7843, 8957, 7843, 8586, 8535,
+    4902, 9747, 9805, 8957, 10228, 8957, 9805, 9747, 4293, 8535, 8586, 7843, 8957, 7843, 8586, 8535,
+    4699, 9343, 9399, 8586, 9805, 8586, 9399, 9343,  4672, 9288, 9343, 8535, 9747, 8535, 9343, 9288,
+};
+static __device__ const uint8_t kZigzag[64] = {
+    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5,
+    12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21, 28,
+    35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
+    58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63,
+};
+static __device__ const uint8_t kZigzagInv[64] = {
+    0, 1, 5, 6, 14, 15, 27, 28, 2, 4, 7, 13, 16, 26, 29, 42,
+    3, 8, 12, 17, 25, 30, 41, 43, 9, 11, 18, 24, 31, 40, 44, 53,
+    10, 19, 23, 32, 39, 45, 52, 54, 20, 22, 33, 38, 46, 51, 55, 60,
+    21, 34, 37, 47, 50, 56, 59, 61, 35, 36, 48, 49, 57, 58, 62, 63,
+};
+
+// Run-time tables uploaded once by tcg_set_tables.
+struct TcgConsts {
+    double cost[4097];      // 12 - log2(f), the model-choice cost per symbol
+    uint32_t crc[256];      // bytewise CRC32 table (reflected 0xEDB88320)
+    uint32_t x2n[32];       // x^(2^k) mod P
+};
+
+// host-side runtime services (tc_runtime.hip)
+hipStream_t tcg_stream();
+const TcgConsts* tcg_consts_dev();
+int32_t tcg_check(const char* what);
+int tcg_wide_mode();
+
+struct TcgTimer {
+    void mark(hipStream_t s, int i);
+    void finish_encode();
+    void finish_decode();
+};
+
+// f64 cost accumulation in the reference's order with no contraction: the
+// model choice compares these sums with strict '<', so every rounding must
+// match the unfused x86-64 reference build.
+__device__ static inline double cost_add(const TcgConsts* K, double bits, uint32_t cnt, uint32_t f)
+{
+#pragma clang fp contract(off)
+    const double term = (double)cnt * K->cost[f];
+    return bits + term;
+}
+__device__ static inline double cost_add_const(double bits, double c)
+{
+#pragma clang fp contract(off)
+    return bits + c;
+}
+
+__device__ static inline uint32_t crc32_bytes_tab(const uint32_t* tab, const uint8_t* p, uint32_t n)
+{
+    uint32_t c = 0xFFFFFFFFu;
+    for (uint32_t i = 0; i < n; ++i) { c = tab[(c ^ p[i]) & 0xFFu] ^ (c >> 8); }
+    return c ^ 0xFFFFFFFFu;
+}
+
+// bit depth > 12 encoder pre-distortion F' = round(F * 2^32 / (W E_u E_v))
+// (native/src/transform/transform.c:125-150)
+__device__ static inline int32_t predistort(int32_t F, uint32_t nat)
+{
+    const int64_t w = (int64_t)kTW[nat] * (int64_t)kTE[nat >> 3] * (int64_t)kTE[nat & 7u];
+    const int64_t num = (int64_t)F * ((int64_t)1 << 32);
+    if (num >= 0) { return (int32_t)((2 * num + w) / (2 * w)); }
+    return -(int32_t)((-2 * num + w) / (2 * w));
+}
+
+// alpha plane kernels (tc_alpha.hip)
+void tcg_alpha_encode_launch(hipStream_t st, const tcg_eplane* planes, const tcg_eslice* slices,
+                             uint32_t nslices, const int32_t* unused, uint8_t* scratch,
+                             tcg_eresult* results);
+void tcg_alpha_decode_launch(hipStream_t st, const tcg_dplane* planes, const tcg_dslice* slices,
+                             uint32_t nslices, int32_t* status);
+
+#endif

@@ -1,0 +1,615 @@
+// gfx950 encode kernels: fused level-shift + 8x8 integer DCT + dead-zone
+// quantisation, per-slice rANS2 ("V7-R2") entropy coding with one wavefront
+// per slice, packet offset scan, and payload copy + CRC32 + slice headers.
+//
+// Reference semantics (paths relative to the reference tree):
+//   transform   native/src/transform/transform.c:18-41, simd/dispatch.c:402-413
+//   quantiser   native/src/transform/quant.c:310-328
+//   tokens      native/src/codec/codec.c:1702-1848
+//   rANS2       native/src/codec/codec.c:2122-2515, native/src/entropy/rans.c
+//   framing     native/src/bitstream/frame_header.c:590-631, slice_map.c:9-21
+#include <hip/hip_runtime.h>
+
+#include "tcg.h"
+#include "tc_math.h"
+#include "tc_dev.h"
+
+// ---------------------------------------------------------------------------
+// K1: fused FDCT + quantisation.
+// One workgroup = 256 threads = one tile of 32 blocks along a block row.
+// Pass 1 (rows): thread (r, j) loads row r of block j (8 u16 = 16 B, the 32
+// threads of a row read 512 contiguous bytes), computes A[r][v].
+// Pass 2 (columns) through LDS: thread (v, j) computes F[u][v], u = 0..7, and
+// quantises; the 32 x 64 coefficients are staged in LDS in scan (zigzag) order
+// and written out as one contiguous 8 KB span.
+// ---------------------------------------------------------------------------
+#define FQ_BLOCKS 32
+
+__device__ static inline uint32_t find_plane(const tcg_eplane* planes, uint32_t nplanes, uint32_t tile)
+{
+    // planes[i].tile0 holds the first tile index of plane i (ascending)
+    uint32_t lo = 0, hi = nplanes - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (planes[mid].tile0 <= tile) { lo = mid; } else { hi = mid - 1; }
+    }
+    return lo;
+}
+
+template <bool WIDE>
+__device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const tcg_eplane& P,
+                                       uint32_t brow, uint32_t bx0, uint32_t nb,
+                                       int32_t* __restrict__ qbuf, int32_t* sA, uint32_t* sQ,
+                                       uint32_t* sHalf, uint32_t* sDz, uint64_t* sMag)
+{
+    const tcg_qctx* qc = &qctxs[P.qctx];
+    const uint32_t t = threadIdx.x;
+    if (t < 64u) {
+        sQ[t] = qc->Q[t];
+        sHalf[t] = qc->half[t];
+        sDz[t] = qc->dz[t];
+        sMag[t] = ((uint64_t)qc->mhi[t] << 32) | qc->mlo[t];
+    }
+    const int32_t mid = qc->mid;
+    const int32_t fclamp = qc->fclamp;
+
+    // ---- pass 1: rows ----
+    {
+        const uint32_t r = t >> 5, j = t & 31u;
+        if (j < nb) {
+            const uint16_t* src = (const uint16_t*)P.src;
+            uint32_t y = brow * 8u + r;
+            if (y >= P.vis_h) { y = P.vis_h - 1u; }
+            const uint32_t x0 = (bx0 + j) * 8u;
+            const uint16_t* row = src + (size_t)y * P.stride;
+            int32_t x[8];
+            if (x0 + 8u <= P.vis_w && ((((uintptr_t)(row + x0)) & 15u) == 0u)) {
+                const uint4 v = *(const uint4*)(row + x0);
+                const uint32_t w[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    x[2 * k] = (int32_t)(w[k] & 0xFFFFu) - mid;
+                    x[2 * k + 1] = (int32_t)(w[k] >> 16) - mid;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    uint32_t xx = x0 + (uint32_t)k;
+                    if (xx >= P.vis_w) { xx = P.vis_w - 1u; }
+                    x[k] = (int32_t)row[xx] - mid;
+                }
+            }
+            if (!WIDE) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) { x[k] = (int32_t)(int16_t)x[k]; }
+            }
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                int32_t acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) { acc += __mul24(x[k], (int32_t)kTM[v * 8 + k]); }
+                sA[j * 64 + r * 8 + v] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- pass 2: columns + quantisation ----
+    int32_t qv[8];
+    const uint32_t v = t >> 5, j = t & 31u;
+    if (j < nb) {
+        int32_t a[8];
+#pragma unroll
+        for (int y = 0; y < 8; ++y) { a[y] = sA[j * 64 + y * 8 + v]; }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int32_t F = 0;
+#pragma unroll
+            for (int y = 0; y < 8; ++y) { F += __mul24((int32_t)kTM[u * 8 + y], a[y]); }
+            const uint32_t nat = (uint32_t)u * 8u + v;
+            if (WIDE) { F = predistort(F, nat); }
+            int32_t c = F > fclamp ? fclamp : (F < -fclamp ? -fclamp : F);
+            const uint32_t mag = (uint32_t)(c < 0 ? -c : c);
+            const uint32_t num = mag + sHalf[nat];
+            const uint32_t ne = num > sDz[nat] ? num - sDz[nat] : 0u;
+            uint32_t qm;
+            if (WIDE) {
+                qm = ne / sQ[nat];
+            } else {
+                qm = tc_qdiv_apply(ne, sMag[nat]);
+            }
+            qv[u] = c < 0 ? -(int32_t)qm : (int32_t)qm;
+        }
+    }
+    __syncthreads();
+    if (j < nb) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { sA[j * 64 + kZigzagInv[u * 8 + v]] = qv[u]; }
+    }
+    __syncthreads();
+    // contiguous store of nb * 64 coefficients (scan order per block)
+    int32_t* dst = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0) * 64u;
+    const uint32_t total = nb * 64u;
+    for (uint32_t i = t * 4u; i < total; i += 256u * 4u) {
+        *(int4*)(dst + i) = *(const int4*)(sA + i);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fdct_quant(const tcg_qctx* __restrict__ qctxs,
+                                                     const tcg_eplane* __restrict__ planes,
+                                                     uint32_t nplanes, int32_t* __restrict__ qbuf)
+{
+    __shared__ int32_t sA[FQ_BLOCKS * 64];
+    __shared__ uint32_t sQ[64], sHalf[64], sDz[64];
+    __shared__ uint64_t sMag[64];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t pi = find_plane(planes, nplanes, tile);
+    const tcg_eplane P = planes[pi];
+    if (P.kind != 0u) { return; }
+    const uint32_t tiles_per_row = (P.bcols + FQ_BLOCKS - 1u) / FQ_BLOCKS;
+    const uint32_t local = tile - P.tile0;
+    const uint32_t brow = local / tiles_per_row;
+    const uint32_t bx0 = (local % tiles_per_row) * FQ_BLOCKS;
+    const uint32_t nb = min((uint32_t)FQ_BLOCKS, P.bcols - bx0);
+    if (qctxs[P.qctx].bit_depth > 12u) {
+        fdct_quant_tile<true>(qctxs, P, brow, bx0, nb, qbuf, sA, sQ, sHalf, sDz, sMag);
+    } else {
+        fdct_quant_tile<false>(qctxs, P, brow, bx0, nb, qbuf, sA, sQ, sHalf, sDz, sMag);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2: per-slice rANS2 encoder, one 64-lane wavefront per slice.
+// Lane i holds scan position i of the current block. Forward pass: DC
+// prediction, tokens and the five joint histograms (LDS atomics). Model
+// choice: lanes 0..16 build one candidate row each; five lanes sum the exact
+// f64 costs in the reference's order. Backward pass: symbols are pushed in
+// reverse decode order with the wave-uniform state; lanes precompute each
+// pair's table entries so the serial chain only does readlane + state math.
+// ---------------------------------------------------------------------------
+
+// LDS histogram layout (u32)
+#define H_DC 0          // 29
+#define H_RUN 29        // 64
+#define H_LVL 93        // 28
+#define H_LPOS 121      // 4 x 28
+#define H_LPREV 233     // 5 x 28
+#define H_DPREV 373     // 5 x 29
+#define H_TOTAL 518
+
+// candidate rows: 0 lvl-none, 1..4 lvl-pos, 5..9 lvl-prev, 10 dc-none,
+// 11..15 dc-prev, 16 run
+#define NROWS 17
+
+struct EncEmit {
+    uint8_t* end;     // one past the last byte of the slice scratch region
+    uint32_t n;       // bytes emitted so far
+    uint32_t acc;     // up to 3 pending bytes (most recent in the low byte)
+    uint32_t limit;   // emission cap (bytes)
+    uint32_t over;
+};
+
+__device__ static inline void emit_byte(EncEmit& E, uint32_t b)
+{
+    E.acc = (E.acc << 8) | (b & 0xFFu);
+    E.n += 1u;
+    if ((E.n & 3u) == 0u) {
+        if (E.n <= E.limit) {
+            if (__lane_id() == 0u) { *(uint32_t*)(E.end - E.n) = E.acc; }
+        } else {
+            E.over = 1u;
+        }
+        E.acc = 0u;
+    }
+}
+
+__device__ static inline uint32_t put_sym(uint32_t x, uint32_t fc, uint32_t mag, uint32_t sh, EncEmit& E)
+{
+    const uint32_t f = fc >> 16, cum = fc & 0xFFFFu;
+    const uint32_t xmax = f << 19;
+    while (x >= xmax) { emit_byte(E, x); x >>= 8; }
+    const uint32_t q = f == 1u ? x : (__umulhi(x, mag) >> sh);
+    return (q << 12) + (x - q * f) + cum;
+}
+
+__device__ static inline uint32_t put_chunk(uint32_t x, uint32_t w, uint32_t c, EncEmit& E)
+{
+    const uint32_t thr = 1u << (31u - w);
+    while (x >= thr) { emit_byte(E, x); x >>= 8; }
+    return (x << w) | (c & ((1u << w) - 1u));
+}
+
+__device__ static inline uint32_t put_suffix(uint32_t x, uint32_t cat, uint32_t m, EncEmit& E)
+{
+    uint32_t b = cat > 1u ? cat - 1u : 0u;
+    if (b == 0u || b > 27u) { return x; }
+    if (b > 23u) {
+        x = put_chunk(x, b - 16u, m >> 16, E);
+        b = 16u;
+    }
+    return put_chunk(x, b, m, E);
+}
+
+__device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict__ K,
+                                                      const tcg_eplane* __restrict__ planes,
+                                                      const tcg_eslice* __restrict__ slices,
+                                                      const int32_t* __restrict__ qbuf,
+                                                      uint32_t* __restrict__ dcmbuf,
+                                                      uint8_t* __restrict__ scratch,
+                                                      tcg_eresult* __restrict__ results)
+{
+    __shared__ uint32_t sH[H_TOTAL];
+    __shared__ int32_t sDcRow[2048];
+    __shared__ uint8_t sRowB[NROWS][64];
+    __shared__ uint16_t sFreq[NROWS][64];
+    __shared__ uint16_t sCum[NROWS][65];
+    __shared__ double sCost[5];
+    __shared__ uint32_t sTabFC[11 * 64];   // packed freq << 16 | cum: dc rows, run, lvl rows
+    __shared__ uint32_t sTabMag[11 * 64];
+    __shared__ uint8_t sTabSh[11 * 64];
+    __shared__ uint8_t sPrefix[TCG_PREFIX_MAX];
+
+    const uint32_t lane = threadIdx.x;
+    const tcg_eslice S = slices[blockIdx.x];
+    const tcg_eplane P = planes[S.plane];
+    if (P.kind != 0u) { return; }
+    const uint32_t cols = P.bcols;
+
+    for (uint32_t i = lane; i < H_TOTAL; i += 64u) { sH[i] = 0u; }
+    __syncthreads();
+
+    // ---------------- forward pass ----------------
+    uint32_t viol = 0u;
+    uint32_t prev_dcc = 0u;
+    int32_t left = 0;
+    const uint64_t below = lane == 0u ? 0ull : ((1ull << lane) - 1ull);
+    for (uint32_t by = 0; by < S.h; ++by) {
+        for (uint32_t bx = 0; bx < cols; ++bx) {
+            const uint32_t b = by * cols + bx;
+            const uint32_t blk = S.blk0 + b;
+            const int32_t q = qbuf[(size_t)blk * 64u + lane];
+            const uint64_t nz = __ballot(q != 0) & ~1ull;
+            const int32_t q0 = __builtin_amdgcn_readlane(q, 0);
+            const int32_t top = by ? sDcRow[bx] : 0;
+            const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
+            uint32_t dcm = tc_map_signed(q0 - pred);
+            if (dcm > 0x0FFFFFFFu) { dcm = 0x0FFFFFFFu; viol = 1u; }
+            const uint32_t dcc = tc_bitlen(dcm);
+            if (lane == 0u) {
+                dcmbuf[blk] = dcm;
+                atomicAdd(&sH[H_DC + dcc], 1u);
+                atomicAdd(&sH[H_DPREV + (b == 0u ? 0u : tc_ctx_dc(prev_dcc, 1u)) * 29u + dcc], 1u);
+                atomicAdd(&sH[H_RUN + 63u], 1u);
+            }
+            prev_dcc = dcc;
+            left = q0;
+            __syncthreads();
+            if (lane == 0u) { sDcRow[bx] = q0; }
+            const uint32_t lm_raw = tc_map_signed(q);
+            const uint32_t lm = min(lm_raw, 0x07FFFFFFu);
+            const uint32_t cat = tc_bitlen(lm);
+            const uint64_t bl = nz & below;
+            const uint32_t pp = bl ? 63u - (uint32_t)__clzll(bl) : 0u;
+            const uint32_t pcat = (uint32_t)__shfl((int)cat, (int)pp);
+            if (lane != 0u && q != 0) {
+                if (lm_raw > 0x07FFFFFFu) { viol = 1u; }
+                const uint32_t run = lane - pp - 1u;
+                atomicAdd(&sH[H_RUN + run], 1u);
+                atomicAdd(&sH[H_LVL + cat], 1u);
+                atomicAdd(&sH[H_LPOS + tc_ctx_pos(lane) * 28u + cat], 1u);
+                atomicAdd(&sH[H_LPREV + tc_ctx_prev(pcat, pp != 0u) * 28u + cat], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t any_viol = __ballot(viol != 0u) != 0ull;
+
+    // ---------------- model choice ----------------
+    if (lane < NROWS) {
+        const uint32_t* cnt;
+        uint32_t nsym;
+        if (lane == 0u) { cnt = sH + H_LVL; nsym = 28u; }
+        else if (lane <= 4u) { cnt = sH + H_LPOS + (lane - 1u) * 28u; nsym = 28u; }
+        else if (lane <= 9u) { cnt = sH + H_LPREV + (lane - 5u) * 28u; nsym = 28u; }
+        else if (lane == 10u) { cnt = sH + H_DC; nsym = 29u; }
+        else if (lane <= 15u) { cnt = sH + H_DPREV + (lane - 11u) * 29u; nsym = 29u; }
+        else { cnt = sH + H_RUN; nsym = 64u; }
+        uint32_t c[64];
+        for (uint32_t s = 0; s < nsym; ++s) { c[s] = cnt[s]; }
+        tc_row_bytes(sRowB[lane], nsym, c);
+        tc_model_from_bytes(sRowB[lane], nsym, sFreq[lane], sCum[lane]);
+    }
+    __syncthreads();
+    // exact cost sums, one candidate per lane, in the reference's order:
+    // lane 0 lvl-none, 1 lvl-pos, 2 lvl-prev, 3 dc-none, 4 dc-prev
+    if (lane < 5u) {
+        uint32_t r0, nr, nsym;
+        const uint32_t* base;
+        if (lane == 0u) { r0 = 0; nr = 1; nsym = 28; base = sH + H_LVL; }
+        else if (lane == 1u) { r0 = 1; nr = 4; nsym = 28; base = sH + H_LPOS; }
+        else if (lane == 2u) { r0 = 5; nr = 5; nsym = 28; base = sH + H_LPREV; }
+        else if (lane == 3u) { r0 = 10; nr = 1; nsym = 29; base = sH + H_DC; }
+        else { r0 = 11; nr = 5; nsym = 29; base = sH + H_DPREV; }
+        double bits = 0.0;
+        for (uint32_t r = 0; r < nr; ++r) {
+            for (uint32_t s = 0; s < nsym; ++s) {
+                const uint32_t cn = base[r * nsym + s];
+                if (cn != 0u) { bits = cost_add(K, bits, cn, sFreq[r0 + r][s]); }
+            }
+        }
+        sCost[lane] = cost_add_const(bits, 8.0 * (double)(nr * nsym));
+    }
+    __syncthreads();
+    uint32_t lvl_model = 0u, dc_model = 0u;
+    {
+        double best = sCost[0];
+        if (sCost[1] < best) { best = sCost[1]; lvl_model = 1u; }
+        if (sCost[2] < best) { lvl_model = 2u; }
+        if (sCost[4] < sCost[3]) { dc_model = 1u; }
+    }
+    const uint32_t lvl_n = lvl_model == 1u ? 4u : (lvl_model == 2u ? 5u : 1u);
+    const uint32_t dc_n = dc_model ? 5u : 1u;
+    const uint32_t lvl_r0 = lvl_model == 1u ? 1u : (lvl_model == 2u ? 5u : 0u);
+    const uint32_t dc_r0 = dc_model ? 11u : 10u;
+    // table slots: [0, dc_n) dc, dc_n run, then lvl rows
+    const uint32_t prefix_len = 1u + dc_n * 29u + 64u + lvl_n * 28u;
+    {
+        // prefix bytes
+        if (lane == 0u) { sPrefix[0] = (uint8_t)(lvl_model | (dc_model << 2)); }
+        for (uint32_t i = lane; i < dc_n * 29u; i += 64u) {
+            sPrefix[1u + i] = sRowB[dc_r0 + i / 29u][i % 29u];
+        }
+        sPrefix[1u + dc_n * 29u + lane] = sRowB[16][lane];
+        for (uint32_t i = lane; i < lvl_n * 28u; i += 64u) {
+            sPrefix[1u + dc_n * 29u + 64u + i] = sRowB[lvl_r0 + i / 28u][i % 28u];
+        }
+        // packed symbol tables
+        for (uint32_t slot = 0; slot < dc_n + 1u + lvl_n; ++slot) {
+            uint32_t row, nsym;
+            if (slot < dc_n) { row = dc_r0 + slot; nsym = 29u; }
+            else if (slot == dc_n) { row = 16u; nsym = 64u; }
+            else { row = lvl_r0 + (slot - dc_n - 1u); nsym = 28u; }
+            if (lane < nsym) {
+                const uint32_t f = sFreq[row][lane];
+                uint32_t mg, sh;
+                tc_rans_recip(f, &mg, &sh);
+                sTabFC[slot * 64u + lane] = (f << 16) | sCum[row][lane];
+                sTabMag[slot * 64u + lane] = mg;
+                sTabSh[slot * 64u + lane] = (uint8_t)sh;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---------------- backward pass ----------------
+    const uint32_t run_slot = dc_n;
+    const uint32_t lvl_slot0 = dc_n + 1u;
+    const uint32_t eob_fc = wave_u32(sTabFC[run_slot * 64u + 63u]);
+    const uint32_t eob_mg = wave_u32(sTabMag[run_slot * 64u + 63u]);
+    const uint32_t eob_sh = wave_u32(sTabSh[run_slot * 64u + 63u]);
+    EncEmit E;
+    E.end = scratch + S.scratch + S.scratch_cap;
+    E.n = 0u;
+    E.acc = 0u;
+    E.limit = S.scratch_cap - TCG_PREFIX_MAX - 8u;
+    E.over = 0u;
+    uint32_t x = 1u << 23;
+    const uint32_t nblk = S.nblk;
+    for (uint32_t bb = nblk; bb-- > 0u;) {
+        const uint32_t blk = S.blk0 + bb;
+        const int32_t q = qbuf[(size_t)blk * 64u + lane];
+        const uint64_t nz = __ballot(q != 0) & ~1ull;
+        const uint32_t lm = min(tc_map_signed(q), 0x07FFFFFFu);
+        const uint32_t cat = tc_bitlen(lm);
+        const uint64_t bl = nz & below;
+        const uint32_t pp = bl ? 63u - (uint32_t)__clzll(bl) : 0u;
+        const uint32_t pcat = (uint32_t)__shfl((int)cat, (int)pp);
+        const uint32_t run = lane - pp - 1u;
+        uint32_t lc = 0u;
+        if (lvl_model == 1u) { lc = tc_ctx_pos(lane); }
+        else if (lvl_model == 2u) { lc = tc_ctx_prev(pcat, pp != 0u); }
+        const uint32_t active = (lane != 0u && q != 0) ? 1u : 0u;
+        const uint32_t li = active ? (lvl_slot0 + lc) * 64u + cat : 0u;
+        const uint32_t ri = active ? run_slot * 64u + run : 0u;
+        const uint32_t l_fc = sTabFC[li], l_mg = sTabMag[li], l_sh = sTabSh[li];
+        const uint32_t r_fc = sTabFC[ri], r_mg = sTabMag[ri], r_sh = sTabSh[ri];
+        const uint32_t dcm = wave_u32(dcmbuf[blk]);
+        const uint32_t dcm_prev = bb ? wave_u32(dcmbuf[blk - 1u]) : 0u;
+
+        x = put_sym(x, eob_fc, eob_mg, eob_sh, E);
+        uint64_t m = nz;
+        while (m) {
+            const uint32_t p = 63u - (uint32_t)__clzll(m);
+            m &= ~(1ull << p);
+            const uint32_t pcat_p = __builtin_amdgcn_readlane(cat, p);
+            const uint32_t plm = __builtin_amdgcn_readlane(lm, p);
+            x = put_suffix(x, pcat_p, plm, E);
+            x = put_sym(x, __builtin_amdgcn_readlane(l_fc, p), __builtin_amdgcn_readlane(l_mg, p),
+                        __builtin_amdgcn_readlane(l_sh, p), E);
+            x = put_sym(x, __builtin_amdgcn_readlane(r_fc, p), __builtin_amdgcn_readlane(r_mg, p),
+                        __builtin_amdgcn_readlane(r_sh, p), E);
+        }
+        const uint32_t dsym = tc_bitlen(dcm);
+        x = put_suffix(x, dsym, dcm, E);
+        const uint32_t dctx = dc_model ? tc_ctx_dc(tc_bitlen(dcm_prev), bb != 0u) : 0u;
+        const uint32_t di = dctx * 64u + dsym;
+        x = put_sym(x, wave_u32(sTabFC[di]), wave_u32(sTabMag[di]), wave_u32(sTabSh[di]), E);
+        if (E.over) { break; }
+    }
+    // ---------------- flush ----------------
+    // emitted bytes occupy [end - n, end); pending (n & 3) bytes go first
+    const uint32_t n = E.n;
+    const uint32_t ok_len = (E.over == 0u && n <= E.limit) ? 1u : 0u;
+    if (ok_len) {
+        const uint32_t r = n & 3u;
+        uint8_t* end = E.end;
+        if (lane < r) { end[-(int64_t)n + (int64_t)lane] = (uint8_t)(E.acc >> (8u * lane)); }
+        uint8_t* st = end - n - 4;
+        if (lane < 4u) { st[lane] = (uint8_t)(x >> (24u - 8u * lane)); }
+        uint8_t* pre = st - prefix_len;
+        for (uint32_t i = lane; i < prefix_len; i += 64u) { pre[i] = sPrefix[i]; }
+    }
+    if (lane == 0u) {
+        tcg_eresult R;
+        R.start = ok_len ? S.scratch_cap - n - 4u - prefix_len : 0u;
+        R.size = ok_len ? n + 4u + prefix_len : 0u;
+        R.status = any_viol ? -1 : (ok_len ? 0 : -15);
+        R.k1 = 0; R.k2 = 0; R.k3 = 0; R.crc = 0; R.offset = 0;
+        results[blockIdx.x] = R;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3: per-frame offset scan + frame header (53 B, CRC32 over bytes 0..48).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_frame_scan(const TcgConsts* __restrict__ K,
+                                                     const tcg_eframe* __restrict__ frames,
+                                                     tcg_eresult* __restrict__ results,
+                                                     const tcg_eslice* __restrict__ slices,
+                                                     const tcg_eplane* __restrict__ planes,
+                                                     tcg_eframe_out* __restrict__ fout)
+{
+    __shared__ uint32_t sSum[256];
+    __shared__ int32_t sStat[256];
+    __shared__ uint32_t sCol[256], sAl[256], sColN[256], sAlN[256];
+    const tcg_eframe F = frames[blockIdx.x];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = F.s1 - F.s0;
+    const uint32_t per = (n + 255u) / 256u;
+    const uint32_t a = F.s0 + min(n, t * per), b = F.s0 + min(n, (t + 1u) * per);
+    uint32_t sum = 0u, col = 0u, al = 0u, coln = 0u, aln = 0u;
+    int32_t stat = 0;
+    for (uint32_t s = a; s < b; ++s) {
+        sum += 17u + results[s].size;
+        if (results[s].status != 0 && stat == 0) { stat = results[s].status; }
+        if (planes[slices[s].plane].kind == 1u) { al += results[s].size; aln++; }
+        else { col += results[s].size; coln++; }
+    }
+    sSum[t] = sum; sStat[t] = stat; sCol[t] = col; sAl[t] = al; sColN[t] = coln; sAlN[t] = aln;
+    __syncthreads();
+    // exclusive scan (small: serial over 256 partials by thread 0)
+    if (t == 0u) {
+        uint32_t acc = 53u, c = 0u, l = 0u, cn = 0u, ln = 0u;
+        int32_t st = 0;
+        for (uint32_t i = 0; i < 256u; ++i) {
+            const uint32_t v = sSum[i];
+            sSum[i] = acc;
+            acc += v;
+            c += sCol[i]; l += sAl[i]; cn += sColN[i]; ln += sAlN[i];
+            if (st == 0 && sStat[i] != 0) { st = sStat[i]; }
+        }
+        if (st == 0 && acc > F.cap) { st = -15; }
+        if (st == 0 && acc > 268435456u) { st = -8; }
+        tcg_eframe_out o;
+        o.size = acc; o.status = st; o.color_payload = c; o.alpha_payload = l;
+        o.color_slices = cn; o.alpha_slices = ln;
+        fout[blockIdx.x] = o;
+        if (st == 0) {
+            uint8_t h[53];
+            for (int i = 0; i < 53; ++i) { h[i] = F.hdr[i]; }
+            h[41] = (uint8_t)(acc >> 24); h[42] = (uint8_t)(acc >> 16);
+            h[43] = (uint8_t)(acc >> 8); h[44] = (uint8_t)acc;
+            const uint32_t crc = crc32_bytes_tab(K->crc, h, 49u);
+            h[49] = (uint8_t)(crc >> 24); h[50] = (uint8_t)(crc >> 16);
+            h[51] = (uint8_t)(crc >> 8); h[52] = (uint8_t)crc;
+            uint8_t* pk = (uint8_t*)F.packet;
+            for (int i = 0; i < 53; ++i) { pk[i] = h[i]; }
+        }
+    }
+    __syncthreads();
+    {
+        uint32_t off = sSum[t];
+        for (uint32_t s = a; s < b; ++s) {
+            results[s].offset = off + 17u;
+            off += 17u + results[s].size;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4: per-slice payload copy into the packet + CRC32 (chunked, combined in
+// GF(2)) + 17-byte slice header.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_slice_emit(const TcgConsts* __restrict__ K,
+                                                     const tcg_eframe* __restrict__ frames,
+                                                     const tcg_eslice* __restrict__ slices,
+                                                     const uint8_t* __restrict__ scratch,
+                                                     tcg_eresult* __restrict__ results,
+                                                     const tcg_eframe_out* __restrict__ fout)
+{
+    __shared__ uint32_t sTab[256];
+    __shared__ uint32_t sRed[256];
+    __shared__ uint32_t sX2n[32];
+    const uint32_t s = blockIdx.x, t = threadIdx.x;
+    const tcg_eslice S = slices[s];
+    const tcg_eresult R = results[s];
+    const tcg_eframe F = frames[S.frame];
+    sTab[t] = K->crc[t];
+    if (t < 32u) { sX2n[t] = K->x2n[t]; }
+    __syncthreads();
+    if (fout[S.frame].status != 0) { return; }
+    const uint8_t* src = scratch + S.scratch + R.start;
+    uint8_t* dst = (uint8_t*)F.packet + R.offset;
+    const uint32_t n = R.size;
+    const uint32_t per = (n + 255u) / 256u;
+    const uint32_t a = min(n, t * per), b = min(n, (t + 1u) * per);
+    uint32_t c = 0u;
+    for (uint32_t i = a; i < b; ++i) {
+        const uint8_t v = src[i];
+        dst[i] = v;
+        c = sTab[(c ^ v) & 0xFFu] ^ (c >> 8);
+    }
+    sRed[t] = a < b ? tc_crc_shift(sX2n, c, (uint64_t)(n - b)) : 0u;
+    __syncthreads();
+    for (uint32_t w = 128u; w > 0u; w >>= 1) {
+        if (t < w) { sRed[t] ^= sRed[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0u) {
+        const uint32_t crc = sRed[0] ^ tc_crc_shift(sX2n, 0xFFFFFFFFu, n) ^ 0xFFFFFFFFu;
+        results[s].crc = crc;
+        uint8_t* h = dst - 17;
+        h[0] = (uint8_t)(n >> 24); h[1] = (uint8_t)(n >> 16); h[2] = (uint8_t)(n >> 8); h[3] = (uint8_t)n;
+        h[4] = (uint8_t)S.hdr_plane;
+        h[5] = (uint8_t)(S.y0 >> 8); h[6] = (uint8_t)S.y0;
+        h[7] = (uint8_t)(S.h >> 8); h[8] = (uint8_t)S.h;
+        h[9] = (uint8_t)S.qpd;
+        h[10] = (uint8_t)R.k1; h[11] = (uint8_t)R.k2; h[12] = (uint8_t)R.k3;
+        h[13] = (uint8_t)(crc >> 24); h[14] = (uint8_t)(crc >> 16); h[15] = (uint8_t)(crc >> 8);
+        h[16] = (uint8_t)crc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launch
+// ---------------------------------------------------------------------------
+extern "C" int32_t tcg_encode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
+                              uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
+                              const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
+                              uint32_t total_tiles, uint64_t scratch, tcg_eresult* results_dev,
+                              tcg_eframe_out* frame_out_dev)
+{
+    hipStream_t st = tcg_stream();
+    const TcgConsts* K = tcg_consts_dev();
+    TcgTimer tm;
+    tm.mark(st, 0);
+    if (total_tiles > 0u) {
+        hipLaunchKernelGGL(k_fdct_quant, dim3(total_tiles), dim3(256), 0, st, qctx_dev, planes_dev,
+                           nplanes, (int32_t*)qbuf);
+    }
+    tm.mark(st, 1);
+    hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
+                       (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
+    tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch,
+                            results_dev);
+    tm.mark(st, 2);
+    hipLaunchKernelGGL(k_frame_scan, dim3(nframes), dim3(256), 0, st, K, frames_dev, results_dev,
+                       slices_dev, planes_dev, frame_out_dev);
+    tm.mark(st, 3);
+    hipLaunchKernelGGL(k_slice_emit, dim3(nslices), dim3(256), 0, st, K, frames_dev, slices_dev,
+                       (const uint8_t*)scratch, results_dev, frame_out_dev);
+    tm.mark(st, 4);
+    tm.finish_encode();
+    return tcg_check("encode launch");
+}

@@ -1,0 +1,188 @@
+/* Integer helpers shared by the C11 host layer and the gfx950 kernels.
+ * Compiled as plain C on the host (TC_HD empty) and as __host__ __device__
+ * inline functions inside the HIP translation units, so the CPU test suite
+ * exercises exactly the arithmetic the kernels run.
+ */
+#ifndef TC_MATH_H
+#define TC_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define TC_HD __host__ __device__ static inline
+#else
+#define TC_HD static inline
+#endif
+
+#define TC_CRC_POLY 0xEDB88320u
+
+/* a * b mod P over GF(2) in the reflected bit order of zlib's CRC32
+ * (bit 31 is x^0). Fixed 32-step loop: no data-dependent trip count. */
+TC_HD uint32_t tc_gf2_mulmod(uint32_t a, uint32_t b)
+{
+    uint32_t p = 0u;
+    for (int i = 0; i < 32; ++i) {
+        if (a & (0x80000000u >> i)) { p ^= b; }
+        b = (b & 1u) ? (b >> 1) ^ TC_CRC_POLY : (b >> 1);
+    }
+    return p;
+}
+
+/* x^(8*nbytes) mod P, from a table pw[k] = x^(2^k) mod P (k = 0..31). */
+TC_HD uint32_t tc_gf2_xpow8n(const uint32_t* pw, uint64_t nbytes)
+{
+    uint32_t p = 0x80000000u; /* x^0 */
+    uint32_t k = 3u;          /* 8 = 2^3 bits per byte */
+    while (nbytes != 0u) {
+        if (nbytes & 1u) { p = tc_gf2_mulmod(pw[k & 31u], p); }
+        nbytes >>= 1;
+        ++k;
+    }
+    return p;
+}
+
+/* Shift a raw (zero-initialised, un-inverted) CRC register over nbytes of
+ * zero data: the term a prefix contributes after nbytes more input. */
+TC_HD uint32_t tc_crc_shift(const uint32_t* pw, uint32_t crc, uint64_t nbytes)
+{
+    return tc_gf2_mulmod(tc_gf2_xpow8n(pw, nbytes), crc);
+}
+
+/* Exact floor(x / f) for x < 2^31, 1 <= f <= 4096 (the rANS state update):
+ * with l = ceil(log2 f) and m = ceil(2^(31+l) / f) < 2^32,
+ * floor(x/f) = umulhi(x, m) >> (l - 1) because x * (m*f - 2^(31+l)) < 2^(31+l).
+ * f == 1 is handled by the caller (q = x). */
+TC_HD void tc_rans_recip(uint32_t f, uint32_t* mag, uint32_t* sh)
+{
+    if (f <= 1u) { *mag = 0u; *sh = 0u; return; }
+    uint32_t l = 0u;
+    while ((1u << l) < f) { ++l; }
+    const uint64_t num = (uint64_t)1 << (31u + l);
+    *mag = (uint32_t)((num + f - 1u) / f);
+    *sh = l - 1u;
+}
+
+/* Exact floor(n / d) for n * d < 2^51 via m = floor(2^51 / d) + 1:
+ * floor(n / d) = (n * m) >> 51 (the quantiser's divide, bit depth <= 12). */
+TC_HD uint64_t tc_qdiv_magic(uint32_t d) { return (((uint64_t)1 << 51) / d) + 1u; }
+
+TC_HD uint32_t tc_qdiv_apply(uint32_t n, uint64_t m)
+{
+    const uint64_t lo = (uint64_t)n * (uint32_t)m;
+    const uint64_t hi = (uint64_t)n * (uint32_t)(m >> 32);
+    return (uint32_t)(((lo >> 32) + hi) >> 19);
+}
+
+/* ---- rANS tables ---- */
+
+/* 8-bit proportional row bytes: round-half-up share of 255, used symbols at
+ * least 1, an empty row becomes {1, 0, ...} (the per-context table format of
+ * the V7-R2 slice prefix). */
+TC_HD void tc_row_bytes(uint8_t* out, uint32_t nsym, const uint32_t* count)
+{
+    uint64_t total = 0u;
+    for (uint32_t s = 0; s < nsym; ++s) { total += count[s]; }
+    if (total == 0u) {
+        for (uint32_t s = 0; s < nsym; ++s) { out[s] = 0u; }
+        out[0] = 1u;
+        return;
+    }
+    for (uint32_t s = 0; s < nsym; ++s) {
+        if (count[s] == 0u) { out[s] = 0u; continue; }
+        const uint64_t num = (uint64_t)count[s] * 255u + total / 2u;
+        uint64_t v;
+        if ((num >> 32) == 0u && (total >> 32) == 0u) { v = (uint32_t)num / (uint32_t)total; }
+        else { v = num / total; }
+        if (v == 0u) { v = 1u; }
+        if (v > 255u) { v = 255u; }
+        out[s] = (uint8_t)v;
+    }
+}
+
+/* Normalise byte counts (<= 255 each) to frequencies summing to 4096:
+ * floor share with a floor of 1 for used symbols, then remove one from the
+ * largest frequency (first on ties) while the sum is over, then add one to
+ * the largest remainder (first on ties) while under. Returns 0 on success,
+ * -1 for an all-zero row. Cum is written with cum[nsym] = 4096. */
+TC_HD int tc_model_from_bytes(const uint8_t* bytes, uint32_t nsym, uint16_t* freq, uint16_t* cum)
+{
+    uint32_t total = 0u;
+    for (uint32_t s = 0; s < nsym; ++s) { total += bytes[s]; }
+    if (total == 0u) { return -1; }
+    uint32_t f[64], rem[64], sum = 0u;
+    for (uint32_t s = 0; s < nsym; ++s) {
+        const uint32_t sc = (uint32_t)bytes[s] * 4096u;
+        f[s] = bytes[s] == 0u ? 0u : sc / total;
+        rem[s] = sc % total;
+        if (bytes[s] != 0u && f[s] == 0u) { f[s] = 1u; }
+        sum += f[s];
+    }
+    while (sum > 4096u) {
+        uint32_t big = 0xFFFFFFFFu;
+        for (uint32_t s = 0; s < nsym; ++s) {
+            if (f[s] > 1u && (big == 0xFFFFFFFFu || f[s] > f[big])) { big = s; }
+        }
+        if (big == 0xFFFFFFFFu) { break; }
+        f[big] -= 1u; rem[big] = 0u; sum -= 1u;
+    }
+    while (sum < 4096u) {
+        uint32_t best = 0xFFFFFFFFu;
+        for (uint32_t s = 0; s < nsym; ++s) {
+            if (bytes[s] == 0u) { continue; }
+            if (best == 0xFFFFFFFFu || rem[s] > rem[best]) { best = s; }
+        }
+        if (best == 0xFFFFFFFFu) { break; }
+        f[best] += 1u; rem[best] = 0u; sum += 1u;
+    }
+    if (sum != 4096u) { return -1; }
+    uint32_t acc = 0u;
+    for (uint32_t s = 0; s < nsym; ++s) {
+        freq[s] = (uint16_t)f[s];
+        cum[s] = (uint16_t)acc;
+        acc += f[s];
+    }
+    cum[nsym] = (uint16_t)acc;
+    return 0;
+}
+
+TC_HD uint32_t tc_bitlen(uint32_t m)
+{
+    uint32_t n = 0u;
+    while (m != 0u) { ++n; m >>= 1; }
+    return n;
+}
+
+TC_HD uint32_t tc_map_signed(int32_t v)
+{
+    const int64_t s = v;
+    return s >= 0 ? (uint32_t)((uint64_t)s << 1) : (uint32_t)(((uint64_t)(-s) << 1) - 1u);
+}
+
+TC_HD int32_t tc_unmap_signed(uint32_t m)
+{
+    return (m & 1u) == 0u ? (int32_t)(m >> 1) : -(int32_t)(((uint64_t)m + 1u) >> 1);
+}
+
+/* context buckets of the V7-R2 conditional models */
+TC_HD uint32_t tc_ctx_pos(uint32_t pos) { return pos < 8u ? 0u : pos < 16u ? 1u : pos < 32u ? 2u : 3u; }
+TC_HD uint32_t tc_ctx_prev(uint32_t cat, uint32_t has_prev)
+{
+    return has_prev == 0u ? 4u : cat == 0u ? 0u : cat == 1u ? 1u : cat <= 3u ? 2u : 3u;
+}
+TC_HD uint32_t tc_ctx_dc(uint32_t cat, uint32_t has_prev)
+{
+    return has_prev == 0u ? 0u : 1u + (cat == 0u ? 0u : cat == 1u ? 1u : cat <= 3u ? 2u : 3u);
+}
+
+TC_HD int32_t tc_dc_predict(uint32_t has_left, int32_t left, uint32_t has_top, int32_t top)
+{
+    if (has_left && has_top) {
+        const int64_t s = (int64_t)left + top;
+        return s >= 0 ? (int32_t)((s + 1) >> 1) : -(int32_t)((-s) >> 1);
+    }
+    if (has_left) { return left; }
+    if (has_top) { return top; }
+    return 0;
+}
+
+#endif

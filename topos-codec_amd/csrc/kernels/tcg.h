@@ -1,0 +1,170 @@
+/* Internal C ABI between the C11 host layer and the gfx950 HIP kernels.
+ *
+ * Plain structs and pointers only: the host side is compiled as C11 with gcc
+ * and never sees HIP types. Every device pointer is carried as uint64_t so the
+ * descriptor tables can be uploaded verbatim.
+ *
+ * Work units
+ *  - encode: one descriptor per (frame, plane) [tcg_eplane] and one per band
+ *    slice [tcg_eslice]; all frames of a batch share one flat block index
+ *    space so a single launch of each kernel covers the whole batch;
+ *  - decode: one descriptor per slice [tcg_dslice] and per (frame, plane)
+ *    [tcg_dplane].
+ */
+#ifndef TCG_H
+#define TCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCG_SLICE_BYTES_PER_BLOCK 512u /* entropy scratch bound per 8x8 block */
+#define TCG_SLICE_BYTES_EXTRA 1024u    /* prefix tables + state + slack */
+#define TCG_PREFIX_MAX 352u            /* flags + 5x29 + 64 + 5x28 = 350 */
+
+/* quantiser context of one (plane, qp): natural-order step, half step, dead
+ * zone and the 51-bit exact reciprocal m = floor(2^51/Q)+1 split in two words
+ * (q = (n * m) >> 51, exact for bit depth <= 12). */
+typedef struct tcg_qctx {
+    uint32_t Q[64];
+    uint32_t half[64];
+    uint32_t dz[64];
+    uint32_t mlo[64];
+    uint32_t mhi[64];
+    int32_t fclamp;
+    uint32_t bit_depth;
+    int32_t mid;
+    int32_t maxv;
+} tcg_qctx;
+
+typedef struct tcg_eplane {
+    uint64_t src;        /* device u16 samples, visible area */
+    uint32_t stride;     /* elements */
+    uint32_t vis_w, vis_h;
+    uint32_t bcols, brows;
+    uint32_t blk0;       /* first block (flat batch index) of this plane */
+    uint32_t qctx;       /* index into the tcg_qctx table */
+    uint32_t kind;       /* 0 colour, 1 alpha */
+    uint32_t alpha_shift;/* alpha mode 2: pre-quantisation shift (0 = off) */
+    uint32_t coded_w;    /* coded width in samples */
+    uint32_t tile0;      /* first transform tile of this plane (ascending) */
+} tcg_eplane;
+
+typedef struct tcg_eslice {
+    uint32_t plane;      /* index into the tcg_eplane table */
+    uint32_t frame;      /* frame index in the batch */
+    uint32_t y0, h;      /* band in block rows */
+    uint32_t blk0, nblk; /* flat block range */
+    uint64_t scratch;    /* byte offset of this slice's scratch region */
+    uint32_t scratch_cap;
+    uint32_t qpd;        /* qp_delta_biased written to the slice header */
+    uint32_t hdr_plane;  /* plane number written to the slice header */
+    uint32_t pad_;
+} tcg_eslice;
+
+/* per-slice encode result */
+typedef struct tcg_eresult {
+    uint32_t start;      /* payload start, byte offset inside the scratch region */
+    uint32_t size;       /* payload bytes */
+    int32_t status;      /* 0 ok, else a TC_ERR_* code */
+    uint32_t k1, k2, k3; /* slice header k fields (alpha Rice parameters) */
+    uint32_t crc;        /* payload CRC32 (written by the emit kernel) */
+    uint32_t offset;     /* payload offset inside the frame packet */
+} tcg_eresult;
+
+typedef struct tcg_eframe {
+    uint64_t packet;     /* device output buffer */
+    uint32_t cap;        /* its capacity */
+    uint32_t s0, s1;     /* slice range */
+    uint8_t hdr[56];     /* 53-byte header template (size/CRC filled on device) */
+} tcg_eframe;
+
+typedef struct tcg_eframe_out {
+    uint32_t size;       /* packet bytes */
+    int32_t status;
+    uint32_t color_payload, alpha_payload;
+    uint32_t color_slices, alpha_slices;
+} tcg_eframe_out;
+
+/* decode side */
+typedef struct tcg_dslice {
+    uint64_t payload;    /* device pointer to slice payload */
+    uint32_t size;
+    uint32_t crc;        /* expected CRC32 from the slice header */
+    uint32_t plane;      /* index into the tcg_dplane table */
+    uint32_t y0, h;      /* band in block rows */
+    uint32_t blk0;       /* first block (flat) of the band */
+    uint32_t qctx;       /* index into the tcg_qctx table */
+    uint32_t kind;       /* 0 colour (rANS2), 1 alpha (Rice/MED) */
+    uint32_t k1, k2;     /* alpha Rice parameters */
+    uint32_t dc_max;     /* colour: largest legal mapped DC residual */
+    uint32_t lv_max;     /* colour: largest legal mapped AC level */
+    uint32_t pad_;
+} tcg_dslice;
+
+typedef struct tcg_dplane {
+    uint64_t dst;        /* device u16 output, visible area */
+    uint64_t tmp;        /* unfiltered plane (same layout) when deblock != 0 */
+    uint32_t stride;     /* elements */
+    uint32_t vis_w, vis_h;
+    uint32_t bcols, brows;
+    uint32_t blk0;       /* flat block index of block (0,0) */
+    uint32_t row_slice;  /* offset into the row->slice table (one u16 per block row) */
+    uint32_t kind;       /* 0 colour, 1 alpha */
+    uint32_t alpha_mask; /* alpha mode 2 multiple-of check mask */
+    uint32_t fill;       /* concealment value */
+    uint32_t maxv;       /* largest sample value */
+    uint32_t coded_w;
+    uint32_t deblock;    /* 1 = run the output deblocking filter on this plane */
+    uint32_t db_alpha;   /* deblock strength alpha (beta and clip derive from it) */
+    uint32_t tile0;      /* first transform tile of this plane (ascending) */
+    uint32_t pad_;
+} tcg_dplane;
+
+/* ---- runtime ---- */
+int32_t tcg_init(void);
+const char* tcg_last_error(void);
+int32_t tcg_malloc(uint64_t* dptr, size_t bytes);
+int32_t tcg_free(uint64_t dptr);
+int32_t tcg_host_alloc(void** hptr, size_t bytes);
+int32_t tcg_host_free(void* hptr);
+int32_t tcg_h2d(uint64_t dst, const void* src, size_t bytes);
+int32_t tcg_d2h(void* dst, uint64_t src, size_t bytes);
+int32_t tcg_h2d_2d(uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows);
+int32_t tcg_d2h_2d(void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows);
+int32_t tcg_sync(void);
+/* upload the (12 - log2 f) cost table, f = 0..4096 (entry 0 unused), and
+ * the CRC32 tables (bytewise table, x^(2^k) mod P) */
+int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uint32_t* x2n32);
+/* wide (bit depth > 12) integer paths for the next launches */
+void tcg_set_wide(int wide);
+
+/* ---- encode ---- */
+int32_t tcg_encode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
+                   uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
+                   const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
+                   uint32_t total_tiles, uint64_t scratch, tcg_eresult* results_dev,
+                   tcg_eframe_out* frame_out_dev);
+
+/* ---- decode ---- */
+int32_t tcg_decode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
+                   uint32_t nslices, const tcg_dslice* slices_dev, const uint16_t* row_slice_dev,
+                   uint64_t qbuf, uint32_t total_tiles, int32_t* slice_status_dev,
+                   uint32_t db_tiles, const uint32_t* db_tile0_dev);
+
+/* timing of the most recent tcg_encode / tcg_decode launch sequence, per
+ * kernel, in milliseconds (HIP events on the launch stream) */
+typedef struct tcg_timing {
+    float fdct_ms, slice_enc_ms, scan_ms, emit_ms;
+    float slice_dec_ms, idct_ms, deblock_ms;
+} tcg_timing;
+void tcg_get_timing(tcg_timing* t);
+void tcg_set_timing(int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
