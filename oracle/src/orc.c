@@ -1144,3 +1144,93 @@ void orc_deblock_plane(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uin
         }
     }
 }
+
+/* -------------------------------------------------- test image synthesiser */
+
+/* xorshift64* driven test images of the reference test suite (flat, gradient,
+ * grain, detail, mixed quadrants; 10-bit patterns x4 for 12-bit; 4:2:2 or
+ * full-width chroma; alpha band pattern), restated from
+ * native/tests/support/image_synth.c:1-150. Used only to regenerate the
+ * inputs behind the reference's golden packets. */
+static uint64_t isyn_next(uint64_t* s)
+{
+    uint64_t x = *s;
+    x ^= x >> 12; x ^= x << 25; x ^= x >> 27;
+    *s = x;
+    return x * 0x2545F4914F6CDD1Dull;
+}
+
+static uint32_t isyn_clamp(int32_t v, uint32_t scale)
+{
+    if (scale == 1u) { return v < 0 ? 0u : v > 1023 ? 1023u : (uint32_t)v; }
+    int64_t s = (int64_t)v * scale;
+    return s < 0 ? 0u : s > 4095 ? 4095u : (uint32_t)s;
+}
+
+static int32_t isyn_luma(uint32_t kind, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint64_t* s)
+{
+    uint32_t k = kind;
+    if (kind == 4u) {
+        const uint32_t q = (y >= h / 2u ? 2u : 0u) + (x >= w / 2u ? 1u : 0u);
+        k = q == 0u ? 0u : q == 1u ? 1u : q == 2u ? 2u : 3u;
+    }
+    switch (k) {
+    case 0: return (int32_t)(320u + (((x / 32u) + (y / 32u)) % 3u) * 6u);
+    case 1: {
+        const int64_t gx = (int64_t)x * 700 / (int64_t)(w > 1u ? w - 1u : 1u);
+        const int64_t gy = (int64_t)y * 300 / (int64_t)(h > 1u ? h - 1u : 1u);
+        return (int32_t)(64 + gx + gy);
+    }
+    case 2: {
+        const uint64_t r = isyn_next(s);
+        return 350 + (int32_t)(((x / 48u) + (y / 48u)) % 4u) * 40 + (int32_t)(r % 13u) - 6;
+    }
+    case 3: return (((x / 8u) + (y / 8u)) % 2u == 0u ? 180 : 760) + (((x + y) % 16u) < 2u ? 200 : 0);
+    default: return 512;
+    }
+}
+
+int orc_image_synth(uint64_t seed, uint32_t w, uint32_t h, uint32_t kind, uint32_t bit_depth,
+                    uint32_t full_chroma, uint16_t* y, uint16_t* u, uint16_t* v, uint16_t* a)
+{
+    const uint32_t cw = full_chroma ? w : (w + 1u) / 2u;
+    const uint32_t scale = bit_depth >= 12u ? 4u : 1u;
+    uint64_t sy = seed ^ 0x4D4F565549564531ull, su = seed ^ 0x5543425732534544ull;
+    uint64_t sv = seed ^ 0x5643425732534544ull, sa = seed ^ 0x414C504841323131ull;
+    for (int i = 0; i < 8; ++i) { isyn_next(&sy); isyn_next(&su); isyn_next(&sv); isyn_next(&sa); }
+    for (uint32_t yy = 0; yy < h; ++yy) {
+        for (uint32_t x = 0; x < w; ++x) {
+            const int32_t l = isyn_luma(kind, x, yy, w, h, &sy);
+            if (y) { y[(size_t)yy * w + x] = (uint16_t)isyn_clamp(l, scale); }
+        }
+    }
+    for (uint32_t yy = 0; yy < h; ++yy) {
+        for (uint32_t x = 0; x < cw; ++x) {
+            int32_t bu = 512 + (int32_t)((x / 24u) % 5u) * 24 - 48;
+            int32_t bv = 512 + (int32_t)((yy / 24u) % 5u) * 20 - 40;
+            if (kind == 2u || kind == 4u || kind == 3u) {
+                bu += (int32_t)(isyn_next(&su) % 9u) - 4;
+                bv += (int32_t)(isyn_next(&sv) % 9u) - 4;
+            }
+            if (u) { u[(size_t)yy * cw + x] = (uint16_t)isyn_clamp(bu, scale); }
+            if (v) { v[(size_t)yy * cw + x] = (uint16_t)isyn_clamp(bv, scale); }
+        }
+    }
+    if (a) {
+        for (uint32_t yy = 0; yy < h; ++yy) {
+            for (uint32_t x = 0; x < w; ++x) {
+                uint32_t val;
+                if (x < w / 4u) { val = 65535u; }
+                else if (x < w / 2u) { val = (uint32_t)(65535ull * (x - w / 4u) / (w / 4u)); }
+                else if (((x / 8u) + (yy / 8u)) % 2u == 0u) { val = 49152u; }
+                else { val = 0u; }
+                if (kind == 2u || kind == 4u) {
+                    const uint32_t d = (uint32_t)(isyn_next(&sa) % 512u);
+                    val = d > val ? 0u : val - d;
+                }
+                a[(size_t)yy * w + x] = (uint16_t)val;
+            }
+        }
+    }
+    return 0;
+}

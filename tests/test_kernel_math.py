@@ -1,0 +1,129 @@
+"""The integer helpers the kernels share with the host (tc_math.h), checked on
+the CPU: exact rANS state division, exact quantiser division, GF(2) CRC
+combination against zlib, and the rANS2 row tables against the compiled
+reference's own row codec (native/src/entropy/rans.c:283-362)."""
+import ctypes
+import os
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "csrc", "mathcheck.c")
+
+
+@pytest.fixture(scope="module")
+def mc(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("mc") / "libmathcheck.so")
+    subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-I",
+                    os.path.join(ROOT, "topos-codec_amd", "csrc", "kernels"), SRC, "-o", out], check=True)
+    lib = ctypes.CDLL(out)
+    lib.mc_rans_div.restype = ctypes.c_uint32
+    lib.mc_qdiv_magic.restype = ctypes.c_uint64
+    lib.mc_qdiv_apply.restype = ctypes.c_uint32
+    lib.mc_qdiv_apply.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
+    lib.mc_crc_chunked.restype = ctypes.c_uint32
+    lib.mc_gf2_mulmod.restype = ctypes.c_uint32
+    return lib
+
+
+def test_rans_division_exact(mc):
+    rng = np.random.default_rng(1)
+    for f in range(1, 4097):
+        xmax = (1 << 19) * f  # the state is below 2^19 * f after renormalisation
+        xs = [0, 1, f - 1, f, f + 1, xmax - 1, xmax - f, xmax - f - 1, (1 << 31) - 1 if xmax > (1 << 31) - 1 else xmax - 2]
+        xs += rng.integers(0, min(xmax, 1 << 31), 24).tolist()
+        for x in xs:
+            x = int(min(x, xmax - 1, (1 << 31) - 1))
+            assert mc.mc_rans_div(x, f) == x // f, (x, f)
+
+
+def test_quant_division_exact(mc):
+    rng = np.random.default_rng(2)
+    divisors = [1, 2, 3, 5, 7, 16, 1448, 4096, 27555, 18045205] + rng.integers(1, 1 << 25, 200).tolist()
+    for d in divisors:
+        m = mc.mc_qdiv_magic(int(d))
+        ns = [0, 1, d - 1, d, d + 1, (1 << 26) - 1, (1 << 25) + (d >> 1)] + rng.integers(0, 1 << 26, 50).tolist()
+        for n in ns:
+            n = int(n)
+            if n * d >= (1 << 51):
+                continue
+            assert mc.mc_qdiv_apply(n, m) == n // d, (n, d)
+
+
+def crc_tables():
+    tab = []
+    for i in range(256):
+        c = i
+        for _ in range(8):
+            c = (c >> 1) ^ 0xEDB88320 if c & 1 else c >> 1
+        tab.append(c)
+    return tab
+
+
+def test_crc_combine_matches_zlib(mc, pkg):
+    lib = pkg.load_library()
+    t = (ctypes.c_uint32 * 256)()
+    x2n = (ctypes.c_uint32 * 32)()
+    lib.tc_crc_tables(t, x2n)
+    assert list(t) == crc_tables()
+    rng = np.random.default_rng(3)
+    for n in [0, 1, 2, 63, 64, 65, 255, 1000, 4097, 80000]:
+        data = rng.integers(0, 256, n, dtype=np.uint8)
+        for parts in (1, 64, 256):
+            got = mc.mc_crc_chunked(data.ctypes.data_as(ctypes.c_void_p), n, t, x2n, parts)
+            assert got == zlib.crc32(data.tobytes()) & 0xFFFFFFFF, (n, parts)
+
+
+class RansModel(ctypes.Structure):
+    _fields_ = [("syms", ctypes.c_uint32), ("freq", ctypes.c_uint16 * 64), ("cum", ctypes.c_uint16 * 65)]
+
+
+def test_row_tables_match_reference(mc, ref):
+    rng = np.random.default_rng(4)
+    for trial in range(400):
+        nsym = [28, 29, 64][trial % 3]
+        kind = trial % 5
+        if kind == 0:
+            cnt = np.zeros(nsym, np.uint32)
+        elif kind == 1:
+            cnt = rng.integers(0, 3, nsym).astype(np.uint32)
+        elif kind == 2:
+            cnt = (rng.pareto(1.0, nsym) * 50).astype(np.uint32)
+        elif kind == 3:
+            cnt = np.zeros(nsym, np.uint32); cnt[rng.integers(0, nsym)] = rng.integers(1, 10**6)
+        else:
+            cnt = rng.integers(0, 10**6, nsym).astype(np.uint32)
+            cnt[rng.random(nsym) < 0.5] = 0
+        c = (ctypes.c_uint32 * nsym)(*cnt.tolist())
+        a, b = (ctypes.c_uint8 * 64)(), (ctypes.c_uint8 * 64)()
+        mc.mc_row_bytes(a, nsym, c)
+        ref.tc_rans2_row_encode(b, nsym, c)
+        assert bytes(a)[:nsym] == bytes(b)[:nsym]
+        f, cu = (ctypes.c_uint16 * 64)(), (ctypes.c_uint16 * 65)()
+        rc_a = mc.mc_model(a, nsym, f, cu)
+        m = RansModel()
+        rc_b = ref.tc_rans2_row_decode(b, nsym, ctypes.byref(m))
+        assert (rc_a == 0) == (rc_b == 0)
+        if rc_a == 0:
+            assert list(f)[:nsym] == list(m.freq)[:nsym]
+            assert list(cu)[:nsym + 1] == list(m.cum)[:nsym + 1]
+
+
+def test_ctx_log2_matches_reference(pkg, ref):
+    lib = pkg.load_library()
+    lib.tc_ctx_log2.restype = ctypes.c_double
+    lib.tc_ctx_log2.argtypes = [ctypes.c_double]
+    ref.tc_ctx_log2.restype = ctypes.c_double
+    ref.tc_ctx_log2.argtypes = [ctypes.c_double]
+    for f in range(1, 4097):
+        a, b = lib.tc_ctx_log2(float(f)), ref.tc_ctx_log2(float(f))
+        assert a.hex() == b.hex(), f
+    # closed form sanity: exact at powers of two, close to log2 elsewhere
+    import math
+    for k in range(13):
+        assert lib.tc_ctx_log2(float(1 << k)) == float(k)
+    for f in range(1, 4097, 37):
+        assert abs(lib.tc_ctx_log2(float(f)) - math.log2(f)) < 1e-7  # series to z^13
