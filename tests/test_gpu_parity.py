@@ -33,3 +33,36 @@ def test_encode_decode_422(pkg, abi, syn, oracle, codec, w, h, qp, qm, feat, sr)
     assert d_rc == o_rc == 0
     for a, b in zip(d_planes, o_planes):
         assert (a == b).all()
+
+
+CASES_OTHER = [
+    # (kind, w, h, cfg kwargs)
+    ("4444", 73, 39, dict(qp=69, profile=6, pixel_format=2, bit_depth=12, slice_rows=4, alpha_mode=1, alpha_bit_depth=16)),
+    ("4444", 256, 128, dict(qp=40, profile=6, pixel_format=2, bit_depth=12, slice_rows=2, alpha_mode=2, alpha_bit_depth=8)),
+    ("444", 64, 40, dict(qp=30, profile=3, pixel_format=1, bit_depth=12, slice_rows=1, qmatrix_id=1)),
+    ("raw", 128, 96, dict(qp=64, profile=7, pixel_format=3, bit_depth=12, slice_rows=8)),
+    ("422", 96, 64, dict(qp=8, profile=3, bit_depth=16, slice_rows=3)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,kw", CASES_OTHER, ids=[f"{c[0]}-{c[1]}x{c[2]}-{i}" for i, c in enumerate(CASES_OTHER)])
+def test_encode_decode_other_formats(pkg, abi, syn, oracle, codec, kind, w, h, kw):
+    kw = dict(kw)
+    qp = kw.pop("qp")
+    planes = frame_for(pkg, syn, kind, w, h, seed=w + 3 * h)
+    if kw.get("bit_depth") == 16:
+        planes = [p.astype("uint16") * 64 for p in planes]
+    cfg = abi.make_config(w, h, qp, entropy_mode=8, **kw)
+    orc_rc, orc_pkt = orc_encode(oracle, cfg, planes)
+    st, pkt, stats = codec.encode_status(cfg, planes)
+    assert st == orc_rc, codec.last_error()
+    if st != 0:
+        return
+    assert pkt == orc_pkt
+    dims = abi.plane_dims(w, h, kw.get("pixel_format", 0), kw.get("alpha_mode", 0))
+    o_rc, o_planes, _, _ = orc_decode(oracle, orc_pkt, dims)
+    d_rc, d_planes, info = codec.decode(orc_pkt, strict=False)
+    assert d_rc == o_rc
+    for a, b in zip(d_planes, o_planes):
+        assert (a == b).all()

@@ -125,7 +125,8 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
             }
             for (uint32_t y0 = 0; y0 < fh->brows[p]; y0 += P->slice_rows) {
                 const uint32_t h = fh->brows[p] - y0 < P->slice_rows ? fh->brows[p] - y0 : P->slice_rows;
-                const size_t cap = (size_t)h * fh->bcols[p] * TCG_SLICE_BYTES_PER_BLOCK + TCG_SLICE_BYTES_EXTRA;
+                const size_t per_blk = tc_is_alpha(fh, p) ? 2u * TCG_SLICE_BYTES_PER_BLOCK : TCG_SLICE_BYTES_PER_BLOCK;
+                const size_t cap = (size_t)h * fh->bcols[p] * per_blk + TCG_SLICE_BYTES_EXTRA;
                 scratch_bytes += al256(cap);
                 pkt_bytes += TC_SLICE_HDR_SIZE + cap;
             }
@@ -213,7 +214,9 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
                 S->blk0 = blk + y0 * fh->bcols[p];
                 S->nblk = h * fh->bcols[p];
                 S->scratch = scr_off;
-                S->scratch_cap = (uint32_t)(S->nblk * TCG_SLICE_BYTES_PER_BLOCK + TCG_SLICE_BYTES_EXTRA);
+                S->scratch_cap = (uint32_t)(S->nblk * (tc_is_alpha(fh, p) ? 2u * TCG_SLICE_BYTES_PER_BLOCK
+                                                                        : TCG_SLICE_BYTES_PER_BLOCK)
+                                            + TCG_SLICE_BYTES_EXTRA);
                 S->qpd = tc_is_alpha(fh, p) ? 64u : (uint32_t)(64 + P->qp_eff[p] - (int32_t)fh->qp_base);
                 S->hdr_plane = p;
                 scr_off += al256(S->scratch_cap);
@@ -251,7 +254,7 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode launch failed: %s", tcg_last_error());
     }
-    if (tcg_d2h(g_hres + o_fo, g_res.p + o_fo, (size_t)n * sizeof(tcg_eframe_out)) != 0) {
+    if (tcg_d2h(g_hres, g_res.p, res_bytes) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode result download: %s", tcg_last_error());
     }
@@ -276,7 +279,10 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
                 st->alpha_header_bytes = TC_SLICE_HDR_SIZE * o->alpha_slices;
                 st->slice_count = (uint16_t)plan[f].nslices;
                 st->qp_base = plan[f].fh.qp_base;
-                st->alpha_max_abs_error = 0u;
+                uint32_t merr = 0u;
+                const tcg_eresult* rs = (const tcg_eresult*)(g_hres + o_res);
+                for (uint32_t i = hf[f].s0; i < hf[f].s1; ++i) { merr = rs[i].aux > merr ? rs[i].aux : merr; }
+                st->alpha_max_abs_error = (uint16_t)merr;
             }
             r->rc = TC_OK;
             if (r->out != NULL || r->dev_out == 0u) {

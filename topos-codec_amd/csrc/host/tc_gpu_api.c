@@ -5,7 +5,7 @@
 #include "tc_host.h"
 #include "topos_gpu.h"
 
-int32_t tcx_alpha_supported(void) { return 0; }
+int32_t tcx_alpha_supported(void) { return 1; }
 
 int32_t tcx_gpu_init(void)
 {

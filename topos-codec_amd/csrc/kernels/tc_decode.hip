@@ -484,7 +484,7 @@ extern "C" int32_t tcg_decode(const tcg_qctx* qctx_dev, uint32_t nplanes, const 
     tm.mark(st, 0);
     hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
                        (int32_t*)qbuf, slice_status_dev);
-    tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, slice_status_dev);
+    tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
     tm.mark(st, 1);
     if (total_tiles > 0u) {
         hipLaunchKernelGGL(k_dequant_idct, dim3(total_tiles), dim3(256), 0, st, qctx_dev, planes_dev,

@@ -93,6 +93,6 @@ void tcg_alpha_encode_launch(hipStream_t st, const tcg_eplane* planes, const tcg
                              uint32_t nslices, const int32_t* unused, uint8_t* scratch,
                              tcg_eresult* results);
 void tcg_alpha_decode_launch(hipStream_t st, const tcg_dplane* planes, const tcg_dslice* slices,
-                             uint32_t nslices, int32_t* status);
+                             uint32_t nslices, int32_t* qbuf, int32_t* status);
 
 #endif

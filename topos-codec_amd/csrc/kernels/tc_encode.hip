@@ -455,7 +455,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         R.start = ok_len ? S.scratch_cap - n - 4u - prefix_len : 0u;
         R.size = ok_len ? n + 4u + prefix_len : 0u;
         R.status = any_viol ? -1 : (ok_len ? 0 : -15);
-        R.k1 = 0; R.k2 = 0; R.k3 = 0; R.crc = 0; R.offset = 0;
+        R.k1 = 0; R.k2 = 0; R.k3 = 0; R.crc = 0; R.offset = 0; R.aux = 0; R.pad_ = 0;
         results[blockIdx.x] = R;
     }
 }

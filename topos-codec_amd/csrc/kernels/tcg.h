@@ -73,6 +73,8 @@ typedef struct tcg_eresult {
     uint32_t k1, k2, k3; /* slice header k fields (alpha Rice parameters) */
     uint32_t crc;        /* payload CRC32 (written by the emit kernel) */
     uint32_t offset;     /* payload offset inside the frame packet */
+    uint32_t aux;        /* alpha mode 2: largest |source - coded| in the band */
+    uint32_t pad_;
 } tcg_eresult;
 
 typedef struct tcg_eframe {
