@@ -32,6 +32,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 W, H, QP, NFRAMES = 3840, 2160, 62, 8
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved by one k_fdct_quant launch over the 8-frame batch, from two
+# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same
+# counters read for a plain 256 MiB device copy (fetch x2.0, write x1.0).
+FDCT_TRAFFIC_BYTES = 796_840_000
 
 
 def load_pkg():
@@ -55,45 +59,60 @@ def pct(v, q):
 
 
 def cpu_baseline(abi, frames, cfg, budget_s):
-    """Reference C library (oracle/_ref, built from the reference sources) timed
-    on this host's cores on a bounded sample of the same workload."""
+    """CPU leg timed on this host's cores on a bounded sample of the same
+    workload (whole frames, encode + decode each, until budget_s has passed).
+    Uses the reference C library built from the reference sources
+    (oracle/_ref, kind "reference", its own slice thread pool) when present,
+    else the single-threaded C restatement in oracle/ (kind "port").
+    Returns (record, packet of frame 0)."""
     ref_path = os.path.join(ROOT, "oracle", "_ref", "libtopos_ref.so")
-    kind = "reference"
-    threads = min(16, os.cpu_count() or 1)
-    os.environ["TOPOS_SLICE_THREADS"] = str(threads)
-    if not os.path.exists(ref_path):
-        return None
-    ref = ctypes.CDLL(ref_path)
-    ref.tc_frame_packet_bound.restype = ctypes.c_size_t
-    bound = ref.tc_frame_packet_bound(ctypes.byref(cfg))
-    out = ctypes.create_string_buffer(bound)
     dims = abi.plane_dims(W, H, 0, 0)
-    dec = [np.zeros((h, w), np.uint16) for (w, h) in dims]
-    dptr = (ctypes.c_void_p * 4)(*([d.ctypes.data for d in dec] + [None]))
-    done, t0 = 0, time.perf_counter()
-    pk0 = None
+    if os.path.exists(ref_path):
+        kind, threads = "reference", min(16, os.cpu_count() or 1)
+        os.environ["TOPOS_SLICE_THREADS"] = str(threads)
+        ref = ctypes.CDLL(ref_path)
+        ref.tc_frame_packet_bound.restype = ctypes.c_size_t
+        bound = ref.tc_frame_packet_bound(ctypes.byref(cfg))
+        out = ctypes.create_string_buffer(bound)
+        dec = [np.zeros((h, w), np.uint16) for (w, h) in dims]
+        dptr = (ctypes.c_void_p * 4)(*([d.ctypes.data for d in dec] + [None]))
+
+        def one(planes):
+            inp = abi.FrameInput()
+            inp.struct_size, inp.abi_version = ctypes.sizeof(inp), 2
+            for i, p in enumerate(planes):
+                inp.planes[i] = p.ctypes.data
+            st = abi.FrameStats()
+            rc = ref.tc_frame_encode(ctypes.byref(cfg), ctypes.byref(inp), out, ctypes.c_size_t(bound),
+                                     ctypes.byref(st))
+            assert rc == 0
+            pkt = out.raw[: st.packet_size]
+            info = abi.FrameOutput()
+            assert ref.tc_frame_decode(pkt, ctypes.c_size_t(len(pkt)), dptr, None, ctypes.byref(info)) == 0
+            return pkt
+        what = f"reference C library built from the reference sources, {threads} slice threads"
+    else:
+        kind, threads = "port", 1
+        orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
+        from helpers import orc_decode, orc_encode
+
+        def one(planes):
+            rc, pkt = orc_encode(orc, cfg, planes)
+            assert rc == 0
+            assert orc_decode(orc, pkt, dims)[0] == 0
+            return pkt
+        what = "single-threaded C restatement (oracle/)"
+    done, pk0, t0 = 0, None, time.perf_counter()
     while True:
-        planes = frames[done % len(frames)]
-        inp = abi.FrameInput()
-        inp.struct_size, inp.abi_version = ctypes.sizeof(inp), 2
-        for i, p in enumerate(planes):
-            inp.planes[i] = p.ctypes.data
-        st = abi.FrameStats()
-        rc = ref.tc_frame_encode(ctypes.byref(cfg), ctypes.byref(inp), out, ctypes.c_size_t(bound), ctypes.byref(st))
-        assert rc == 0
-        pkt = out.raw[: st.packet_size]
+        pkt = one(frames[done % len(frames)])
         if done == 0:
             pk0 = pkt
-        info = abi.FrameOutput()
-        rc = ref.tc_frame_decode(pkt, ctypes.c_size_t(len(pkt)), dptr, None, ctypes.byref(info))
-        assert rc == 0
         done += 1
         if time.perf_counter() - t0 >= budget_s and done >= 2:
             break
     dt = time.perf_counter() - t0
     return {"value": round(done / dt, 3), "unit": "frames/s enc+dec", "cores": threads, "kind": kind,
-            "sample": f"{done} frames of the 4K 422 HQ qp{QP} workload (encode+decode each), "
-                      f"reference C library with {threads} slice threads"}, pk0
+            "sample": f"{done} frames of the 4K 422 HQ qp{QP} workload (encode+decode each), {what}"}, pk0
 
 
 def main():
@@ -105,6 +124,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only "
@@ -156,6 +177,12 @@ def main():
         assert st == 0, (rcs, codec.last_error())
 
     all_idx = list(range(nf))
+    if a.copy_calib:
+        nb = a.copy_calib << 20
+        src_b, dst_b = codec.alloc(nb), codec.alloc(nb)
+        codec.copy(dst_b, src_b, nb)
+        print(f"# copy calibration: {nb} bytes read + {nb} bytes written by k_copy", file=sys.stderr, flush=True)
+        src_b.close(); dst_b.close()
     # first pass (untimed): packets for the decode leg's structure parse
     sizes = encode_batch(all_idx)
     fetch_packets(sizes, all_idx)
@@ -215,23 +242,16 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        r = cpu_baseline(abi, frames[:2], cfg, a.cpu_seconds)
-        if r is not None:
-            cpu, ref_pkt0 = r
-            bitexact["packet0_equals_reference"] = ref_pkt0 == host_pkts[0]
-            ref = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libtopos_ref.so"))
-            outs = [np.zeros((h, w), np.uint16) for (w, h) in dims]
-            ptrs = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None]))
-            info = abi.FrameOutput()
-            rc = ref.tc_frame_decode(host_pkts[0], ctypes.c_size_t(len(host_pkts[0])), ptrs, None, ctypes.byref(info))
-            bitexact["decode0_equals_reference"] = rc == 0 and all((x == y).all() for x, y in zip(outs, dec0))
+        cpu, cpu_pkt0 = cpu_baseline(abi, frames[:2], cfg, a.cpu_seconds)
+        bitexact[f"packet0_equals_{cpu['kind']}"] = cpu_pkt0 == host_pkts[0]
+    # decode of packet 0 against the oracle's decode of the same packet
+    orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
+    from helpers import orc_decode, orc_encode
     if "packet0_equals_reference" not in bitexact:
-        orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
-        from helpers import orc_decode, orc_encode
         rc, opk = orc_encode(orc, cfg, frames[0])
         bitexact["packet0_equals_oracle"] = rc == 0 and opk == host_pkts[0]
-        rc, oplanes, _, _ = orc_decode(orc, host_pkts[0], dims)
-        bitexact["decode0_equals_oracle"] = rc == 0 and all((x == y).all() for x, y in zip(oplanes, dec0))
+    rc, oplanes, _, _ = orc_decode(orc, host_pkts[0], dims)
+    bitexact["decode0_equals_oracle"] = rc == 0 and all((x == y).all() for x, y in zip(oplanes, dec0))
 
     # roofline of the dominant HBM-bound kernel: fused FDCT + quantisation,
     # 2 B (u16 sample in) + 4 B (i32 coefficient out) per sample
@@ -265,7 +285,8 @@ def main():
         "latency_ms": {"batch_step_p50": round(pct(times, 50) * 1e3, 3), "batch_step_p99": round(pct(times, 99) * 1e3, 3)},
         "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": "k_fdct_quant", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": FDCT_TRAFFIC_BYTES if nf == NFRAMES else None,
                      "bytes_per_launch": samples * 6,
                      "idct_achieved": round(samples * 6 / (idct_ms * 1e-3) / 1e9, 1) if idct_ms > 0 else None,
                      "deblock_achieved": round(samples * 4 / (db_ms * 1e-3) / 1e9, 1) if db_ms > 0 else None},

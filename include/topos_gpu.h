@@ -29,6 +29,9 @@ int32_t tcx_dev_alloc(uint64_t* ptr, size_t bytes);
 int32_t tcx_dev_free(uint64_t ptr);
 int32_t tcx_h2d(uint64_t dst, const void* src, size_t bytes);
 int32_t tcx_d2h(void* dst, uint64_t src, size_t bytes);
+/* device-to-device copy with a plain vector-copy kernel (bytes multiple of
+ * 16); the calibration point for HBM traffic counters */
+int32_t tcx_copy(uint64_t dst, uint64_t src, size_t bytes);
 
 /* Encode n frames in one launch sequence. cfgs[i] per frame; dev_planes and
  * dev_strides are n x 4 (strides in elements, 0 = tight); dev_out[i] receives

@@ -54,6 +54,7 @@ def load_library(path=LIB_PATH):
     lib.tcx_d2h.argtypes = [vp, ctypes.c_uint64, ctypes.c_size_t]
     lib.tcx_encode_batch_dev.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
     lib.tcx_decode_batch_dev.argtypes = [ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
+    lib.tcx_copy.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
     del u8p
@@ -209,6 +210,11 @@ class DeviceCodec(ToposCodec):
 
     def alloc(self, nbytes):
         return DeviceBuffer(self._lib, nbytes)
+
+    def copy(self, dst, src, nbytes):
+        st = self._lib.tcx_copy(ctypes.c_uint64(dst.addr), ctypes.c_uint64(src.addr), ctypes.c_size_t(nbytes))
+        if st != 0:
+            raise ToposCodecError(st, self.last_error())
 
     def set_timing(self, on):
         self._lib.tcx_set_timing(1 if on else 0)

@@ -53,6 +53,15 @@ int32_t tcx_d2h(void* dst, uint64_t src, size_t bytes)
     return rc;
 }
 
+int32_t tcx_copy(uint64_t dst, uint64_t src, size_t bytes)
+{
+    int32_t rc = tc_engine_acquire();
+    if (rc != TC_OK) { return rc; }
+    rc = tcg_copy(dst, src, bytes) == 0 ? TC_OK : tc_fail(TC_ERR_INVALID_ARGUMENT, "%s", tcg_last_error());
+    tc_engine_release();
+    return rc;
+}
+
 int32_t tcx_encode_batch_dev(const topos_frame_config* cfgs, uint32_t n, const uint64_t* dev_planes,
                              const size_t* dev_strides, const uint64_t* dev_out, const size_t* out_caps,
                              uint32_t* out_sizes, int32_t* rcs, topos_frame_stats* stats)

@@ -138,6 +138,8 @@ int32_t tcg_d2h(void* dst, uint64_t src, size_t bytes);
 int32_t tcg_h2d_2d(uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows);
 int32_t tcg_d2h_2d(void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows);
 int32_t tcg_sync(void);
+/* device copy of bytes (multiple of 16) with a plain vector-copy kernel */
+int32_t tcg_copy(uint64_t dst, uint64_t src, size_t bytes);
 /* upload the (12 - log2 f) cost table, f = 0..4096 (entry 0 unused), and
  * the CRC32 tables (bytewise table, x^(2^k) mod P) */
 int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uint32_t* x2n32);
