@@ -35,3 +35,35 @@ uint32_t mc_crc_chunked(const uint8_t* p, uint32_t n, const uint32_t* tab, const
     }
     return acc ^ tc_crc_shift(pw, 0xFFFFFFFFu, n) ^ 0xFFFFFFFFu;
 }
+
+/* encoder update with precomputed constants against the plain division, for
+ * every frequency 1..4096: state edges around every multiple boundary of f
+ * near the renormalisation bound plus pseudo-random states below it. Returns
+ * the number of mismatches. */
+uint32_t mc_enc_update_check(uint32_t per_f)
+{
+    uint32_t bad = 0u;
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    for (uint32_t f = 1u; f <= 4096u; ++f) {
+        uint32_t A, B, X;
+        const uint32_t cum = (f * 2654435761u) % (4097u - f);
+        tc_rans_enc_consts(f, cum, &A, &B, &X);
+        if (X != (f << 19)) { ++bad; }
+        const uint32_t hi = X - 1u < 0x7FFFFFFFu ? X - 1u : 0x7FFFFFFFu;
+        for (uint32_t i = 0; i < per_f + 8u; ++i) {
+            uint32_t x;
+            if (i < 4u) { x = hi - i; }
+            else if (i < 8u) { x = (uint32_t)((uint64_t)hi / f * f) - 2u + i - 4u; }
+            else {
+                s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+                x = (uint32_t)(s % ((uint64_t)hi + 1u));
+            }
+            if (x > hi) { continue; }
+            const uint64_t want = (uint64_t)(x / f) * 4096u + x % f + cum;
+            if ((uint64_t)tc_rans_enc_update(x, A, B) != want) { ++bad; }
+        }
+    }
+    return bad;
+}
+
+uint32_t mc_zigzag_pos(uint32_t n) { return tc_zigzag_pos(n); }

@@ -26,6 +26,8 @@ def mc(tmp_path_factory):
     lib.mc_qdiv_apply.argtypes = [ctypes.c_uint32, ctypes.c_uint64]
     lib.mc_crc_chunked.restype = ctypes.c_uint32
     lib.mc_gf2_mulmod.restype = ctypes.c_uint32
+    lib.mc_enc_update_check.restype = ctypes.c_uint32
+    lib.mc_zigzag_pos.restype = ctypes.c_uint32
     return lib
 
 
@@ -38,6 +40,33 @@ def test_rans_division_exact(mc):
         for x in xs:
             x = int(min(x, xmax - 1, (1 << 31) - 1))
             assert mc.mc_rans_div(x, f) == x // f, (x, f)
+
+
+def test_rans_encoder_update_exact(mc):
+    # x' = (x / f) * 4096 + x % f + cum through the reciprocal constants the
+    # encode kernel uses, for every frequency and states up to the bound
+    assert mc.mc_enc_update_check(2000) == 0
+
+
+def test_zigzag_closed_form(mc):
+    # the scan position the decode kernel computes per lane, against the
+    # zigzag walk built step by step (right / down-left / down / up-right)
+    order, r, c, up = [], 0, 0, True
+    while len(order) < 64:
+        order.append(8 * r + c)
+        if up:
+            if c == 7: r, up = r + 1, False
+            elif r == 0: c, up = c + 1, False
+            else: r, c = r - 1, c + 1
+        else:
+            if r == 7: c, up = c + 1, True
+            elif c == 0: r, up = r + 1, True
+            else: r, c = r + 1, c - 1
+    inv = [0] * 64
+    for i, n in enumerate(order):
+        inv[n] = i
+    assert order[:6] == [0, 1, 8, 16, 9, 2]
+    assert [mc.mc_zigzag_pos(n) for n in range(64)] == inv
 
 
 def test_quant_division_exact(mc):

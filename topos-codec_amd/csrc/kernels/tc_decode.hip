@@ -34,88 +34,199 @@ __device__ static uint32_t wave_crc32(const uint8_t* p, uint32_t n, const uint32
 }
 
 // ---------------------------------------------------------------------------
-// K5: rANS2 colour slice decoder. The 32-bit state is wave-uniform; each lane
-// holds one symbol's cumulative frequency so a slot is resolved with a single
-// compare + ballot. The payload is read through a 256-byte window held one
-// dword per lane (plus a prefetched next window). Coefficients are assembled
-// in registers (lane = natural index) and written as one 256-byte block.
+// K5: rANS2 colour slice decoder, one wavefront per slice.
+// The decode chain is serial (one 32-bit state per slice) and a wavefront
+// issues at most one instruction per four cycles, so the loop is written as a
+// short, nearly branch-free chain of wave-uniform (SGPR) arithmetic:
+//  - symbol search: each context row is held in two vector registers with
+//    symbol s in lane 63 - s: the cumulative frequency (unused symbols hold
+//    0xFFFF) and the entry f << 19 | s << 12 | cum. The lowest lane whose
+//    cumulative frequency is <= slot (s_ff1 of a compare mask) is the symbol;
+//  - renormalisation: the next payload bits wait MSB-first in a 64-bit
+//    scalar reservoir; taking k = 0..3 bytes is one 64-bit shift of
+//    (state : reservoir) by 8k with k from the state's leading zeros;
+//  - the reservoir is refilled with whole big-endian dwords from a 256-byte
+//    window held one dword per lane (the following window prefetched), at two
+//    points per (run, level) pair, which bounds what a pair can consume;
+//  - a run symbol of 63 (end of block) and a position overflow leave the pair
+//    loop through one compare; range errors are collected per block.
+// The payload must be readable 768 bytes past its end (window prefetch).
 // ---------------------------------------------------------------------------
-struct DecStream {
-    uint32_t x;
-    uint32_t pos;     // next byte index inside the payload
-    uint32_t n;       // payload bytes
-    uintptr_t base;   // payload address
-    uintptr_t wbase;  // 256-aligned address of the current window
-    uint32_t cur, nxt;
-    int32_t err;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+__device__ static inline uint32_t wuni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+struct RansRd {
+    uint32_t x;       // rANS state
+    uint64_t R;       // reservoir: next payload bits, MSB-first, rc valid
+    uint32_t rc;
+    uint32_t widx;    // next dword of `cur` to move into the reservoir
+    uint32_t nwin;    // windows fetched so far
+    uint32_t maxwin;  // windows that may be fetched (payload end + 256)
+    uint32_t loaded;  // payload bits moved into the reservoir
+    uint32_t cur;     // lane i: dword i of the current window, big-endian
+    uint32_t nxt;     // lane i: dword i of the next window, as stored
+    gcu32* wp;        // address of the window after `nxt`
 };
 
-__device__ static inline void ds_window_load(DecStream& D, uintptr_t w)
+__device__ static inline uint32_t bswap32(uint32_t v)
 {
-    D.wbase = w;
-    D.cur = *(const uint32_t*)(w + 4u * __lane_id());
-    D.nxt = *(const uint32_t*)(w + 256u + 4u * __lane_id());
+    return (v >> 24) | ((v >> 8) & 0xFF00u) | ((v << 8) & 0xFF0000u) | (v << 24);
 }
 
-__device__ static inline uint32_t ds_byte(DecStream& D)
+__device__ static inline void rr_refill(RansRd& D, uint32_t lane)
 {
-    const uintptr_t a = D.base + D.pos;
-    if (a >= D.wbase + 256u) {
-        D.cur = D.nxt;
-        D.wbase += 256u;
-        D.nxt = *(const uint32_t*)(D.wbase + 256u + 4u * __lane_id());
+    if (D.widx == 64u) {
+        // past the last window that may hold payload bytes the same window is
+        // read again: those bits are only consumed by a malformed stream,
+        // which the final length check rejects
+        D.cur = bswap32(D.nxt);
+        D.nxt = D.wp[lane];
+        if (D.nwin < D.maxwin) { D.wp += 64; }
+        D.nwin += 1u;
+        D.widx = 0u;
     }
-    const uint32_t off = (uint32_t)(a - D.wbase);
-    const uint32_t w = __builtin_amdgcn_readlane(D.cur, off >> 2);
-    D.pos += 1u;
-    return (w >> (8u * (off & 3u))) & 0xFFu;
+    const uint32_t w = __builtin_amdgcn_readlane(D.cur, D.widx);
+    D.R |= (uint64_t)w << (32u - D.rc);
+    D.rc += 32u;
+    D.loaded += 32u;
+    D.widx += 1u;
 }
 
-__device__ static inline void ds_renorm(DecStream& D)
+__device__ static inline void rr_need(RansRd& D, uint32_t bits, uint32_t lane)
 {
-    while (D.x < (1u << 23)) {
-        if (D.pos >= D.n) { D.err = -10; return; }
-        D.x = (D.x << 8) | ds_byte(D);
-    }
+    if (D.rc < bits) { rr_refill(D, lane); }
 }
 
-__device__ static inline uint32_t ds_sym(DecStream& D, const uint16_t* freq, const uint16_t* cum,
-                                         uint32_t nsym)
+// payload = first byte after the 4-byte initial state, n bytes
+__device__ static inline void rr_init(RansRd& D, uint32_t x0, const uint8_t* payload, uint32_t n, uint32_t lane)
 {
-    const uint32_t lane = __lane_id();
+    const uintptr_t a = (uintptr_t)payload;
+    const uintptr_t w0 = a & ~(uintptr_t)255u;
+    gcu32* wp = (gcu32*)w0;
+    D.x = x0;
+    D.cur = bswap32(wp[lane]);
+    D.nxt = wp[64u + lane];
+    D.wp = wp + 128;
+    D.nwin = 2u;
+    D.maxwin = (uint32_t)((a + n + 256u - w0 + 255u) >> 8);
+    const uint32_t off = (uint32_t)(a - w0);
+    const uint32_t mis = (off & 3u) * 8u;
+    D.widx = off >> 2;
+    const uint32_t w = __builtin_amdgcn_readlane(D.cur, D.widx);
+    D.R = ((uint64_t)w << 32) << mis;
+    D.rc = 32u - mis;
+    D.loaded = 32u - mis;
+    D.widx += 1u;
+}
+
+__device__ static inline void rr_norm(RansRd& D)
+{
+    // x > 0 here; k bytes with k = ceil((23 - floor(log2 x)) / 8) for x < 2^23
+    const uint32_t n8 = ((uint32_t)__builtin_clz(D.x) - 1u) & 0x18u;
+    const uint64_t t = ((uint64_t)D.x << 32) | (D.R >> 32);
+    D.x = (uint32_t)((t << n8) >> 32);
+    D.R <<= n8;
+    D.rc -= n8;
+}
+
+__device__ static inline uint32_t rr_sym(RansRd& D, uint32_t cumv, uint32_t entv)
+{
     const uint32_t slot = D.x & 4095u;
-    const uint32_t cv = lane < nsym ? (uint32_t)cum[lane] : 0xFFFFu;
-    const uint64_t m = __ballot(cv <= slot);
-    const uint32_t s = 63u - (uint32_t)__clzll(m);
-    const uint32_t f = freq[s], c = cum[s];
-    D.x = f * (D.x >> 12) + slot - c;
-    ds_renorm(D);
-    return s;
+    const uint32_t L = (uint32_t)__builtin_ctzll(__ballot(cumv <= slot));
+    const uint32_t e = __builtin_amdgcn_readlane(entv, L);
+    D.x = (e >> 19) * (D.x >> 12) + slot - (e & 0xFFFu);
+    rr_norm(D);
+    return (e >> 12) & 63u;
 }
 
-__device__ static inline uint32_t ds_chunk(DecStream& D, uint32_t w)
+__device__ static inline uint32_t rr_bits(RansRd& D, uint32_t w)
 {
     const uint32_t c = D.x & ((1u << w) - 1u);
     D.x >>= w;
-    ds_renorm(D);
+    rr_norm(D);
     return c;
 }
 
-__device__ static inline uint32_t ds_suffix(DecStream& D, uint32_t cat)
+// mapped magnitude of a category: 0 for 0, else 2^(cat-1) | (cat - 1 raw bits);
+// reservoir holds >= 24 bits on entry
+__device__ static inline uint32_t rr_mag(RansRd& D, uint32_t cat, uint32_t lane)
 {
-    uint32_t v = 0u;
-    if (cat > 1u) {
+    if (cat > 24u) {
         const uint32_t b = cat - 1u;
-        if (b > 23u) {
-            const uint32_t lo = ds_chunk(D, 16u);
-            if (D.err) { return 0u; }
-            const uint32_t hi = ds_chunk(D, b - 16u);
-            v = (hi << 16) | lo;
-        } else {
-            v = ds_chunk(D, b);
+        const uint32_t lo = rr_bits(D, 16u);
+        rr_need(D, 16u, lane);
+        const uint32_t hi = rr_bits(D, b - 16u);
+        return (1u << b) | (hi << 16) | lo;
+    }
+    const uint32_t w = max(cat, 1u) - 1u;
+    const uint32_t v = rr_bits(D, w);
+    return cat == 0u ? 0u : ((1u << w) | v);
+}
+
+// context rows of the level models as packed nibbles (row offset added by the caller):
+// position buckets by idx >> 3 (0, 1, 2, 2, 3, 3, 3, 3) and previous-category
+// buckets by min(cat, 7) (0, 1, 2, 2, 3, 3, 3, 3)
+#define TC_CTX_NIBBLES 0x33332210u
+
+template <uint32_t LM, uint32_t DM>
+__device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, const uint32_t* entv,
+                                            const tcg_dslice& S, uint32_t cols, int32_t* __restrict__ qbuf,
+                                            int32_t* sDcRow, uint32_t lane)
+{
+    constexpr uint32_t DC_N = DM ? 5u : 1u;
+    constexpr uint32_t RUN_ROW = DC_N;
+    constexpr uint32_t LVL0 = DC_N + 1u;
+    const uint32_t dc_max = S.dc_max, lv_max = S.lv_max;
+    const int64_t dlim = dc_max > 0x08000000u ? ((int64_t)1 << 27) : ((int64_t)1 << 25);
+    const uint32_t myscan = tc_zigzag_pos(lane);
+    const uint32_t run_cum = cumv[RUN_ROW], run_ent = entv[RUN_ROW];
+    int32_t* out = qbuf + (size_t)S.blk0 * 64u + lane;
+    uint32_t pdc = 0u;
+    int32_t left = 0;
+    for (uint32_t by = 0; by < S.h; ++by) {
+        for (uint32_t bx = 0; bx < cols; ++bx) {
+            const int32_t top_v = sDcRow[bx];
+            uint32_t drow = 0u;
+            if (DM) {
+                drow = (by == 0u && bx == 0u) ? 0u : 1u + ((TC_CTX_NIBBLES >> (min(pdc, 7u) * 4u)) & 0xFu);
+            }
+            rr_need(D, 32u, lane);
+            const uint32_t dcat = rr_sym(D, cumv[drow], entv[drow]);
+            rr_need(D, 32u, lane);
+            const uint32_t dm = rr_mag(D, dcat, lane);
+            const int32_t top = by ? (int32_t)wuni((uint32_t)top_v) : 0;
+            const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
+            const int64_t dv = (int64_t)pred + (int64_t)tc_unmap_signed(dm);
+            uint32_t bad = (dm > dc_max || dv > dlim || dv < -dlim) ? 1u : 0u;
+            const int32_t dc = (int32_t)dv;
+            if (lane == 0u) { sDcRow[bx] = dc; }
+            left = dc;
+            pdc = dcat;
+            uint32_t val = 0u;
+            uint32_t pos = 1u, mmax = 0u, rs;
+            uint32_t lrow = LM == 2u ? LVL0 + 4u : LVL0;
+            for (;;) {
+                rr_need(D, 32u, lane);
+                rs = rr_sym(D, run_cum, run_ent);
+                const uint32_t idx = pos + rs;
+                if (idx > 63u) { break; }
+                if (LM == 1u) { lrow = LVL0 + ((TC_CTX_NIBBLES >> ((idx >> 1) & 0x1Cu)) & 0xFu); }
+                const uint32_t cat = rr_sym(D, cumv[lrow], entv[lrow]);
+                rr_need(D, 24u, lane);
+                const uint32_t m = rr_mag(D, cat, lane);
+                mmax = max(mmax, m);
+                val = myscan == idx ? m : val;
+                if (LM == 2u) { lrow = LVL0 + ((TC_CTX_NIBBLES >> (min(cat, 7u) * 4u)) & 0xFu); }
+                pos = idx + 1u;
+            }
+            bad |= (rs != 63u || mmax > lv_max) ? 1u : 0u;
+            if (bad) { return -9; }
+            const int32_t v = lane == 0u ? dc : tc_unmap_signed(val);
+            out[((size_t)by * cols + bx) * 64u] = v;
         }
     }
-    return cat == 0u ? 0u : ((1u << (cat - 1u)) | v);
+    return 0;
 }
 
 __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict__ K,
@@ -143,13 +254,11 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
     const uint8_t* pl = (const uint8_t*)S.payload;
     const uint32_t n = S.size;
     int32_t rc = 0;
-    if (wave_crc32(pl, n, sTab, sX2n) != S.crc) { rc = -11; }
-    uint32_t lvl_model = 0u, dc_model = 0u, lvl_n = 1u, dc_n = 1u, prefix = 0u;
+    if (wuni(wave_crc32(pl, n, sTab, sX2n)) != S.crc) { rc = -11; }
+    uint32_t flags = 0u, lvl_model = 0u, dc_model = 0u, lvl_n = 1u, dc_n = 1u, prefix = 0u;
+    if (rc == 0 && n < 1u + 121u + 4u) { rc = -10; }
     if (rc == 0) {
-        if (n < 1u + 121u + 4u) { rc = -10; }
-    }
-    if (rc == 0) {
-        const uint32_t flags = pl[0];
+        flags = wuni(pl[0]);
         lvl_model = flags & 3u;
         dc_model = (flags >> 2) & 1u;
         if ((flags & 0xF8u) != 0u || lvl_model > 2u) { rc = -9; }
@@ -158,9 +267,9 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
         prefix = 1u + dc_n * 29u + 64u + lvl_n * 28u;
         if (rc == 0 && n < prefix + 4u) { rc = -10; }
     }
+    const uint32_t rows = dc_n + 1u + lvl_n;
     if (rc == 0) {
         // rows: [0, dc_n) dc, dc_n run, then lvl rows; lane r builds row r
-        const uint32_t rows = dc_n + 1u + lvl_n;
         if (lane < rows) {
             uint32_t off, nsym;
             if (lane < dc_n) { off = 1u + lane * 29u; nsym = 29u; }
@@ -171,71 +280,39 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
             if (tc_model_from_bytes(rb, nsym, sFreq[lane], sCum[lane]) != 0) { atomicOr(&sBad[0], 1); }
         }
         __syncthreads();
-        if (sBad[0] != 0) { rc = -9; }
-    }
-    DecStream D;
-    D.err = 0;
-    if (rc == 0) {
-        D.base = (uintptr_t)(pl + prefix);
-        D.n = n - prefix;
-        D.x = ((uint32_t)pl[prefix] << 24) | ((uint32_t)pl[prefix + 1u] << 16) |
-              ((uint32_t)pl[prefix + 2u] << 8) | (uint32_t)pl[prefix + 3u];
-        D.pos = 4u;
-        if (D.x < (1u << 23)) { rc = -9; }
-        ds_window_load(D, (D.base + D.pos) & ~(uintptr_t)255u);
+        if (wuni((uint32_t)sBad[0]) != 0u) { rc = -9; }
     }
     if (rc == 0) {
-        const uint32_t cols = P.bcols;
-        const uint32_t dc_max = S.dc_max;
-        const uint32_t lv_max = S.lv_max;
-        const int64_t dlim = dc_max > 0x08000000u ? ((int64_t)1 << 27) : ((int64_t)1 << 25);
-        const uint32_t run_row = dc_n, lvl_row0 = dc_n + 1u;
-        uint32_t pdc = 0u;
-        int32_t left = 0;
-        for (uint32_t by = 0; by < S.h && rc == 0; ++by) {
-            for (uint32_t bx = 0; bx < cols; ++bx) {
-                const uint32_t dctx = (by == 0u && bx == 0u) ? 0u : tc_ctx_dc(pdc, 1u);
-                const uint32_t drow = dc_model ? dctx : 0u;
-                uint32_t cat = ds_sym(D, sFreq[drow], sCum[drow], 29u);
-                if (D.err) { rc = D.err; break; }
-                uint32_t m = ds_suffix(D, cat);
-                if (D.err) { rc = D.err; break; }
-                if (m > dc_max) { rc = -9; break; }
-                const int32_t top = by ? sDcRow[bx] : 0;
-                const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
-                const int64_t dv = (int64_t)pred + (int64_t)tc_unmap_signed(m);
-                if (dv > dlim || dv < -dlim) { rc = -9; break; }
-                const int32_t dc = (int32_t)dv;
-                int32_t val = lane == 0u ? dc : 0;
-                __syncthreads();
-                if (lane == 0u) { sDcRow[bx] = dc; }
-                left = dc;
-                pdc = cat;
-                uint32_t pos = 1u, pcat = 0u, hp = 0u;
-                for (;;) {
-                    const uint32_t rs = ds_sym(D, sFreq[run_row], sCum[run_row], 64u);
-                    if (D.err) { rc = D.err; break; }
-                    if (rs == 63u) { break; }
-                    const uint32_t idx = pos + rs;
-                    if (idx > 63u) { rc = -9; break; }
-                    uint32_t lc = 0u;
-                    if (lvl_model == 1u) { lc = tc_ctx_pos(idx); }
-                    else if (lvl_model == 2u) { lc = tc_ctx_prev(pcat, hp); }
-                    cat = ds_sym(D, sFreq[lvl_row0 + lc], sCum[lvl_row0 + lc], 28u);
-                    if (D.err) { rc = D.err; break; }
-                    m = ds_suffix(D, cat);
-                    if (D.err) { rc = D.err; break; }
-                    if (m > lv_max) { rc = -9; break; }
-                    if (lane == (uint32_t)kZigzag[idx]) { val = tc_unmap_signed(m); }
-                    pcat = cat;
-                    hp = 1u;
-                    pos = idx + 1u;
-                }
-                if (rc != 0) { break; }
-                qbuf[((size_t)S.blk0 + (size_t)by * cols + bx) * 64u + lane] = val;
-            }
+        // symbol s of row r sits in lane 63 - s
+        uint32_t cumv[11], entv[11];
+        const uint32_t sym = 63u - lane;
+#pragma unroll
+        for (uint32_t r = 0; r < 11u; ++r) {
+            const uint32_t nsym = r < dc_n ? 29u : (r == dc_n ? 64u : 28u);
+            const bool live = r < rows && sym < nsym;
+            const uint32_t c = live ? (uint32_t)sCum[r][sym] : 0xFFFFu;
+            const uint32_t f = live ? (uint32_t)sFreq[r][sym] : 0u;
+            cumv[r] = c;
+            entv[r] = (f << 19) | (sym << 12) | (c & 0xFFFu);
         }
-        if (rc == 0 && D.pos != D.n) { rc = -9; }
+        const uint32_t x0 = wuni(((uint32_t)pl[prefix] << 24) | ((uint32_t)pl[prefix + 1u] << 16) |
+                                 ((uint32_t)pl[prefix + 2u] << 8) | (uint32_t)pl[prefix + 3u]);
+        if (x0 < (1u << 23)) { rc = -9; }
+        if (rc == 0) {
+            RansRd D;
+            rr_init(D, x0, pl + prefix + 4u, n - prefix - 4u, lane);
+            const uint32_t cols = P.bcols;
+            switch (flags & 7u) {
+            case 0u: rc = slice_decode_body<0u, 0u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 1u: rc = slice_decode_body<1u, 0u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 2u: rc = slice_decode_body<2u, 0u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 4u: rc = slice_decode_body<0u, 1u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 5u: rc = slice_decode_body<1u, 1u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            default: rc = slice_decode_body<2u, 1u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            }
+            // the stream must be consumed exactly
+            if (rc == 0 && D.loaded - D.rc != 8u * (n - prefix - 4u)) { rc = -9; }
+        }
     }
     if (lane == 0u) { status[blockIdx.x] = rc == 0 ? 0 : 1; }
 }

@@ -180,56 +180,67 @@ __global__ __launch_bounds__(256) void k_fdct_quant(const tcg_qctx* __restrict__
 // 11..15 dc-prev, 16 run
 #define NROWS 17
 
+// Backward-pass byte sink. Bytes are produced last-to-first; the most recent
+// byte sits at the lowest address. Complete dwords are placed into a vector
+// register (lane 63 - (k & 63) holds dword k of the current 256-byte chunk,
+// counting down from the end of the slice scratch region) by a lane select;
+// a full chunk leaves as one coalesced 256-byte store.
 struct EncEmit {
     uint8_t* end;     // one past the last byte of the slice scratch region
-    uint32_t n;       // bytes emitted so far
-    uint32_t acc;     // up to 3 pending bytes (most recent in the low byte)
-    uint32_t limit;   // emission cap (bytes)
-    uint32_t over;
+    uint32_t n;       // bytes emitted so far (uniform)
+    uint32_t acc;     // up to 3 pending bytes, most recent in the low byte (uniform)
+    uint32_t limit;   // emission cap in bytes (uniform)
+    uint32_t over;    // cap exceeded (uniform)
+    uint32_t obuf;    // lane i: dword of the current chunk
 };
 
-__device__ static inline void emit_byte(EncEmit& E, uint32_t b)
+__device__ static inline void emit_byte(EncEmit& E, uint32_t b, uint32_t lane)
 {
     E.acc = (E.acc << 8) | (b & 0xFFu);
     E.n += 1u;
     if ((E.n & 3u) == 0u) {
-        if (E.n <= E.limit) {
-            if (__lane_id() == 0u) { *(uint32_t*)(E.end - E.n) = E.acc; }
-        } else {
-            E.over = 1u;
-        }
+        const uint32_t k = (E.n >> 2) - 1u;  // dword index counted from the end
+        const uint32_t slot = 63u - (k & 63u);
+        E.obuf = lane == slot ? E.acc : E.obuf;
         E.acc = 0u;
+        if (slot == 0u) {
+            if (E.n <= E.limit) {
+                *(uint32_t*)(E.end - E.n + 4u * lane) = E.obuf;
+            } else {
+                E.over = 1u;
+            }
+        }
     }
 }
 
-__device__ static inline uint32_t put_sym(uint32_t x, uint32_t fc, uint32_t mag, uint32_t sh, EncEmit& E)
+// one symbol with the constants of tc_rans_enc_consts (built in LDS per slice)
+__device__ static inline uint32_t put_sym(uint32_t x, uint32_t A, uint32_t B, uint32_t X, EncEmit& E,
+                                          uint32_t lane)
 {
-    const uint32_t f = fc >> 16, cum = fc & 0xFFFFu;
-    const uint32_t xmax = f << 19;
-    while (x >= xmax) { emit_byte(E, x); x >>= 8; }
-    const uint32_t q = f == 1u ? x : (__umulhi(x, mag) >> sh);
-    return (q << 12) + (x - q * f) + cum;
+    while (x >= X) { emit_byte(E, x, lane); x >>= 8; }
+    const uint32_t q = __umulhi(x << 1, A) >> (B >> 24);
+    return x + q * ((B >> 12) & 0xFFFu) + (B & 0xFFFu);
 }
 
-__device__ static inline uint32_t put_chunk(uint32_t x, uint32_t w, uint32_t c, EncEmit& E)
+__device__ static inline uint32_t put_chunk(uint32_t x, uint32_t w, uint32_t c, EncEmit& E, uint32_t lane)
 {
     const uint32_t thr = 1u << (31u - w);
-    while (x >= thr) { emit_byte(E, x); x >>= 8; }
-    return (x << w) | (c & ((1u << w) - 1u));
+    while (x >= thr) { emit_byte(E, x, lane); x >>= 8; }
+    return (x << w) | c;
 }
 
-__device__ static inline uint32_t put_suffix(uint32_t x, uint32_t cat, uint32_t m, EncEmit& E)
+// suffix of a magnitude category: cat - 1 raw bits of m (> 23 bits: the top
+// bits first as one chunk, then the low 16)
+__device__ static inline uint32_t put_suffix_wide(uint32_t x, uint32_t b, uint32_t m, EncEmit& E,
+                                                  uint32_t lane)
 {
-    uint32_t b = cat > 1u ? cat - 1u : 0u;
-    if (b == 0u || b > 27u) { return x; }
-    if (b > 23u) {
-        x = put_chunk(x, b - 16u, m >> 16, E);
-        b = 16u;
-    }
-    return put_chunk(x, b, m, E);
+    x = put_chunk(x, b - 16u, (m >> 16) & ((1u << (b - 16u)) - 1u), E, lane);
+    return put_chunk(x, 16u, m & 0xFFFFu, E, lane);
 }
 
 __device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+#define ENC_GROUP 8
 
 __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict__ K,
                                                       const tcg_eplane* __restrict__ planes,
@@ -245,9 +256,9 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
     __shared__ uint16_t sFreq[NROWS][64];
     __shared__ uint16_t sCum[NROWS][65];
     __shared__ double sCost[5];
-    __shared__ uint32_t sTabFC[11 * 64];   // packed freq << 16 | cum: dc rows, run, lvl rows
-    __shared__ uint32_t sTabMag[11 * 64];
-    __shared__ uint8_t sTabSh[11 * 64];
+    __shared__ uint32_t sTabA[11 * 64];
+    __shared__ uint32_t sTabB[11 * 64];
+    __shared__ uint32_t sTabX[11 * 64];
     __shared__ uint8_t sPrefix[TCG_PREFIX_MAX];
 
     const uint32_t lane = threadIdx.x;
@@ -260,47 +271,65 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
     __syncthreads();
 
     // ---------------- forward pass ----------------
+    // One wavefront: LDS accesses of this wave complete in program order, so
+    // the row of DC values and the histograms need no barrier per block.
     uint32_t viol = 0u;
     uint32_t prev_dcc = 0u;
     int32_t left = 0;
     const uint64_t below = lane == 0u ? 0ull : ((1ull << lane) - 1ull);
-    for (uint32_t by = 0; by < S.h; ++by) {
-        for (uint32_t bx = 0; bx < cols; ++bx) {
-            const uint32_t b = by * cols + bx;
-            const uint32_t blk = S.blk0 + b;
-            const int32_t q = qbuf[(size_t)blk * 64u + lane];
-            const uint64_t nz = __ballot(q != 0) & ~1ull;
-            const int32_t q0 = __builtin_amdgcn_readlane(q, 0);
-            const int32_t top = by ? sDcRow[bx] : 0;
-            const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
-            uint32_t dcm = tc_map_signed(q0 - pred);
-            if (dcm > 0x0FFFFFFFu) { dcm = 0x0FFFFFFFu; viol = 1u; }
-            const uint32_t dcc = tc_bitlen(dcm);
-            if (lane == 0u) {
-                dcmbuf[blk] = dcm;
-                atomicAdd(&sH[H_DC + dcc], 1u);
-                atomicAdd(&sH[H_DPREV + (b == 0u ? 0u : tc_ctx_dc(prev_dcc, 1u)) * 29u + dcc], 1u);
-                atomicAdd(&sH[H_RUN + 63u], 1u);
-            }
-            prev_dcc = dcc;
-            left = q0;
-            __syncthreads();
-            if (lane == 0u) { sDcRow[bx] = q0; }
-            const uint32_t lm_raw = tc_map_signed(q);
-            const uint32_t lm = min(lm_raw, 0x07FFFFFFu);
-            const uint32_t cat = tc_bitlen(lm);
-            const uint64_t bl = nz & below;
-            const uint32_t pp = bl ? 63u - (uint32_t)__clzll(bl) : 0u;
-            const uint32_t pcat = (uint32_t)__shfl((int)cat, (int)pp);
-            if (lane != 0u && q != 0) {
-                if (lm_raw > 0x07FFFFFFu) { viol = 1u; }
-                const uint32_t run = lane - pp - 1u;
-                atomicAdd(&sH[H_RUN + run], 1u);
-                atomicAdd(&sH[H_LVL + cat], 1u);
-                atomicAdd(&sH[H_LPOS + tc_ctx_pos(lane) * 28u + cat], 1u);
-                atomicAdd(&sH[H_LPREV + tc_ctx_prev(pcat, pp != 0u) * 28u + cat], 1u);
-            }
+    const uint32_t nblk = S.nblk;
+    const int32_t* qs = qbuf + (size_t)S.blk0 * 64u + lane;
+    uint32_t bx = 0u, by = 0u;
+    auto fwd_block = [&](int32_t q, uint32_t b) {
+        const uint64_t nz = __ballot(q != 0) & ~1ull;
+        const int32_t q0 = __builtin_amdgcn_readlane(q, 0);
+        const int32_t top = by ? (int32_t)wave_u32((uint32_t)sDcRow[bx]) : 0;
+        const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
+        uint32_t dcm = tc_map_signed(q0 - pred);
+        if (dcm > 0x0FFFFFFFu) { dcm = 0x0FFFFFFFu; viol = 1u; }
+        const uint32_t dcc = tc_bitlen(dcm);
+        if (lane == 0u) {
+            dcmbuf[S.blk0 + b] = dcm;
+            atomicAdd(&sH[H_DC + dcc], 1u);
+            atomicAdd(&sH[H_DPREV + (b == 0u ? 0u : tc_ctx_dc(prev_dcc, 1u)) * 29u + dcc], 1u);
+            atomicAdd(&sH[H_RUN + 63u], 1u);
+            sDcRow[bx] = q0;
         }
+        prev_dcc = dcc;
+        left = q0;
+        const uint32_t lm_raw = tc_map_signed(q);
+        const uint32_t lm = min(lm_raw, 0x07FFFFFFu);
+        const uint32_t cat = tc_bitlen(lm);
+        const uint64_t bl = nz & below;
+        const uint32_t pp = bl ? 63u - (uint32_t)__clzll(bl) : 0u;
+        const uint32_t pcat = (uint32_t)__shfl((int)cat, (int)pp);
+        if (lane != 0u && q != 0) {
+            if (lm_raw > 0x07FFFFFFu) { viol = 1u; }
+            const uint32_t run = lane - pp - 1u;
+            atomicAdd(&sH[H_RUN + run], 1u);
+            atomicAdd(&sH[H_LVL + cat], 1u);
+            atomicAdd(&sH[H_LPOS + tc_ctx_pos(lane) * 28u + cat], 1u);
+            atomicAdd(&sH[H_LPREV + tc_ctx_prev(pcat, pp != 0u) * 28u + cat], 1u);
+        }
+        if (++bx == cols) { bx = 0u; ++by; }
+    };
+    // groups of ENC_GROUP blocks, the next group's loads in flight while the
+    // current one is processed
+    int32_t qa[ENC_GROUP], qb[ENC_GROUP];
+#pragma unroll
+    for (uint32_t j = 0; j < ENC_GROUP; ++j) { qa[j] = j < nblk ? qs[(size_t)j * 64u] : 0; }
+    for (uint32_t g = 0; g < nblk; g += ENC_GROUP) {
+#pragma unroll
+        for (uint32_t j = 0; j < ENC_GROUP; ++j) {
+            const uint32_t b = g + ENC_GROUP + j;
+            qb[j] = b < nblk ? qs[(size_t)b * 64u] : 0;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < ENC_GROUP; ++j) {
+            if (g + j < nblk) { fwd_block(qa[j], g + j); }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < ENC_GROUP; ++j) { qa[j] = qb[j]; }
     }
     __syncthreads();
     const uint32_t any_viol = __ballot(viol != 0u) != 0ull;
@@ -348,6 +377,8 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         if (sCost[2] < best) { lvl_model = 2u; }
         if (sCost[4] < sCost[3]) { dc_model = 1u; }
     }
+    lvl_model = wave_u32(lvl_model);
+    dc_model = wave_u32(dc_model);
     const uint32_t lvl_n = lvl_model == 1u ? 4u : (lvl_model == 2u ? 5u : 1u);
     const uint32_t dc_n = dc_model ? 5u : 1u;
     const uint32_t lvl_r0 = lvl_model == 1u ? 1u : (lvl_model == 2u ? 5u : 0u);
@@ -364,41 +395,51 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         for (uint32_t i = lane; i < lvl_n * 28u; i += 64u) {
             sPrefix[1u + dc_n * 29u + 64u + i] = sRowB[lvl_r0 + i / 28u][i % 28u];
         }
-        // packed symbol tables
+        // per-symbol encoder constants
         for (uint32_t slot = 0; slot < dc_n + 1u + lvl_n; ++slot) {
             uint32_t row, nsym;
             if (slot < dc_n) { row = dc_r0 + slot; nsym = 29u; }
             else if (slot == dc_n) { row = 16u; nsym = 64u; }
             else { row = lvl_r0 + (slot - dc_n - 1u); nsym = 28u; }
             if (lane < nsym) {
-                const uint32_t f = sFreq[row][lane];
-                uint32_t mg, sh;
-                tc_rans_recip(f, &mg, &sh);
-                sTabFC[slot * 64u + lane] = (f << 16) | sCum[row][lane];
-                sTabMag[slot * 64u + lane] = mg;
-                sTabSh[slot * 64u + lane] = (uint8_t)sh;
+                uint32_t A, B, X;
+                tc_rans_enc_consts(sFreq[row][lane], sCum[row][lane], &A, &B, &X);
+                sTabA[slot * 64u + lane] = A;
+                sTabB[slot * 64u + lane] = B;
+                sTabX[slot * 64u + lane] = X;
             }
         }
     }
     __syncthreads();
 
     // ---------------- backward pass ----------------
+    // Per block, lanes gather their pair's table entries (level and run) and
+    // suffix; the serial chain then walks the nonzero positions downwards with
+    // v_readlane + scalar state arithmetic.
     const uint32_t run_slot = dc_n;
     const uint32_t lvl_slot0 = dc_n + 1u;
-    const uint32_t eob_fc = wave_u32(sTabFC[run_slot * 64u + 63u]);
-    const uint32_t eob_mg = wave_u32(sTabMag[run_slot * 64u + 63u]);
-    const uint32_t eob_sh = wave_u32(sTabSh[run_slot * 64u + 63u]);
+    const uint32_t eobA = wave_u32(sTabA[run_slot * 64u + 63u]);
+    const uint32_t eobB = wave_u32(sTabB[run_slot * 64u + 63u]);
+    const uint32_t eobX = wave_u32(sTabX[run_slot * 64u + 63u]);
     EncEmit E;
     E.end = scratch + S.scratch + S.scratch_cap;
     E.n = 0u;
     E.acc = 0u;
     E.limit = S.scratch_cap - TCG_PREFIX_MAX - 8u;
     E.over = 0u;
+    E.obuf = 0u;
     uint32_t x = 1u << 23;
-    const uint32_t nblk = S.nblk;
+    // one block ahead: its coefficients and the DC value of the block after it
+    int32_t q_nx = nblk ? qs[(size_t)(nblk - 1u) * 64u] : 0;
+    uint32_t d_cur = nblk ? dcmbuf[S.blk0 + nblk - 1u] : 0u;
+    uint32_t d_nx = nblk >= 2u ? dcmbuf[S.blk0 + nblk - 2u] : 0u;
     for (uint32_t bb = nblk; bb-- > 0u;) {
-        const uint32_t blk = S.blk0 + bb;
-        const int32_t q = qbuf[(size_t)blk * 64u + lane];
+        const int32_t q = q_nx;
+        const uint32_t dcm = wave_u32(d_cur);
+        const uint32_t dcm_prev = wave_u32(d_nx);
+        if (bb >= 1u) { q_nx = qs[(size_t)(bb - 1u) * 64u]; }
+        d_cur = d_nx;
+        d_nx = bb >= 2u ? dcmbuf[S.blk0 + bb - 2u] : 0u;
         const uint64_t nz = __ballot(q != 0) & ~1ull;
         const uint32_t lm = min(tc_map_signed(q), 0x07FFFFFFu);
         const uint32_t cat = tc_bitlen(lm);
@@ -412,38 +453,54 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         const uint32_t active = (lane != 0u && q != 0) ? 1u : 0u;
         const uint32_t li = active ? (lvl_slot0 + lc) * 64u + cat : 0u;
         const uint32_t ri = active ? run_slot * 64u + run : 0u;
-        const uint32_t l_fc = sTabFC[li], l_mg = sTabMag[li], l_sh = sTabSh[li];
-        const uint32_t r_fc = sTabFC[ri], r_mg = sTabMag[ri], r_sh = sTabSh[ri];
-        const uint32_t dcm = wave_u32(dcmbuf[blk]);
-        const uint32_t dcm_prev = bb ? wave_u32(dcmbuf[blk - 1u]) : 0u;
+        const uint32_t lA = sTabA[li], lB = sTabB[li], lX = sTabX[li];
+        const uint32_t rA = sTabA[ri], rB = sTabB[ri], rX = sTabX[ri];
+        // suffix: width w = cat - 1 (0 when cat <= 1), value = low w bits of lm
+        const uint32_t w = cat > 1u ? cat - 1u : 0u;
+        const uint32_t sv = (w <= 23u ? (lm & ((1u << w) - 1u)) : 0u) | (w << 24);
 
-        x = put_sym(x, eob_fc, eob_mg, eob_sh, E);
+        x = put_sym(x, eobA, eobB, eobX, E, lane);
         uint64_t m = nz;
         while (m) {
             const uint32_t p = 63u - (uint32_t)__clzll(m);
             m &= ~(1ull << p);
-            const uint32_t pcat_p = __builtin_amdgcn_readlane(cat, p);
-            const uint32_t plm = __builtin_amdgcn_readlane(lm, p);
-            x = put_suffix(x, pcat_p, plm, E);
-            x = put_sym(x, __builtin_amdgcn_readlane(l_fc, p), __builtin_amdgcn_readlane(l_mg, p),
-                        __builtin_amdgcn_readlane(l_sh, p), E);
-            x = put_sym(x, __builtin_amdgcn_readlane(r_fc, p), __builtin_amdgcn_readlane(r_mg, p),
-                        __builtin_amdgcn_readlane(r_sh, p), E);
+            const uint32_t s = __builtin_amdgcn_readlane(sv, p);
+            const uint32_t sw = s >> 24;
+            if (sw <= 23u) {
+                x = put_chunk(x, sw, s & 0xFFFFFFu, E, lane);
+            } else {
+                x = put_suffix_wide(x, sw, __builtin_amdgcn_readlane(lm, p), E, lane);
+            }
+            x = put_sym(x, __builtin_amdgcn_readlane(lA, p), __builtin_amdgcn_readlane(lB, p),
+                        __builtin_amdgcn_readlane(lX, p), E, lane);
+            x = put_sym(x, __builtin_amdgcn_readlane(rA, p), __builtin_amdgcn_readlane(rB, p),
+                        __builtin_amdgcn_readlane(rX, p), E, lane);
         }
         const uint32_t dsym = tc_bitlen(dcm);
-        x = put_suffix(x, dsym, dcm, E);
+        if (dsym > 1u) {
+            const uint32_t dw = dsym - 1u;
+            if (dw <= 23u) { x = put_chunk(x, dw, dcm & ((1u << dw) - 1u), E, lane); }
+            else { x = put_suffix_wide(x, dw, dcm, E, lane); }
+        }
         const uint32_t dctx = dc_model ? tc_ctx_dc(tc_bitlen(dcm_prev), bb != 0u) : 0u;
         const uint32_t di = dctx * 64u + dsym;
-        x = put_sym(x, wave_u32(sTabFC[di]), wave_u32(sTabMag[di]), wave_u32(sTabSh[di]), E);
+        x = put_sym(x, wave_u32(sTabA[di]), wave_u32(sTabB[di]), wave_u32(sTabX[di]), E, lane);
         if (E.over) { break; }
     }
     // ---------------- flush ----------------
-    // emitted bytes occupy [end - n, end); pending (n & 3) bytes go first
+    // emitted bytes occupy [end - n, end): whole dwords of the partial chunk
+    // from the vector register, then the (n & 3) pending bytes, the state and
+    // the prefix in front.
     const uint32_t n = E.n;
     const uint32_t ok_len = (E.over == 0u && n <= E.limit) ? 1u : 0u;
     if (ok_len) {
-        const uint32_t r = n & 3u;
+        const uint32_t nd = n >> 2;           // whole dwords
+        const uint32_t part = nd & 63u;       // dwords in the unfinished chunk
         uint8_t* end = E.end;
+        if (part != 0u && lane >= 64u - part) {
+            *(uint32_t*)(end - 256u * ((nd >> 6) + 1u) + 4u * lane) = E.obuf;
+        }
+        const uint32_t r = n & 3u;
         if (lane < r) { end[-(int64_t)n + (int64_t)lane] = (uint8_t)(E.acc >> (8u * lane)); }
         uint8_t* st = end - n - 4;
         if (lane < 4u) { st[lane] = (uint8_t)(x >> (24u - 8u * lane)); }

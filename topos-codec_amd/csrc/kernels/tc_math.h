@@ -62,6 +62,30 @@ TC_HD void tc_rans_recip(uint32_t f, uint32_t* mag, uint32_t* sh)
     *sh = l - 1u;
 }
 
+/* Encoder constants of one symbol with frequency f (1..4096) and cumulative
+ * frequency cum, for the update x' = (x / f) * 4096 + x % f + cum:
+ *   A = ceil(2^(31+l) / f) with l = ceil(log2 f); since x < f * 2^19 <= 2^31
+ *       after renormalisation, floor(x / f) = umulhi(2x, A) >> l exactly
+ *       (A * f - 2^(31+l) < f <= 2^l, so 2x * (A*f - 2^(31+l)) < 2^(32+l));
+ *       f = 1 gives A = 2^31, l = 0, i.e. x itself;
+ *   B = cum | (4096 - f) << 12 | l << 24, so x' = x + q * (4096 - f) + cum;
+ *   X = f << 19, the bound the state must be below before the update. */
+TC_HD void tc_rans_enc_consts(uint32_t f, uint32_t cum, uint32_t* A, uint32_t* B, uint32_t* X)
+{
+    uint32_t l = 0u;
+    while ((1u << l) < f) { ++l; }
+    const uint64_t num = (uint64_t)1 << (31u + l);
+    *A = f == 0u ? 0u : (uint32_t)((num + f - 1u) / f);
+    *B = (cum & 0xFFFu) | (((4096u - f) & 0xFFFu) << 12) | (l << 24);
+    *X = f << 19;
+}
+
+TC_HD uint32_t tc_rans_enc_update(uint32_t x, uint32_t A, uint32_t B)
+{
+    const uint32_t q = (uint32_t)(((uint64_t)(x << 1) * A) >> 32) >> (B >> 24);
+    return x + q * ((B >> 12) & 0xFFFu) + (B & 0xFFFu);
+}
+
 /* Exact floor(n / d) for n * d < 2^51 via m = floor(2^51 / d) + 1:
  * floor(n / d) = (n * m) >> 51 (the quantiser's divide, bit depth <= 12). */
 TC_HD uint64_t tc_qdiv_magic(uint32_t d) { return (((uint64_t)1 << 51) / d) + 1u; }
@@ -161,6 +185,17 @@ TC_HD uint32_t tc_map_signed(int32_t v)
 TC_HD int32_t tc_unmap_signed(uint32_t m)
 {
     return (m & 1u) == 0u ? (int32_t)(m >> 1) : -(int32_t)(((uint64_t)m + 1u) >> 1);
+}
+
+/* Scan (zigzag) position of natural coefficient index n = 8r + c in the
+ * 8x8 zigzag order that starts along the first row: diagonal d = r + c holds
+ * positions from d(d+1)/2 (d < 8) or 64 - (15-d)(16-d)/2 (d >= 8); even
+ * diagonals run bottom-left to top-right, odd ones top-right to bottom-left. */
+TC_HD uint32_t tc_zigzag_pos(uint32_t n)
+{
+    const uint32_t r = n >> 3, c = n & 7u, d = r + c;
+    if (d < 8u) { return d * (d + 1u) / 2u + ((d & 1u) ? r : c); }
+    return 64u - (15u - d) * (16u - d) / 2u + ((d & 1u) ? 7u - c : 7u - r);
 }
 
 /* context buckets of the V7-R2 conditional models */
