@@ -49,7 +49,37 @@ int32_t tcx_decode_batch_dev(uint32_t n, const uint8_t* const* host_packets, con
                              const uint64_t* dev_packets, const uint64_t* dev_planes,
                              const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs);
 
-/* per-kernel times of the last encode / decode launch sequence (ms), measured
+/* One serving step: an encode batch and a decode batch run concurrently, each
+ * on its own stream (either may be NULL or empty); returns when both are done.
+ * Fields mean what the arguments of tcx_encode_batch_dev / tcx_decode_batch_dev
+ * mean. Returns the first non-OK status of the encode batch, else of the
+ * decode batch. */
+typedef struct tcx_enc_batch {
+    const topos_frame_config* cfgs;
+    uint32_t n;
+    const uint64_t* dev_planes;
+    const size_t* dev_strides;
+    const uint64_t* dev_out;
+    const size_t* out_caps;
+    uint32_t* out_sizes;
+    int32_t* rcs;
+    topos_frame_stats* stats;
+} tcx_enc_batch;
+
+typedef struct tcx_dec_batch {
+    uint32_t n;
+    const uint8_t* const* host_packets;
+    const size_t* sizes;
+    const uint64_t* dev_packets;
+    const uint64_t* dev_planes;
+    const size_t* dev_strides;
+    topos_frame_output* infos;
+    int32_t* rcs;
+} tcx_dec_batch;
+
+int32_t tcx_serve_step(const tcx_enc_batch* enc, const tcx_dec_batch* dec);
+
+/* per-kernel times of the last finished encode / decode launch sequence (ms), measured
  * with HIP events on the launch stream when enabled: fdct, slice_enc, scan,
  * emit, slice_dec, idct, deblock */
 void tcx_set_timing(int on);

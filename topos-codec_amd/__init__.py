@@ -55,6 +55,7 @@ def load_library(path=LIB_PATH):
     lib.tcx_encode_batch_dev.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
     lib.tcx_decode_batch_dev.argtypes = [ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
     lib.tcx_copy.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t]
+    lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
     del u8p
@@ -224,44 +225,86 @@ class DeviceCodec(ToposCodec):
         self._lib.tcx_get_timing(t)
         return dict(zip(["fdct", "slice_enc", "scan", "emit", "slice_dec", "idct", "deblock"], list(t)))
 
+    @staticmethod
+    def _enc_args(cfgs, plane_addrs, out_bufs, strides):
+        n = len(cfgs)
+        a = {"n": n, "cfgs": (abi.FrameConfig * n)(*cfgs), "planes": (ctypes.c_uint64 * (4 * n))(),
+             "strides": (ctypes.c_size_t * (4 * n))(), "outs": (ctypes.c_uint64 * n)(*[b.addr for b in out_bufs]),
+             "caps": (ctypes.c_size_t * n)(*[b.nbytes for b in out_bufs]), "sizes": (ctypes.c_uint32 * n)(),
+             "rcs": (ctypes.c_int32 * n)(), "stats": (abi.FrameStats * n)()}
+        for i, pa in enumerate(plane_addrs):
+            for p, addr in enumerate(pa):
+                a["planes"][4 * i + p] = addr
+                if strides is not None:
+                    a["strides"][4 * i + p] = strides[i][p]
+        return a
+
+    @staticmethod
+    def _dec_args(host_packets, dev_packets, plane_addrs, strides):
+        n = len(host_packets)
+        keep = [np.frombuffer(p, np.uint8) for p in host_packets]
+        a = {"n": n, "keep": keep, "hp": (ctypes.c_void_p * n)(*[k.ctypes.data for k in keep]),
+             "sizes": (ctypes.c_size_t * n)(*[len(p) for p in host_packets]),
+             "dp": (ctypes.c_uint64 * n)(*[b.addr for b in dev_packets]),
+             "planes": (ctypes.c_uint64 * (4 * n))(), "strides": (ctypes.c_size_t * (4 * n))(),
+             "infos": (abi.FrameOutput * n)(), "rcs": (ctypes.c_int32 * n)()}
+        for i, pa in enumerate(plane_addrs):
+            for p, addr in enumerate(pa):
+                a["planes"][4 * i + p] = addr
+                if strides is not None:
+                    a["strides"][4 * i + p] = strides[i][p]
+        return a
+
     def encode_batch_dev(self, cfgs, plane_addrs, out_bufs, strides=None):
         """cfgs: list of FrameConfig; plane_addrs: per frame list of 3-4 device addresses."""
-        n = len(cfgs)
-        carr = (abi.FrameConfig * n)(*cfgs)
-        planes = (ctypes.c_uint64 * (4 * n))()
-        strd = (ctypes.c_size_t * (4 * n))()
-        for i, pa in enumerate(plane_addrs):
-            for p, a in enumerate(pa):
-                planes[4 * i + p] = a
-                if strides is not None:
-                    strd[4 * i + p] = strides[i][p]
-        outs = (ctypes.c_uint64 * n)(*[b.addr for b in out_bufs])
-        caps = (ctypes.c_size_t * n)(*[b.nbytes for b in out_bufs])
-        sizes = (ctypes.c_uint32 * n)()
-        rcs = (ctypes.c_int32 * n)()
-        stats = (abi.FrameStats * n)()
-        st = int(self._lib.tcx_encode_batch_dev(carr, n, planes, strd, outs, caps, sizes, rcs, stats))
-        return st, list(sizes), list(rcs), stats
+        a = self._enc_args(cfgs, plane_addrs, out_bufs, strides)
+        st = int(self._lib.tcx_encode_batch_dev(a["cfgs"], a["n"], a["planes"], a["strides"], a["outs"], a["caps"],
+                                                a["sizes"], a["rcs"], a["stats"]))
+        return st, list(a["sizes"]), list(a["rcs"]), a["stats"]
 
     def decode_batch_dev(self, host_packets, dev_packets, plane_addrs, strides=None):
-        n = len(host_packets)
-        hp = (ctypes.c_void_p * n)()
-        keep = [np.frombuffer(p, np.uint8) for p in host_packets]
-        for i, k in enumerate(keep):
-            hp[i] = k.ctypes.data
-        sizes = (ctypes.c_size_t * n)(*[len(p) for p in host_packets])
-        dp = (ctypes.c_uint64 * n)(*[b.addr for b in dev_packets])
-        planes = (ctypes.c_uint64 * (4 * n))()
-        strd = (ctypes.c_size_t * (4 * n))()
-        for i, pa in enumerate(plane_addrs):
-            for p, a in enumerate(pa):
-                planes[4 * i + p] = a
-                if strides is not None:
-                    strd[4 * i + p] = strides[i][p]
-        infos = (abi.FrameOutput * n)()
-        rcs = (ctypes.c_int32 * n)()
-        st = int(self._lib.tcx_decode_batch_dev(n, hp, sizes, dp, planes, strd, infos, rcs))
-        return st, list(rcs), infos
+        a = self._dec_args(host_packets, dev_packets, plane_addrs, strides)
+        st = int(self._lib.tcx_decode_batch_dev(a["n"], a["hp"], a["sizes"], a["dp"], a["planes"], a["strides"],
+                                                a["infos"], a["rcs"]))
+        return st, list(a["rcs"]), a["infos"]
+
+    def serve_step(self, enc=None, dec=None):
+        """One serving step: enc = (cfgs, plane_addrs, out_bufs[, strides]) and
+        dec = (host_packets, dev_packets, plane_addrs[, strides]) run concurrently.
+        Returns (status, (sizes, rcs, stats) or None, (rcs, infos) or None)."""
+        ea = self._enc_args(*(list(enc) + [None] * (4 - len(enc)))) if enc else None
+        da = self._dec_args(*(list(dec) + [None] * (4 - len(dec)))) if dec else None
+        eb = TcxEncBatch()
+        db = TcxDecBatch()
+        if ea:
+            eb.cfgs, eb.n = ctypes.cast(ea["cfgs"], ctypes.c_void_p), ea["n"]
+            eb.dev_planes, eb.dev_strides = ctypes.cast(ea["planes"], ctypes.c_void_p), ctypes.cast(ea["strides"], ctypes.c_void_p)
+            eb.dev_out, eb.out_caps = ctypes.cast(ea["outs"], ctypes.c_void_p), ctypes.cast(ea["caps"], ctypes.c_void_p)
+            eb.out_sizes, eb.rcs = ctypes.cast(ea["sizes"], ctypes.c_void_p), ctypes.cast(ea["rcs"], ctypes.c_void_p)
+            eb.stats = ctypes.cast(ea["stats"], ctypes.c_void_p)
+        if da:
+            db.n, db.host_packets = da["n"], ctypes.cast(da["hp"], ctypes.c_void_p)
+            db.sizes, db.dev_packets = ctypes.cast(da["sizes"], ctypes.c_void_p), ctypes.cast(da["dp"], ctypes.c_void_p)
+            db.dev_planes, db.dev_strides = ctypes.cast(da["planes"], ctypes.c_void_p), ctypes.cast(da["strides"], ctypes.c_void_p)
+            db.infos, db.rcs = ctypes.cast(da["infos"], ctypes.c_void_p), ctypes.cast(da["rcs"], ctypes.c_void_p)
+        st = int(self._lib.tcx_serve_step(ctypes.byref(eb) if ea else None, ctypes.byref(db) if da else None))
+        er = (list(ea["sizes"]), list(ea["rcs"]), ea["stats"]) if ea else None
+        dr = (list(da["rcs"]), da["infos"]) if da else None
+        return st, er, dr
+
+
+class TcxEncBatch(ctypes.Structure):
+    """struct tcx_enc_batch (include/topos_gpu.h)"""
+    _fields_ = [("cfgs", ctypes.c_void_p), ("n", ctypes.c_uint32), ("dev_planes", ctypes.c_void_p),
+                ("dev_strides", ctypes.c_void_p), ("dev_out", ctypes.c_void_p), ("out_caps", ctypes.c_void_p),
+                ("out_sizes", ctypes.c_void_p), ("rcs", ctypes.c_void_p), ("stats", ctypes.c_void_p)]
+
+
+class TcxDecBatch(ctypes.Structure):
+    """struct tcx_dec_batch (include/topos_gpu.h)"""
+    _fields_ = [("n", ctypes.c_uint32), ("host_packets", ctypes.c_void_p), ("sizes", ctypes.c_void_p),
+                ("dev_packets", ctypes.c_void_p), ("dev_planes", ctypes.c_void_p), ("dev_strides", ctypes.c_void_p),
+                ("infos", ctypes.c_void_p), ("rcs", ctypes.c_void_p)]
 
 
 # ---------------------------------------------------------------- workloads

@@ -97,7 +97,8 @@ int32_t tc_frame_encode(const topos_frame_config* cfg, const topos_frame_input* 
     }
     int32_t rc = enc_check(cfg, input, NULL, NULL);
     if (rc != TC_OK) { return rc; }
-    if ((rc = tc_engine_acquire()) != TC_OK) { return rc; }
+    tc_ectx* ec = NULL;
+    if ((rc = tc_engine_acquire(&ec)) != TC_OK) { return rc; }
     tc_enc_req r;
     memset(&r, 0, sizeof(r));
     r.cfg = cfg;
@@ -105,8 +106,8 @@ int32_t tc_frame_encode(const topos_frame_config* cfg, const topos_frame_input* 
     r.out = out;
     r.out_cap = out_cap;
     r.stats = stats;
-    rc = tc_engine_encode(&r, 1u);
-    tc_engine_release();
+    rc = tc_engine_encode(ec, &r, 1u);
+    tc_engine_release(ec);
     return rc == TC_OK ? r.rc : rc;
 }
 
@@ -161,13 +162,14 @@ int32_t tc_frame_decode(const uint8_t* data, size_t size, uint16_t* const planes
                              (unsigned)v->fh.pvw[p]);
             }
         }
-        if (rc == TC_OK && (rc = tc_engine_acquire()) == TC_OK) {
+        tc_ectx* ec = NULL;
+        if (rc == TC_OK && (rc = tc_engine_acquire(&ec)) == TC_OK) {
             tc_dec_req r;
             memset(&r, 0, sizeof(r));
             r.data = data; r.size = size; r.planes = planes_out; r.strides = strides;
             r.info = out_info; r.view = v;
-            rc = tc_engine_decode(&r, 1u);
-            tc_engine_release();
+            rc = tc_engine_decode(ec, &r, 1u);
+            tc_engine_release(ec);
             if (rc >= TC_OK) { rc = r.rc; }
         }
     }
@@ -207,9 +209,10 @@ int32_t tc_frame_decode_batch(const topos_batch_packet* packets, uint32_t count,
         if (r->rc == TC_OK) { map[nrun] = i; run[nrun++] = *r; }
     }
     if (nrun > 0u) {
-        if ((rc = tc_engine_acquire()) == TC_OK) {
-            rc = tc_engine_decode(run, nrun);
-            tc_engine_release();
+        tc_ectx* ec = NULL;
+        if ((rc = tc_engine_acquire(&ec)) == TC_OK) {
+            rc = tc_engine_decode(ec, run, nrun);
+            tc_engine_release(ec);
             for (uint32_t k = 0; k < nrun; ++k) { reqs[map[k]].rc = rc < TC_OK ? rc : run[k].rc; }
         } else {
             for (uint32_t k = 0; k < nrun; ++k) { reqs[map[k]].rc = rc; }

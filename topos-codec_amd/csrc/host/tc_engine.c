@@ -1,6 +1,10 @@
-/* Device engine: grow-only HBM buffers, descriptor tables and the encode /
- * decode launch sequences behind the public calls. One engine per process,
- * serialised by a mutex; all device work runs on one default-priority stream.
+/* Device engine: contexts of grow-only HBM buffers, pinned staging and one
+ * default-priority stream each, plus the encode / decode launch sequences
+ * behind the public calls. A call takes any free context, so calls from
+ * different host threads run concurrently on the device; a context is used by
+ * one call at a time. Every operation is split into submit (plan, uploads,
+ * launches, result download queued) and finish (wait, post-process), so one
+ * thread can keep an encode and a decode batch in flight on two contexts.
  *
  * Batch model: n frames of one call share one flat block index space, one
  * descriptor upload and one launch of every kernel, so a batch costs the same
@@ -14,23 +18,45 @@
 
 typedef struct dbuf { uint64_t p; size_t cap; } dbuf;
 
+typedef struct enc_frame_plan {
+    tc_fhdr fh;
+    uint32_t slice_rows;
+    uint32_t nslices;
+    int32_t qp_eff[4];
+} enc_frame_plan;
+
+struct tc_ectx {
+    pthread_mutex_t mu;
+    tcg_ctx* g;
+    dbuf in, q, dcm, scratch, pkt, desc, res, out, tmp;
+    uint8_t* hdesc;    /* pinned staging for descriptor uploads */
+    size_t hdesc_cap;
+    uint8_t* hres;     /* pinned staging for result downloads */
+    size_t hres_cap;
+    /* operation in flight between submit and finish */
+    enc_frame_plan* plan;
+    size_t o_res, o_fo, o_fr;
+    uint64_t* out_dev;
+    uint32_t nslices;
+};
+
+#define TC_MAX_CTX 8u
+
 static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
 static int g_state = 0;       /* 0 uninitialised, 1 ready, -1 failed */
 static int32_t g_init_rc = 0;
 static char g_init_msg[256];
-
-static dbuf g_in, g_q, g_dcm, g_scratch, g_pkt, g_desc, g_res, g_out, g_tmp;
-static uint8_t* g_hdesc = NULL;   /* pinned staging for descriptor uploads */
-static size_t g_hdesc_cap = 0;
-static uint8_t* g_hres = NULL;    /* pinned staging for result downloads */
-static size_t g_hres_cap = 0;
+static tc_ectx g_ctx[TC_MAX_CTX];
+static uint32_t g_nctx = 0, g_rr = 0;
+static pthread_mutex_t g_tmu = PTHREAD_MUTEX_INITIALIZER;
+static tcg_timing g_last_enc, g_last_dec;
 
 static size_t al256(size_t v) { return (v + 255u) & ~(size_t)255u; }
 
-static int32_t dbuf_need(dbuf* b, size_t bytes)
+static int32_t dbuf_need(tc_ectx* c, dbuf* b, size_t bytes)
 {
     if (bytes <= b->cap) { return TC_OK; }
-    if (b->p) { (void)tcg_sync(); (void)tcg_free(b->p); b->p = 0; b->cap = 0; }
+    if (b->p) { (void)tcg_sync(c->g); (void)tcg_free(b->p); b->p = 0; b->cap = 0; }
     size_t cap = bytes + bytes / 4u + 4096u;
     if (tcg_malloc(&b->p, cap) != 0) {
         b->p = 0;
@@ -40,23 +66,21 @@ static int32_t dbuf_need(dbuf* b, size_t bytes)
     return TC_OK;
 }
 
-static int32_t host_need(uint8_t** p, size_t* cap, size_t bytes)
+static int32_t host_need(tc_ectx* c, uint8_t** p, size_t* cap, size_t bytes)
 {
     if (bytes <= *cap) { return TC_OK; }
-    if (*p) { (void)tcg_sync(); (void)tcg_host_free(*p); *p = NULL; *cap = 0; }
-    size_t c = bytes + bytes / 4u + 4096u;
-    if (tcg_host_alloc((void**)p, c) != 0) {
+    if (*p) { (void)tcg_sync(c->g); (void)tcg_host_free(*p); *p = NULL; *cap = 0; }
+    size_t n = bytes + bytes / 4u + 4096u;
+    if (tcg_host_alloc((void**)p, n) != 0) {
         *p = NULL;
-        return tc_fail(TC_ERR_OUT_OF_MEMORY, "pinned host allocation of %zu bytes failed", c);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "pinned host allocation of %zu bytes failed", n);
     }
-    *cap = c;
+    *cap = n;
     return TC_OK;
 }
 
-int32_t tc_engine_acquire(void)
+static int32_t engine_init_locked(void)
 {
-    pthread_mutex_lock(&g_mu);
-    if (g_state == 1) { return TC_OK; }
     if (g_state == 0) {
         int32_t rc = tcg_init();
         if (rc == 0) {
@@ -75,26 +99,69 @@ int32_t tc_engine_acquire(void)
             g_state = 1;
         }
     }
-    if (g_state != 1) {
-        pthread_mutex_unlock(&g_mu);
-        return tc_fail(g_init_rc, "GPU backend unavailable: %s", g_init_msg);
+    return g_state == 1 ? TC_OK : tc_fail(g_init_rc, "GPU backend unavailable: %s", g_init_msg);
+}
+
+int32_t tc_engine_acquire(tc_ectx** out)
+{
+    pthread_mutex_lock(&g_mu);
+    int32_t rc = engine_init_locked();
+    if (rc != TC_OK) { pthread_mutex_unlock(&g_mu); return rc; }
+    for (uint32_t i = 0; i < g_nctx; ++i) {
+        if (pthread_mutex_trylock(&g_ctx[i].mu) == 0) {
+            pthread_mutex_unlock(&g_mu);
+            *out = &g_ctx[i];
+            return TC_OK;
+        }
     }
+    if (g_nctx < TC_MAX_CTX) {
+        tc_ectx* c = &g_ctx[g_nctx];
+        memset(c, 0, sizeof(*c));
+        if (tcg_ctx_create(&c->g) != 0) {
+            pthread_mutex_unlock(&g_mu);
+            return tc_fail(TC_ERR_OUT_OF_MEMORY, "stream creation failed: %s", tcg_last_error());
+        }
+        pthread_mutex_init(&c->mu, NULL);
+        pthread_mutex_lock(&c->mu);
+        g_nctx++;
+        pthread_mutex_unlock(&g_mu);
+        *out = c;
+        return TC_OK;
+    }
+    tc_ectx* c = &g_ctx[g_rr++ % g_nctx];
+    pthread_mutex_unlock(&g_mu);
+    pthread_mutex_lock(&c->mu);
+    *out = c;
     return TC_OK;
 }
 
-void tc_engine_release(void) { pthread_mutex_unlock(&g_mu); }
+void tc_engine_release(tc_ectx* c) { pthread_mutex_unlock(&c->mu); }
+
+tcg_ctx* tc_engine_stream(tc_ectx* c) { return c->g; }
+
+void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec)
+{
+    pthread_mutex_lock(&g_tmu);
+    *enc = g_last_enc;
+    *dec = g_last_dec;
+    pthread_mutex_unlock(&g_tmu);
+}
+
+static void record_timing(tc_ectx* c, int encode)
+{
+    if (!tcg_timing_on()) { return; }
+    tcg_timing t;
+    tcg_ctx_timing(c->g, &t);
+    pthread_mutex_lock(&g_tmu);
+    if (encode) { g_last_enc = t; } else { g_last_dec = t; }
+    pthread_mutex_unlock(&g_tmu);
+}
 
 /* ------------------------------------------------------------------ encode */
 
-typedef struct enc_frame_plan {
-    tc_fhdr fh;
-    uint32_t slice_rows;
-    uint32_t nslices;
-    int32_t qp_eff[4];
-} enc_frame_plan;
-
-int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
+int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
 {
+    c->plan = NULL;
     enc_frame_plan* plan = (enc_frame_plan*)calloc(n ? n : 1u, sizeof(enc_frame_plan));
     if (plan == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode plan"); }
     uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, nqctx = 0;
@@ -146,26 +213,26 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
     const size_t o_fo = al256((size_t)nslices * sizeof(tcg_eresult));
     const size_t res_bytes = al256(o_fo + (size_t)n * sizeof(tcg_eframe_out));
 
-    if ((rc = dbuf_need(&g_q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
-        (rc = dbuf_need(&g_dcm, (size_t)nblocks * 4u + 256u)) != TC_OK ||
-        (rc = dbuf_need(&g_scratch, scratch_bytes)) != TC_OK ||
-        (rc = dbuf_need(&g_desc, desc_bytes)) != TC_OK ||
-        (rc = dbuf_need(&g_res, res_bytes)) != TC_OK ||
-        (rc = host_need(&g_hdesc, &g_hdesc_cap, desc_bytes)) != TC_OK ||
-        (rc = host_need(&g_hres, &g_hres_cap, res_bytes)) != TC_OK) {
+    if ((rc = dbuf_need(c, &c->q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
+        (rc = dbuf_need(c, &c->dcm, (size_t)nblocks * 4u + 256u)) != TC_OK ||
+        (rc = dbuf_need(c, &c->scratch, scratch_bytes)) != TC_OK ||
+        (rc = dbuf_need(c, &c->desc, desc_bytes)) != TC_OK ||
+        (rc = dbuf_need(c, &c->res, res_bytes)) != TC_OK ||
+        (rc = host_need(c, &c->hdesc, &c->hdesc_cap, desc_bytes)) != TC_OK ||
+        (rc = host_need(c, &c->hres, &c->hres_cap, res_bytes)) != TC_OK) {
         free(plan);
         return rc;
     }
     int need_pkt = 0;
     for (uint32_t f = 0; f < n; ++f) { if (reqs[f].dev_out == 0u) { need_pkt = 1; } }
-    if (need_pkt && (rc = dbuf_need(&g_pkt, pkt_bytes)) != TC_OK) { free(plan); return rc; }
-    if (in_bytes && (rc = dbuf_need(&g_in, in_bytes)) != TC_OK) { free(plan); return rc; }
+    if (need_pkt && (rc = dbuf_need(c, &c->pkt, pkt_bytes)) != TC_OK) { free(plan); return rc; }
+    if (in_bytes && (rc = dbuf_need(c, &c->in, in_bytes)) != TC_OK) { free(plan); return rc; }
 
-    memset(g_hdesc, 0, desc_bytes);
-    tcg_qctx* hq = (tcg_qctx*)(g_hdesc + o_q);
-    tcg_eplane* hp = (tcg_eplane*)(g_hdesc + o_pl);
-    tcg_eslice* hs = (tcg_eslice*)(g_hdesc + o_sl);
-    tcg_eframe* hf = (tcg_eframe*)(g_hdesc + o_fr);
+    memset(c->hdesc, 0, desc_bytes);
+    tcg_qctx* hq = (tcg_qctx*)(c->hdesc + o_q);
+    tcg_eplane* hp = (tcg_eplane*)(c->hdesc + o_pl);
+    tcg_eslice* hs = (tcg_eslice*)(c->hdesc + o_sl);
+    tcg_eframe* hf = (tcg_eframe*)(c->hdesc + o_fr);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0;
     size_t in_off = 0, scr_off = 0, pkt_off = 0;
     for (uint32_t f = 0; f < n && rc == TC_OK; ++f) {
@@ -184,9 +251,9 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
             } else {
                 const size_t stride = ((size_t)vw + 7u) & ~(size_t)7u;
                 const size_t sst = r->input->strides[p] ? r->input->strides[p] : vw;
-                E->src = g_in.p + in_off;
+                E->src = c->in.p + in_off;
                 E->stride = (uint32_t)stride;
-                rc = tcg_h2d_2d(E->src, stride * 2u, r->input->planes[p], sst * 2u, (size_t)vw * 2u, vh);
+                rc = tcg_h2d_2d(c->g, E->src, stride * 2u, r->input->planes[p], sst * 2u, (size_t)vw * 2u, vh);
                 if (rc != 0) { rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "input upload: %s", tcg_last_error()); break; }
                 in_off += al256(stride * vh * 2u);
             }
@@ -231,7 +298,7 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
             F->packet = r->dev_out;
             F->cap = (uint32_t)(r->dev_out_cap < TC_MAX_PACKET ? r->dev_out_cap : TC_MAX_PACKET);
         } else {
-            F->packet = g_pkt.p + pkt_off;
+            F->packet = c->pkt.p + pkt_off;
             F->cap = (uint32_t)(frame_cap < TC_MAX_PACKET ? frame_cap : TC_MAX_PACKET);
             pkt_off = al256(pkt_off + frame_cap + 1024u);
         }
@@ -241,24 +308,42 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
         tc_fhdr_pack(&tmp, F->hdr);
     }
     if (rc != TC_OK) { free(plan); return rc; }
-    if (tcg_h2d(g_desc.p, g_hdesc, desc_bytes) != 0) {
+    if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
     }
-    if (tcg_encode((const tcg_qctx*)(uintptr_t)(g_desc.p + o_q), nplanes,
-                   (const tcg_eplane*)(uintptr_t)(g_desc.p + o_pl), nslices,
-                   (const tcg_eslice*)(uintptr_t)(g_desc.p + o_sl), n,
-                   (const tcg_eframe*)(uintptr_t)(g_desc.p + o_fr), g_q.p, g_dcm.p, tile, g_scratch.p,
-                   (tcg_eresult*)(uintptr_t)(g_res.p + o_res),
-                   (tcg_eframe_out*)(uintptr_t)(g_res.p + o_fo)) != 0) {
+    if (tcg_encode(c->g, (const tcg_qctx*)(uintptr_t)(c->desc.p + o_q), nplanes,
+                   (const tcg_eplane*)(uintptr_t)(c->desc.p + o_pl), nslices,
+                   (const tcg_eslice*)(uintptr_t)(c->desc.p + o_sl), n,
+                   (const tcg_eframe*)(uintptr_t)(c->desc.p + o_fr), c->q.p, c->dcm.p, tile, c->scratch.p,
+                   (tcg_eresult*)(uintptr_t)(c->res.p + o_res),
+                   (tcg_eframe_out*)(uintptr_t)(c->res.p + o_fo)) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode launch failed: %s", tcg_last_error());
     }
-    if (tcg_d2h(g_hres, g_res.p, res_bytes) != 0) {
+    if (tcg_d2h(c->g, c->hres, c->res.p, res_bytes) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode result download: %s", tcg_last_error());
     }
-    const tcg_eframe_out* fo = (const tcg_eframe_out*)(g_hres + o_fo);
+    c->plan = plan;
+    c->o_res = o_res;
+    c->o_fo = o_fo;
+    c->o_fr = o_fr;
+    return TC_OK;
+}
+
+int32_t tc_engine_encode_finish(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
+{
+    enc_frame_plan* plan = c->plan;
+    c->plan = NULL;
+    if (plan == NULL) { return tc_fail(TC_ERR_STATE, "no encode in flight"); }
+    if (tcg_sync(c->g) != 0) {
+        free(plan);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode: %s", tcg_last_error());
+    }
+    record_timing(c, 1);
+    const tcg_eframe* hf = (const tcg_eframe*)(c->hdesc + c->o_fr);
+    const tcg_eframe_out* fo = (const tcg_eframe_out*)(c->hres + c->o_fo);
     int32_t first = TC_OK;
     for (uint32_t f = 0; f < n; ++f) {
         tc_enc_req* r = &reqs[f];
@@ -280,7 +365,7 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
                 st->slice_count = (uint16_t)plan[f].nslices;
                 st->qp_base = plan[f].fh.qp_base;
                 uint32_t merr = 0u;
-                const tcg_eresult* rs = (const tcg_eresult*)(g_hres + o_res);
+                const tcg_eresult* rs = (const tcg_eresult*)(c->hres + c->o_res);
                 for (uint32_t i = hf[f].s0; i < hf[f].s1; ++i) { merr = rs[i].aux > merr ? rs[i].aux : merr; }
                 st->alpha_max_abs_error = (uint16_t)merr;
             }
@@ -290,7 +375,7 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
                     r->rc = tc_fail(TC_ERR_BUFFER_TOO_SMALL, "out_cap %zu < %u bytes needed", r->out_cap,
                                     (unsigned)o->size);
                 } else if (r->out != NULL) {
-                    if (tcg_d2h(r->out, hf[f].packet, o->size) != 0) {
+                    if (tcg_d2h_sync(c->g, r->out, hf[f].packet, o->size) != 0) {
                         r->rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "packet download: %s", tcg_last_error());
                     }
                 }
@@ -302,10 +387,17 @@ int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n)
     return first;
 }
 
+int32_t tc_engine_encode(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
+{
+    int32_t rc = tc_engine_encode_submit(c, reqs, n);
+    return rc != TC_OK ? rc : tc_engine_encode_finish(c, reqs, n);
+}
+
 /* ------------------------------------------------------------------ decode */
 
-int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
+int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
 {
+    c->out_dev = NULL;
     uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, nrows = 0, ndb = 0;
     size_t pkt_bytes = 0, out_bytes = 0, tmp_bytes = 0;
     for (uint32_t f = 0; f < n; ++f) {
@@ -331,23 +423,23 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
     const size_t desc_bytes = al256(o_db + (size_t)(nplanes + 1u) * sizeof(uint32_t));
     const size_t res_bytes = al256((size_t)nslices * sizeof(int32_t));
     int32_t rc;
-    if ((rc = dbuf_need(&g_q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
-        (rc = dbuf_need(&g_desc, desc_bytes)) != TC_OK ||
-        (rc = dbuf_need(&g_res, res_bytes)) != TC_OK ||
-        (rc = host_need(&g_hdesc, &g_hdesc_cap, desc_bytes)) != TC_OK ||
-        (rc = host_need(&g_hres, &g_hres_cap, res_bytes)) != TC_OK) {
+    if ((rc = dbuf_need(c, &c->q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
+        (rc = dbuf_need(c, &c->desc, desc_bytes)) != TC_OK ||
+        (rc = dbuf_need(c, &c->res, res_bytes)) != TC_OK ||
+        (rc = host_need(c, &c->hdesc, &c->hdesc_cap, desc_bytes)) != TC_OK ||
+        (rc = host_need(c, &c->hres, &c->hres_cap, res_bytes)) != TC_OK) {
         return rc;
     }
-    if (pkt_bytes && (rc = dbuf_need(&g_pkt, pkt_bytes)) != TC_OK) { return rc; }
-    if (out_bytes && (rc = dbuf_need(&g_out, out_bytes)) != TC_OK) { return rc; }
-    if (tmp_bytes && (rc = dbuf_need(&g_tmp, tmp_bytes)) != TC_OK) { return rc; }
+    if (pkt_bytes && (rc = dbuf_need(c, &c->pkt, pkt_bytes)) != TC_OK) { return rc; }
+    if (out_bytes && (rc = dbuf_need(c, &c->out, out_bytes)) != TC_OK) { return rc; }
+    if (tmp_bytes && (rc = dbuf_need(c, &c->tmp, tmp_bytes)) != TC_OK) { return rc; }
 
-    memset(g_hdesc, 0, desc_bytes);
-    tcg_qctx* hq = (tcg_qctx*)(g_hdesc + o_q);
-    tcg_dplane* hp = (tcg_dplane*)(g_hdesc + o_pl);
-    tcg_dslice* hs = (tcg_dslice*)(g_hdesc + o_sl);
-    uint16_t* hrs = (uint16_t*)(g_hdesc + o_rs);
-    uint32_t* hdb = (uint32_t*)(g_hdesc + o_db);
+    memset(c->hdesc, 0, desc_bytes);
+    tcg_qctx* hq = (tcg_qctx*)(c->hdesc + o_q);
+    tcg_dplane* hp = (tcg_dplane*)(c->hdesc + o_pl);
+    tcg_dslice* hs = (tcg_dslice*)(c->hdesc + o_sl);
+    uint16_t* hrs = (uint16_t*)(c->hdesc + o_rs);
+    uint32_t* hdb = (uint32_t*)(c->hdesc + o_db);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0;
     size_t pkt_off = 0, out_off = 0, tmp_off = 0;
     uint64_t* out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
@@ -358,8 +450,8 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
         const tc_fhdr* fh = &v->fh;
         uint64_t pk = r->dev_data;
         if (pk == 0u) {
-            pk = g_pkt.p + pkt_off;
-            if (tcg_h2d(pk, r->data, r->size) != 0) {
+            pk = c->pkt.p + pkt_off;
+            if (tcg_h2d(c->g, pk, r->data, r->size) != 0) {
                 free(out_dev);
                 return tc_fail(TC_ERR_OUT_OF_MEMORY, "packet upload: %s", tcg_last_error());
             }
@@ -375,7 +467,7 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
                 D->dst = r->dev_planes[p];
                 D->stride = (uint32_t)(r->dev_strides && r->dev_strides[p] ? r->dev_strides[p] : vw);
             } else {
-                D->dst = g_out.p + out_off;
+                D->dst = c->out.p + out_off;
                 D->stride = vw;
                 out_off += al256((size_t)vw * vh * 2u);
             }
@@ -397,7 +489,7 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
                 D->maxv = (1u << fh->bit_depth) - 1u;
                 tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
                 if (db && p < 3u) {
-                    D->tmp = g_tmp.p + tmp_off;
+                    D->tmp = c->tmp.p + tmp_off;
                     tmp_off += al256((size_t)vw * vh * 2u);
                     if (vw % 8u == 0u && vh % 8u == 0u && vw >= 16u && vh >= 16u && fh->qp_base <= 95u &&
                         fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
@@ -459,23 +551,38 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
     }
     hdb[ip] = dbt;
     (void)ndb;
-    if (tcg_h2d(g_desc.p, g_hdesc, desc_bytes) != 0) {
+    if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
     }
-    if (tcg_decode((const tcg_qctx*)(uintptr_t)(g_desc.p + o_q), nplanes,
-                   (const tcg_dplane*)(uintptr_t)(g_desc.p + o_pl), nslices,
-                   (const tcg_dslice*)(uintptr_t)(g_desc.p + o_sl),
-                   (const uint16_t*)(uintptr_t)(g_desc.p + o_rs), g_q.p, tile,
-                   (int32_t*)(uintptr_t)g_res.p, dbt, (const uint32_t*)(uintptr_t)(g_desc.p + o_db)) != 0) {
+    if (tcg_decode(c->g, (const tcg_qctx*)(uintptr_t)(c->desc.p + o_q), nplanes,
+                   (const tcg_dplane*)(uintptr_t)(c->desc.p + o_pl), nslices,
+                   (const tcg_dslice*)(uintptr_t)(c->desc.p + o_sl),
+                   (const uint16_t*)(uintptr_t)(c->desc.p + o_rs), c->q.p, tile,
+                   (int32_t*)(uintptr_t)c->res.p, dbt, (const uint32_t*)(uintptr_t)(c->desc.p + o_db)) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode launch failed: %s", tcg_last_error());
     }
-    if (tcg_d2h(g_hres, g_res.p, (size_t)nslices * sizeof(int32_t)) != 0) {
+    if (tcg_d2h(c->g, c->hres, c->res.p, (size_t)nslices * sizeof(int32_t)) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "status download: %s", tcg_last_error());
     }
-    const int32_t* st = (const int32_t*)g_hres;
+    c->out_dev = out_dev;
+    c->nslices = nslices;
+    return TC_OK;
+}
+
+int32_t tc_engine_decode_finish(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
+{
+    uint64_t* out_dev = c->out_dev;
+    c->out_dev = NULL;
+    if (out_dev == NULL) { return tc_fail(TC_ERR_STATE, "no decode in flight"); }
+    if (tcg_sync(c->g) != 0) {
+        free(out_dev);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode: %s", tcg_last_error());
+    }
+    record_timing(c, 0);
+    const int32_t* st = (const int32_t*)c->hres;
     int32_t first = TC_OK;
     uint32_t sbase = 0;
     for (uint32_t f = 0; f < n; ++f) {
@@ -494,7 +601,8 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
             for (uint32_t p = 0; p < v->fh.plane_count; ++p) {
                 const size_t vw = v->fh.pvw[p], vh = v->fh.pvh[p];
                 const size_t dst_stride = r->strides && r->strides[p] ? r->strides[p] : vw;
-                if (tcg_d2h_2d(r->planes[p], dst_stride * 2u, out_dev[(size_t)f * 4u + p], vw * 2u, vw * 2u, vh) != 0) {
+                if (tcg_d2h_2d(c->g, r->planes[p], dst_stride * 2u, out_dev[(size_t)f * 4u + p], vw * 2u, vw * 2u,
+                               vh) != 0) {
                     r->rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "plane download: %s", tcg_last_error());
                     break;
                 }
@@ -504,4 +612,10 @@ int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n)
     }
     free(out_dev);
     return first;
+}
+
+int32_t tc_engine_decode(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
+{
+    int32_t rc = tc_engine_decode_submit(c, reqs, n);
+    return rc != TC_OK ? rc : tc_engine_decode_finish(c, reqs, n);
 }

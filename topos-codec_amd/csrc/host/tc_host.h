@@ -81,8 +81,13 @@ uint32_t tc_deblock_alpha(uint32_t tbl, uint32_t qp);
 void tc_fill_output_info(const tc_fhdr* fh, topos_frame_output* info);
 
 /* ---- device engine (tc_engine.c) ---- */
-int32_t tc_engine_acquire(void);   /* lazily initialises the GPU; locks the engine */
-void tc_engine_release(void);
+typedef struct tc_ectx tc_ectx;
+/* lazily initialises the GPU; returns a free context (locked until release) */
+int32_t tc_engine_acquire(tc_ectx** out);
+void tc_engine_release(tc_ectx* c);
+tcg_ctx* tc_engine_stream(tc_ectx* c);
+/* per-kernel times of the last finished encode and decode (timing on) */
+void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec);
 
 /* encode n frames (all with the same cfg-derived header but separate inputs)
  * into device packets; host_out[i] receives frame i's packet when non-NULL */
@@ -100,7 +105,11 @@ typedef struct tc_enc_req {
     uint32_t packet_size;
 } tc_enc_req;
 
-int32_t tc_engine_encode(tc_enc_req* reqs, uint32_t n);
+int32_t tc_engine_encode(tc_ectx* c, tc_enc_req* reqs, uint32_t n);
+/* split form: submit queues everything on the context's stream; finish waits
+ * and fills the requests (the requests and inputs must stay valid between) */
+int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n);
+int32_t tc_engine_encode_finish(tc_ectx* c, tc_enc_req* reqs, uint32_t n);
 
 typedef struct tc_dec_req {
     const uint8_t* data;              /* host packet (API path) */
@@ -115,6 +124,8 @@ typedef struct tc_dec_req {
     int32_t rc;
 } tc_dec_req;
 
-int32_t tc_engine_decode(tc_dec_req* reqs, uint32_t n);
+int32_t tc_engine_decode(tc_ectx* c, tc_dec_req* reqs, uint32_t n);
+int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n);
+int32_t tc_engine_decode_finish(tc_ectx* c, tc_dec_req* reqs, uint32_t n);
 
 #endif

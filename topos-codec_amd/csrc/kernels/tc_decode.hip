@@ -550,28 +550,27 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
 // ---------------------------------------------------------------------------
 // host launch
 // ---------------------------------------------------------------------------
-extern "C" int32_t tcg_decode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
+extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
                               uint32_t nslices, const tcg_dslice* slices_dev,
                               const uint16_t* row_slice_dev, uint64_t qbuf, uint32_t total_tiles,
                               int32_t* slice_status_dev, uint32_t db_tiles, const uint32_t* db_tile0_dev)
 {
-    hipStream_t st = tcg_stream();
+    hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
-    TcgTimer tm;
-    tm.mark(st, 0);
+    c->kind = 2;
+    tcg_mark(c, 0);
     hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
                        (int32_t*)qbuf, slice_status_dev);
     tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
-    tm.mark(st, 1);
+    tcg_mark(c, 1);
     if (total_tiles > 0u) {
         hipLaunchKernelGGL(k_dequant_idct, dim3(total_tiles), dim3(256), 0, st, qctx_dev, planes_dev,
                            nplanes, slices_dev, row_slice_dev, slice_status_dev, (const int32_t*)qbuf);
     }
-    tm.mark(st, 2);
+    tcg_mark(c, 2);
     if (db_tiles > 0u) {
         hipLaunchKernelGGL(k_deblock, dim3(db_tiles), dim3(256), 0, st, planes_dev, nplanes, db_tile0_dev);
     }
-    tm.mark(st, 3);
-    tm.finish_decode();
+    tcg_mark(c, 3);
     return tcg_check("decode launch");
 }

@@ -45,16 +45,16 @@ struct TcgConsts {
 };
 
 // host-side runtime services (tc_runtime.hip)
-hipStream_t tcg_stream();
+struct tcg_ctx {
+    hipStream_t stream;
+    hipEvent_t ev[8];
+    int nmarks;     // events recorded by the last launch sequence (0 = timing off)
+    int kind;       // 1 encode, 2 decode
+};
 const TcgConsts* tcg_consts_dev();
 int32_t tcg_check(const char* what);
-int tcg_wide_mode();
-
-struct TcgTimer {
-    void mark(hipStream_t s, int i);
-    void finish_encode();
-    void finish_decode();
-};
+// records event i of the launch sequence when timing is on
+void tcg_mark(tcg_ctx* c, int i);
 
 // f64 cost accumulation in the reference's order with no contraction: the
 // model choice compares these sums with strict '<', so every rounding must

@@ -641,32 +641,31 @@ __global__ __launch_bounds__(256) void k_slice_emit(const TcgConsts* __restrict_
 // ---------------------------------------------------------------------------
 // host launch
 // ---------------------------------------------------------------------------
-extern "C" int32_t tcg_encode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
+extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
                               uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
                               const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
                               uint32_t total_tiles, uint64_t scratch, tcg_eresult* results_dev,
                               tcg_eframe_out* frame_out_dev)
 {
-    hipStream_t st = tcg_stream();
+    hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
-    TcgTimer tm;
-    tm.mark(st, 0);
+    c->kind = 1;
+    tcg_mark(c, 0);
     if (total_tiles > 0u) {
         hipLaunchKernelGGL(k_fdct_quant, dim3(total_tiles), dim3(256), 0, st, qctx_dev, planes_dev,
                            nplanes, (int32_t*)qbuf);
     }
-    tm.mark(st, 1);
+    tcg_mark(c, 1);
     hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
                        (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
     tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch,
                             results_dev);
-    tm.mark(st, 2);
+    tcg_mark(c, 2);
     hipLaunchKernelGGL(k_frame_scan, dim3(nframes), dim3(256), 0, st, K, frames_dev, results_dev,
                        slices_dev, planes_dev, frame_out_dev);
-    tm.mark(st, 3);
+    tcg_mark(c, 3);
     hipLaunchKernelGGL(k_slice_emit, dim3(nslices), dim3(256), 0, st, K, frames_dev, slices_dev,
                        (const uint8_t*)scratch, results_dev, frame_out_dev);
-    tm.mark(st, 4);
-    tm.finish_encode();
+    tcg_mark(c, 4);
     return tcg_check("encode launch");
 }

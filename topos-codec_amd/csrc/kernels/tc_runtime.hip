@@ -1,22 +1,20 @@
-// HIP runtime services behind the internal C ABI: one default-priority stream
-// per process, device memory, copies, the uploaded constant tables and
-// per-kernel timing with HIP events on the launch stream.
+// HIP runtime services behind the internal C ABI: device initialisation and
+// the uploaded constant tables (process-wide), contexts (one default-priority
+// stream each, with timing events), device memory, copies and per-kernel
+// timing with HIP events on the launch stream.
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "tcg.h"
 #include "tc_dev.h"
 
 namespace {
-hipStream_t g_stream = nullptr;
+int g_ready = 0;
 TcgConsts* g_consts = nullptr;
-int g_wide = 0;
 int g_timing = 0;
-hipEvent_t g_ev[8];
-int g_ev_ready = 0;
-int g_last_kind = 0;  // 1 encode, 2 decode
 thread_local char g_err[256];
 
 int32_t fail(const char* what, hipError_t e)
@@ -26,9 +24,7 @@ int32_t fail(const char* what, hipError_t e)
 }
 }  // namespace
 
-hipStream_t tcg_stream() { return g_stream; }
 const TcgConsts* tcg_consts_dev() { return g_consts; }
-int tcg_wide_mode() { return g_wide; }
 
 int32_t tcg_check(const char* what)
 {
@@ -37,12 +33,14 @@ int32_t tcg_check(const char* what)
     return 0;
 }
 
-void TcgTimer::mark(hipStream_t s, int i)
+void tcg_mark(tcg_ctx* c, int i)
 {
-    if (g_timing && g_ev_ready && i < 8) { (void)hipEventRecord(g_ev[i], s); }
+    if (i == 0) { c->nmarks = 0; }
+    if (g_timing && i < 8) {
+        (void)hipEventRecord(c->ev[i], c->stream);
+        c->nmarks = i + 1;
+    }
 }
-void TcgTimer::finish_encode() { g_last_kind = 1; }
-void TcgTimer::finish_decode() { g_last_kind = 2; }
 
 // plain 16-byte-vector device copy: the reference point for HBM byte counters
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16)
@@ -53,18 +51,9 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 
 extern "C" {
 
-int32_t tcg_copy(uint64_t dst, uint64_t src, size_t bytes)
-{
-    const size_t n16 = bytes / 16u;
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, g_stream, (const uint4*)(uintptr_t)src,
-                       (uint4*)(uintptr_t)dst, n16);
-    const hipError_t e = hipStreamSynchronize(g_stream);
-    return e == hipSuccess ? 0 : fail("copy", e);
-}
-
 int32_t tcg_init(void)
 {
-    if (g_stream != nullptr) { return 0; }
+    if (g_ready) { return 0; }
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n < 1) {
@@ -74,16 +63,54 @@ int32_t tcg_init(void)
     }
     e = hipSetDevice(0);
     if (e != hipSuccess) { return fail("hipSetDevice", e); }
-    e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { return fail("hipStreamCreate", e); }
     e = hipMalloc((void**)&g_consts, sizeof(TcgConsts));
     if (e != hipSuccess) { return fail("hipMalloc consts", e); }
-    for (int i = 0; i < 8; ++i) { (void)hipEventCreate(&g_ev[i]); }
-    g_ev_ready = 1;
+    g_ready = 1;
     return 0;
 }
 
 const char* tcg_last_error(void) { return g_err; }
+
+int32_t tcg_ctx_create(tcg_ctx** out)
+{
+    tcg_ctx* c = (tcg_ctx*)calloc(1, sizeof(tcg_ctx));
+    if (c == nullptr) {
+        snprintf(g_err, sizeof(g_err), "context allocation failed");
+        return -2;
+    }
+    // default priority, non-blocking with respect to the null stream
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { free(c); return fail("hipStreamCreate", e); }
+    for (int i = 0; i < 8; ++i) {
+        e = hipEventCreate(&c->ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; ++j) { (void)hipEventDestroy(c->ev[j]); }
+            (void)hipStreamDestroy(c->stream);
+            free(c);
+            return fail("hipEventCreate", e);
+        }
+    }
+    *out = c;
+    return 0;
+}
+
+void tcg_ctx_destroy(tcg_ctx* c)
+{
+    if (c == nullptr) { return; }
+    (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 8; ++i) { (void)hipEventDestroy(c->ev[i]); }
+    (void)hipStreamDestroy(c->stream);
+    free(c);
+}
+
+int32_t tcg_copy(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes)
+{
+    const size_t n16 = bytes / 16u;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const uint4*)(uintptr_t)src,
+                       (uint4*)(uintptr_t)dst, n16);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? 0 : fail("copy", e);
+}
 
 int32_t tcg_malloc(uint64_t* dptr, size_t bytes)
 {
@@ -114,41 +141,49 @@ int32_t tcg_host_free(void* hptr)
     return e == hipSuccess ? 0 : fail("hipHostFree", e);
 }
 
-int32_t tcg_h2d(uint64_t dst, const void* src, size_t bytes)
+int32_t tcg_h2d(tcg_ctx* c, uint64_t dst, const void* src, size_t bytes)
 {
     if (bytes == 0u) { return 0; }
-    const hipError_t e = hipMemcpyAsync((void*)(uintptr_t)dst, src, bytes, hipMemcpyHostToDevice, g_stream);
+    const hipError_t e = hipMemcpyAsync((void*)(uintptr_t)dst, src, bytes, hipMemcpyHostToDevice, c->stream);
     return e == hipSuccess ? 0 : fail("hipMemcpy H2D", e);
 }
 
-int32_t tcg_d2h(void* dst, uint64_t src, size_t bytes)
+int32_t tcg_d2h(tcg_ctx* c, void* dst, uint64_t src, size_t bytes)
 {
     if (bytes == 0u) { return 0; }
-    hipError_t e = hipMemcpyAsync(dst, (const void*)(uintptr_t)src, bytes, hipMemcpyDeviceToHost, g_stream);
-    if (e == hipSuccess) { e = hipStreamSynchronize(g_stream); }
+    const hipError_t e = hipMemcpyAsync(dst, (const void*)(uintptr_t)src, bytes, hipMemcpyDeviceToHost, c->stream);
     return e == hipSuccess ? 0 : fail("hipMemcpy D2H", e);
 }
 
-int32_t tcg_h2d_2d(uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows)
+int32_t tcg_d2h_sync(tcg_ctx* c, void* dst, uint64_t src, size_t bytes)
+{
+    if (bytes == 0u) { return 0; }
+    hipError_t e = hipMemcpyAsync(dst, (const void*)(uintptr_t)src, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
+    return e == hipSuccess ? 0 : fail("hipMemcpy D2H", e);
+}
+
+int32_t tcg_h2d_2d(tcg_ctx* c, uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                   size_t rows)
 {
     if (width == 0u || rows == 0u) { return 0; }
     const hipError_t e = hipMemcpy2DAsync((void*)(uintptr_t)dst, dpitch, src, spitch, width, rows,
-                                          hipMemcpyHostToDevice, g_stream);
+                                          hipMemcpyHostToDevice, c->stream);
     return e == hipSuccess ? 0 : fail("hipMemcpy2D H2D", e);
 }
 
-int32_t tcg_d2h_2d(void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows)
+int32_t tcg_d2h_2d(tcg_ctx* c, void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows)
 {
     if (width == 0u || rows == 0u) { return 0; }
     hipError_t e = hipMemcpy2DAsync(dst, dpitch, (const void*)(uintptr_t)src, spitch, width, rows,
-                                    hipMemcpyDeviceToHost, g_stream);
-    if (e == hipSuccess) { e = hipStreamSynchronize(g_stream); }
+                                    hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
     return e == hipSuccess ? 0 : fail("hipMemcpy2D D2H", e);
 }
 
-int32_t tcg_sync(void)
+int32_t tcg_sync(tcg_ctx* c)
 {
-    const hipError_t e = hipStreamSynchronize(g_stream);
+    const hipError_t e = hipStreamSynchronize(c->stream);
     return e == hipSuccess ? 0 : fail("hipStreamSynchronize", e);
 }
 
@@ -158,28 +193,26 @@ int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uin
     memcpy(h.cost, cost4097, sizeof(h.cost));
     memcpy(h.crc, crc256, sizeof(h.crc));
     memcpy(h.x2n, x2n32, sizeof(h.x2n));
-    hipError_t e = hipMemcpyAsync(g_consts, &h, sizeof(h), hipMemcpyHostToDevice, g_stream);
-    if (e == hipSuccess) { e = hipStreamSynchronize(g_stream); }
+    const hipError_t e = hipMemcpy(g_consts, &h, sizeof(h), hipMemcpyHostToDevice);
     return e == hipSuccess ? 0 : fail("upload tables", e);
 }
 
-void tcg_set_wide(int wide) { g_wide = wide; }
-
 void tcg_set_timing(int on) { g_timing = on; }
 
-void tcg_get_timing(tcg_timing* t)
+int tcg_timing_on(void) { return g_timing; }
+
+void tcg_ctx_timing(tcg_ctx* c, tcg_timing* t)
 {
     memset(t, 0, sizeof(*t));
-    if (!g_timing || !g_ev_ready) { return; }
-    (void)hipStreamSynchronize(g_stream);
-    float ms[4] = { 0, 0, 0, 0 };
-    const int nmarks = g_last_kind == 1 ? 4 : 3;
-    for (int i = 0; i < nmarks; ++i) {
-        if (hipEventElapsedTime(&ms[i], g_ev[i], g_ev[i + 1]) != hipSuccess) { ms[i] = -1.0f; }
+    const int n = c->nmarks;
+    if (n < 2) { return; }
+    float ms[7] = { 0, 0, 0, 0, 0, 0, 0 };
+    for (int i = 0; i + 1 < n && i < 7; ++i) {
+        if (hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]) != hipSuccess) { ms[i] = -1.0f; }
     }
-    if (g_last_kind == 1) {
+    if (c->kind == 1) {
         t->fdct_ms = ms[0]; t->slice_enc_ms = ms[1]; t->scan_ms = ms[2]; t->emit_ms = ms[3];
-    } else if (g_last_kind == 2) {
+    } else if (c->kind == 2) {
         t->slice_dec_ms = ms[0]; t->idct_ms = ms[1]; t->deblock_ms = ms[2];
     }
 }

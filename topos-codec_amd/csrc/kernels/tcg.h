@@ -127,46 +127,54 @@ typedef struct tcg_dplane {
 } tcg_dplane;
 
 /* ---- runtime ---- */
+/* A context is one default-priority stream plus its timing events; work on
+ * different contexts runs concurrently on the device. */
+typedef struct tcg_ctx tcg_ctx;
+
 int32_t tcg_init(void);
 const char* tcg_last_error(void);
+int32_t tcg_ctx_create(tcg_ctx** out);
+void tcg_ctx_destroy(tcg_ctx* c);
 int32_t tcg_malloc(uint64_t* dptr, size_t bytes);
 int32_t tcg_free(uint64_t dptr);
 int32_t tcg_host_alloc(void** hptr, size_t bytes);
 int32_t tcg_host_free(void* hptr);
-int32_t tcg_h2d(uint64_t dst, const void* src, size_t bytes);
-int32_t tcg_d2h(void* dst, uint64_t src, size_t bytes);
-int32_t tcg_h2d_2d(uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows);
-int32_t tcg_d2h_2d(void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows);
-int32_t tcg_sync(void);
+/* copies are queued on the context's stream; the *_sync forms wait for it */
+int32_t tcg_h2d(tcg_ctx* c, uint64_t dst, const void* src, size_t bytes);
+int32_t tcg_d2h(tcg_ctx* c, void* dst, uint64_t src, size_t bytes);
+int32_t tcg_d2h_sync(tcg_ctx* c, void* dst, uint64_t src, size_t bytes);
+int32_t tcg_h2d_2d(tcg_ctx* c, uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                   size_t rows);
+int32_t tcg_d2h_2d(tcg_ctx* c, void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows);
+int32_t tcg_sync(tcg_ctx* c);
 /* device copy of bytes (multiple of 16) with a plain vector-copy kernel */
-int32_t tcg_copy(uint64_t dst, uint64_t src, size_t bytes);
+int32_t tcg_copy(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes);
 /* upload the (12 - log2 f) cost table, f = 0..4096 (entry 0 unused), and
  * the CRC32 tables (bytewise table, x^(2^k) mod P) */
 int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uint32_t* x2n32);
-/* wide (bit depth > 12) integer paths for the next launches */
-void tcg_set_wide(int wide);
 
 /* ---- encode ---- */
-int32_t tcg_encode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
+int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
                    uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
                    const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
                    uint32_t total_tiles, uint64_t scratch, tcg_eresult* results_dev,
                    tcg_eframe_out* frame_out_dev);
 
 /* ---- decode ---- */
-int32_t tcg_decode(const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
+int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
                    uint32_t nslices, const tcg_dslice* slices_dev, const uint16_t* row_slice_dev,
                    uint64_t qbuf, uint32_t total_tiles, int32_t* slice_status_dev,
                    uint32_t db_tiles, const uint32_t* db_tile0_dev);
 
-/* timing of the most recent tcg_encode / tcg_decode launch sequence, per
- * kernel, in milliseconds (HIP events on the launch stream) */
+/* per-kernel times in milliseconds (HIP events on the launch stream) of the
+ * context's last launch sequence; valid after tcg_sync when timing is on */
 typedef struct tcg_timing {
     float fdct_ms, slice_enc_ms, scan_ms, emit_ms;
     float slice_dec_ms, idct_ms, deblock_ms;
 } tcg_timing;
-void tcg_get_timing(tcg_timing* t);
 void tcg_set_timing(int on);
+int tcg_timing_on(void);
+void tcg_ctx_timing(tcg_ctx* c, tcg_timing* t);
 
 #ifdef __cplusplus
 }
