@@ -67,3 +67,45 @@ uint32_t mc_enc_update_check(uint32_t per_f)
 }
 
 uint32_t mc_zigzag_pos(uint32_t n) { return tc_zigzag_pos(n); }
+
+/* the fp64 quantiser divide against integer division: every Q in qs[0..nq),
+ * numerators around every multiple boundary near the top of the range
+ * (n < 2^31), small numerators and pseudo-random ones. Returns mismatches. */
+uint32_t mc_qdiv_f64_check(const uint32_t* qs, uint32_t nq, uint32_t per_q)
+{
+    uint32_t bad = 0u;
+    uint64_t s = 0x2545F4914F6CDD1Dull;
+    for (uint32_t i = 0; i < nq; ++i) {
+        const uint32_t Q = qs[i];
+        const double r = 1.0 / (double)Q;
+        const uint32_t top = 0x7FFFFFFFu;
+        const uint32_t kmax = top / Q;
+        for (uint32_t j = 0; j < per_q + 16u; ++j) {
+            uint64_t k;
+            if (j < 8u) { k = kmax >= j ? kmax - j : 0u; }
+            else if (j < 16u) { k = j - 8u; }
+            else { s ^= s << 13; s ^= s >> 7; s ^= s << 17; k = s % ((uint64_t)kmax + 1u); }
+            for (int d = -1; d <= 1; ++d) {
+                const int64_t nn = (int64_t)(k * Q) + d;
+                if (nn < 0 || nn > (int64_t)top) { continue; }
+                const uint32_t n = (uint32_t)nn;
+                if (tc_qdiv_f64(n, r) != n / Q) { ++bad; }
+            }
+        }
+    }
+    return bad;
+}
+
+/* the decode kernel's packed context lookups against the plain bucket rules;
+ * returns mismatches */
+uint32_t mc_ctx_tables_check(void)
+{
+    uint32_t bad = 0u;
+    for (uint32_t idx = 0; idx < 64u; ++idx) { bad += tc_ctx_pos_nib(idx) != tc_ctx_pos(idx); }
+    for (uint32_t cat = 0; cat < 32u; ++cat) {
+        const uint32_t w = cat > 1u ? cat - 1u : 0u, nz = cat != 0u;
+        bad += tc_ctx_prev_wnz(w, nz) != tc_ctx_prev(cat, 1u);
+        bad += 1u + tc_ctx_prev_wnz(w, nz) != tc_ctx_dc(cat, 1u);
+    }
+    return bad;
+}

@@ -66,3 +66,45 @@ def test_encode_decode_other_formats(pkg, abi, syn, oracle, codec, kind, w, h, k
     assert d_rc == o_rc
     for a, b in zip(d_planes, o_planes):
         assert (a == b).all()
+
+
+def _slice_flags(pkt):
+    """(plane, first payload byte) of every non-empty slice; for rANS2 colour
+    slices that byte holds the level model (bits 0-1) and the DC model (bit 2)."""
+    count = int.from_bytes(pkt[30:32], "big")
+    out, off = set(), 53
+    for _ in range(count):
+        size = int.from_bytes(pkt[off:off + 4], "big")
+        if size:
+            out.add((pkt[off + 4], pkt[off + 17]))
+        off += 17 + size
+    return out
+
+
+# content and sizes for which the encoder picks each conditional DC model
+# variant: flags 4 (levels unconditioned), 5 (position), 6 (previous level)
+DC_MODEL_CASES = [
+    ("444", 1024, 512, 3, {4, 5}, dict(qp=95, profile=3, pixel_format=1, bit_depth=12, slice_rows=32)),
+    ("422", 1024, 512, 3, {6}, dict(qp=80, profile=3, slice_rows=64)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,w,h,seed,want,kw", DC_MODEL_CASES, ids=["dc-model-4-5", "dc-model-6"])
+def test_dc_context_model_variants(pkg, abi, syn, oracle, codec, kind, w, h, seed, want, kw):
+    kw = dict(kw)
+    qp = kw.pop("qp")
+    planes = frame_for(pkg, syn, kind, w, h, seed=seed)
+    cfg = abi.make_config(w, h, qp, entropy_mode=8, **kw)
+    orc_rc, orc_pkt = orc_encode(oracle, cfg, planes)
+    assert orc_rc == 0
+    assert want <= {f for _, f in _slice_flags(orc_pkt)}, "content no longer selects the DC model variants"
+    st, pkt, _ = codec.encode_status(cfg, planes)
+    assert st == 0, codec.last_error()
+    assert pkt == orc_pkt
+    dims = abi.plane_dims(w, h, cfg.pixel_format, 0)
+    o_rc, o_planes, _, _ = orc_decode(oracle, orc_pkt, dims)
+    d_rc, d_planes, _ = codec.decode(orc_pkt, strict=False)
+    assert d_rc == o_rc == 0
+    for a, b in zip(d_planes, o_planes):
+        assert (a == b).all()

@@ -17,8 +17,8 @@ SRC = os.path.join(ROOT, "tests", "csrc", "mathcheck.c")
 @pytest.fixture(scope="module")
 def mc(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("mc") / "libmathcheck.so")
-    subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-I",
-                    os.path.join(ROOT, "topos-codec_amd", "csrc", "kernels"), SRC, "-o", out], check=True)
+    subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-I",
+                    os.path.join(ROOT, "topos-codec_amd", "csrc", "kernels"), SRC, "-o", out, "-lm"], check=True)
     lib = ctypes.CDLL(out)
     lib.mc_rans_div.restype = ctypes.c_uint32
     lib.mc_qdiv_magic.restype = ctypes.c_uint64
@@ -28,6 +28,8 @@ def mc(tmp_path_factory):
     lib.mc_gf2_mulmod.restype = ctypes.c_uint32
     lib.mc_enc_update_check.restype = ctypes.c_uint32
     lib.mc_zigzag_pos.restype = ctypes.c_uint32
+    lib.mc_qdiv_f64_check.restype = ctypes.c_uint32
+    lib.mc_ctx_tables_check.restype = ctypes.c_uint32
     return lib
 
 
@@ -46,6 +48,12 @@ def test_rans_encoder_update_exact(mc):
     # x' = (x / f) * 4096 + x % f + cum through the reciprocal constants the
     # encode kernel uses, for every frequency and states up to the bound
     assert mc.mc_enc_update_check(2000) == 0
+
+
+def test_context_lookup_tables(mc):
+    # packed-nibble context buckets of the decode kernel vs the bucket rules
+    # of the V7-R2 conditional models (position, previous category, DC)
+    assert mc.mc_ctx_tables_check() == 0
 
 
 def test_zigzag_closed_form(mc):
@@ -67,6 +75,19 @@ def test_zigzag_closed_form(mc):
         inv[n] = i
     assert order[:6] == [0, 1, 8, 16, 9, 2]
     assert [mc.mc_zigzag_pos(n) for n in range(64)] == inv
+
+
+def test_quant_division_f64_exact(mc, oracle):
+    # every step Q = max(1, (qm * qp_scale + 128) >> 8) the quantiser tables can
+    # produce (5 matrices x 2 classes x 64 positions x 2 qp tables x 96 qp)
+    qm = np.ctypeslib.as_array((ctypes.c_uint16 * (5 * 2 * 64)).in_dll(oracle, "orc_qm")).astype(np.int64)
+    sc = np.ctypeslib.as_array((ctypes.c_uint32 * (2 * 96)).in_dll(oracle, "orc_qp_scale")).astype(np.int64)
+    qs = np.unique(np.maximum(1, (qm[:, None] * sc[None, :] + 128) >> 8)).astype(np.uint32)
+    extra = np.array([1, 2, 3, 7, (1 << 21) - 1, 1 << 21, (1 << 24) + 1, (1 << 25) - 3], np.uint32)
+    qs = np.unique(np.concatenate([qs, extra]))
+    assert qs.max() > (1 << 24)
+    arr = (ctypes.c_uint32 * len(qs))(*qs.tolist())
+    assert mc.mc_qdiv_f64_check(arr, len(qs), 64) == 0
 
 
 def test_quant_division_exact(mc):

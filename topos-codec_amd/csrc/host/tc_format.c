@@ -684,9 +684,7 @@ void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bd, ui
         q->Q[i] = Q;
         q->half[i] = Q >> 1;
         q->dz[i] = i == 0 ? 0u : Q / 4u;
-        const uint64_t m = (((uint64_t)1 << 51) / Q) + 1u;
-        q->mlo[i] = (uint32_t)m;
-        q->mhi[i] = (uint32_t)(m >> 32);
+        q->rq[i] = 1.0 / (double)Q;
     }
     q->bit_depth = bd;
     q->mid = 1 << (bd - 1u);

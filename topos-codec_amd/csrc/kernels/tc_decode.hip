@@ -95,7 +95,7 @@ __device__ static inline void rr_refill(RansRd& D, uint32_t lane)
 
 __device__ static inline void rr_need(RansRd& D, uint32_t bits, uint32_t lane)
 {
-    if (D.rc < bits) { rr_refill(D, lane); }
+    if (__builtin_expect(D.rc < bits, 0)) { rr_refill(D, lane); }
 }
 
 // payload = first byte after the 4-byte initial state, n bytes
@@ -120,16 +120,20 @@ __device__ static inline void rr_init(RansRd& D, uint32_t x0, const uint8_t* pay
     D.widx += 1u;
 }
 
+typedef uint32_t tc_u32x2 __attribute__((ext_vector_type(2)));
+
 __device__ static inline void rr_norm(RansRd& D)
 {
     // x > 0 here; k bytes with k = ceil((23 - floor(log2 x)) / 8) for x < 2^23
     const uint32_t n8 = ((uint32_t)__builtin_clz(D.x) - 1u) & 0x18u;
-    const uint64_t t = ((uint64_t)D.x << 32) | (D.R >> 32);
+    const tc_u32x2 pair = { (uint32_t)(D.R >> 32), D.x };  // (state : next 32 payload bits)
+    const uint64_t t = __builtin_bit_cast(uint64_t, pair);
     D.x = (uint32_t)((t << n8) >> 32);
     D.R <<= n8;
     D.rc -= n8;
 }
 
+// decodes one symbol of the row and returns its entry
 __device__ static inline uint32_t rr_sym(RansRd& D, uint32_t cumv, uint32_t entv)
 {
     const uint32_t slot = D.x & 4095u;
@@ -137,7 +141,7 @@ __device__ static inline uint32_t rr_sym(RansRd& D, uint32_t cumv, uint32_t entv
     const uint32_t e = __builtin_amdgcn_readlane(entv, L);
     D.x = (e >> 19) * (D.x >> 12) + slot - (e & 0xFFFu);
     rr_norm(D);
-    return (e >> 12) & 63u;
+    return e;
 }
 
 __device__ static inline uint32_t rr_bits(RansRd& D, uint32_t w)
@@ -148,26 +152,29 @@ __device__ static inline uint32_t rr_bits(RansRd& D, uint32_t w)
     return c;
 }
 
-// mapped magnitude of a category: 0 for 0, else 2^(cat-1) | (cat - 1 raw bits);
-// reservoir holds >= 24 bits on entry
-__device__ static inline uint32_t rr_mag(RansRd& D, uint32_t cat, uint32_t lane)
+// Row entries: f << 19 | field << 12 | cum. For run rows the field is the
+// run (6 bits); for DC and level rows it is w | nz << 5 with w = max(cat-1, 0)
+// raw suffix bits and nz = (cat != 0), so the mapped magnitude is
+// nz << w | (w raw bits) and cat = w + nz.
+__device__ static inline uint32_t ent_w(uint32_t e) { return (e >> 12) & 31u; }
+__device__ static inline uint32_t ent_nz(uint32_t e) { return (e >> 17) & 1u; }
+
+// mapped magnitude of a DC or level symbol; reservoir holds >= 24 bits
+__device__ static inline uint32_t rr_mag(RansRd& D, uint32_t e, uint32_t lane)
 {
-    if (cat > 24u) {
-        const uint32_t b = cat - 1u;
+    const uint32_t w = ent_w(e);
+    if (w > 23u) {
         const uint32_t lo = rr_bits(D, 16u);
         rr_need(D, 16u, lane);
-        const uint32_t hi = rr_bits(D, b - 16u);
-        return (1u << b) | (hi << 16) | lo;
+        const uint32_t hi = rr_bits(D, w - 16u);
+        return (1u << w) | (hi << 16) | lo;
     }
-    const uint32_t w = max(cat, 1u) - 1u;
     const uint32_t v = rr_bits(D, w);
-    return cat == 0u ? 0u : ((1u << w) | v);
+    return (ent_nz(e) << w) | v;
 }
 
-// context rows of the level models as packed nibbles (row offset added by the caller):
-// position buckets by idx >> 3 (0, 1, 2, 2, 3, 3, 3, 3) and previous-category
-// buckets by min(cat, 7) (0, 1, 2, 2, 3, 3, 3, 3)
-#define TC_CTX_NIBBLES 0x33332210u
+// previous-category context bucket of a DC or level entry
+__device__ static inline uint32_t ctx_prev_of(uint32_t e) { return tc_ctx_prev_wnz(ent_w(e), ent_nz(e)); }
 
 template <uint32_t LM, uint32_t DM>
 __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, const uint32_t* entv,
@@ -182,19 +189,17 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, con
     const uint32_t myscan = tc_zigzag_pos(lane);
     const uint32_t run_cum = cumv[RUN_ROW], run_ent = entv[RUN_ROW];
     int32_t* out = qbuf + (size_t)S.blk0 * 64u + lane;
-    uint32_t pdc = 0u;
+    uint32_t pdc_e = 0u;
     int32_t left = 0;
     for (uint32_t by = 0; by < S.h; ++by) {
         for (uint32_t bx = 0; bx < cols; ++bx) {
             const int32_t top_v = sDcRow[bx];
             uint32_t drow = 0u;
-            if (DM) {
-                drow = (by == 0u && bx == 0u) ? 0u : 1u + ((TC_CTX_NIBBLES >> (min(pdc, 7u) * 4u)) & 0xFu);
-            }
+            if (DM) { drow = (by == 0u && bx == 0u) ? 0u : 1u + ctx_prev_of(pdc_e); }
             rr_need(D, 32u, lane);
-            const uint32_t dcat = rr_sym(D, cumv[drow], entv[drow]);
+            const uint32_t de = rr_sym(D, cumv[drow], entv[drow]);
             rr_need(D, 32u, lane);
-            const uint32_t dm = rr_mag(D, dcat, lane);
+            const uint32_t dm = rr_mag(D, de, lane);
             const int32_t top = by ? (int32_t)wuni((uint32_t)top_v) : 0;
             const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
             const int64_t dv = (int64_t)pred + (int64_t)tc_unmap_signed(dm);
@@ -202,25 +207,30 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, con
             const int32_t dc = (int32_t)dv;
             if (lane == 0u) { sDcRow[bx] = dc; }
             left = dc;
-            pdc = dcat;
+            pdc_e = de;
             uint32_t val = 0u;
-            uint32_t pos = 1u, mmax = 0u, rs;
+            uint32_t pos = 1u;
             uint32_t lrow = LM == 2u ? LVL0 + 4u : LVL0;
-            for (;;) {
-                rr_need(D, 32u, lane);
-                rs = rr_sym(D, run_cum, run_ent);
-                const uint32_t idx = pos + rs;
-                if (idx > 63u) { break; }
-                if (LM == 1u) { lrow = LVL0 + ((TC_CTX_NIBBLES >> ((idx >> 1) & 0x1Cu)) & 0xFu); }
-                const uint32_t cat = rr_sym(D, cumv[lrow], entv[lrow]);
+            rr_need(D, 32u, lane);
+            uint32_t e = rr_sym(D, run_cum, run_ent);
+            uint32_t idx = pos + ((e >> 12) & 63u);
+            // a run of 63 (end of block) or a position past 63 leaves the loop
+            while (idx <= 63u) {
+                if (LM == 1u) { lrow = LVL0 + tc_ctx_pos_nib(idx); }
+                const uint32_t el = rr_sym(D, cumv[lrow], entv[lrow]);
                 rr_need(D, 24u, lane);
-                const uint32_t m = rr_mag(D, cat, lane);
-                mmax = max(mmax, m);
+                const uint32_t m = rr_mag(D, el, lane);
                 val = myscan == idx ? m : val;
-                if (LM == 2u) { lrow = LVL0 + ((TC_CTX_NIBBLES >> (min(cat, 7u) * 4u)) & 0xFu); }
+                if (LM == 2u) { lrow = LVL0 + ctx_prev_of(el); }
                 pos = idx + 1u;
+                rr_need(D, 32u, lane);
+                e = rr_sym(D, run_cum, run_ent);
+                idx = pos + ((e >> 12) & 63u);
             }
-            bad |= (rs != 63u || mmax > lv_max) ? 1u : 0u;
+            // the loop ended on an overflow unless the run was 63; levels
+            // above the legal range anywhere in the block reject the slice
+            bad |= ((e >> 12) & 63u) != 63u ? 1u : 0u;
+            bad |= __ballot(lane != 0u && val > lv_max) != 0ull ? 1u : 0u;
             if (bad) { return -9; }
             const int32_t v = lane == 0u ? dc : tc_unmap_signed(val);
             out[((size_t)by * cols + bx) * 64u] = v;
@@ -286,6 +296,7 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
         // symbol s of row r sits in lane 63 - s
         uint32_t cumv[11], entv[11];
         const uint32_t sym = 63u - lane;
+        const uint32_t magf = (sym > 1u ? sym - 1u : 0u) | (sym != 0u ? 32u : 0u);
 #pragma unroll
         for (uint32_t r = 0; r < 11u; ++r) {
             const uint32_t nsym = r < dc_n ? 29u : (r == dc_n ? 64u : 28u);
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
             const uint32_t c = live ? (uint32_t)sCum[r][sym] : 0xFFFFu;
             const uint32_t f = live ? (uint32_t)sFreq[r][sym] : 0u;
             cumv[r] = c;
-            entv[r] = (f << 19) | (sym << 12) | (c & 0xFFFu);
+            entv[r] = (f << 19) | ((r == dc_n ? sym : magf) << 12) | (c & 0xFFFu);
         }
         const uint32_t x0 = wuni(((uint32_t)pl[prefix] << 24) | ((uint32_t)pl[prefix + 1u] << 16) |
                                  ((uint32_t)pl[prefix + 2u] << 8) | (uint32_t)pl[prefix + 3u]);

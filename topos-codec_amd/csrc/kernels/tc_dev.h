@@ -42,6 +42,7 @@ struct TcgConsts {
     double cost[4097];      // 12 - log2(f), the model-choice cost per symbol
     uint32_t crc[256];      // bytewise CRC32 table (reflected 0xEDB88320)
     uint32_t x2n[32];       // x^(2^k) mod P
+    uint32_t x16[256];      // x^(128 j) mod P: shift of a raw CRC over 16 j zero bytes
 };
 
 // host-side runtime services (tc_runtime.hip)

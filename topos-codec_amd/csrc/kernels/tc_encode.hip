@@ -39,16 +39,15 @@ __device__ static inline uint32_t find_plane(const tcg_eplane* planes, uint32_t 
 template <bool WIDE>
 __device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const tcg_eplane& P,
                                        uint32_t brow, uint32_t bx0, uint32_t nb,
-                                       int32_t* __restrict__ qbuf, int32_t* sA, uint32_t* sQ,
-                                       uint32_t* sHalf, uint32_t* sDz, uint64_t* sMag)
+                                       int32_t* __restrict__ qbuf, int32_t* sA, double* sRq,
+                                       uint32_t* sHalf, uint32_t* sDz)
 {
     const tcg_qctx* qc = &qctxs[P.qctx];
     const uint32_t t = threadIdx.x;
     if (t < 64u) {
-        sQ[t] = qc->Q[t];
+        sRq[t] = qc->rq[t];
         sHalf[t] = qc->half[t];
         sDz[t] = qc->dz[t];
-        sMag[t] = ((uint64_t)qc->mhi[t] << 32) | qc->mlo[t];
     }
     const int32_t mid = qc->mid;
     const int32_t fclamp = qc->fclamp;
@@ -111,12 +110,7 @@ __device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const
             const uint32_t mag = (uint32_t)(c < 0 ? -c : c);
             const uint32_t num = mag + sHalf[nat];
             const uint32_t ne = num > sDz[nat] ? num - sDz[nat] : 0u;
-            uint32_t qm;
-            if (WIDE) {
-                qm = ne / sQ[nat];
-            } else {
-                qm = tc_qdiv_apply(ne, sMag[nat]);
-            }
+            const uint32_t qm = tc_qdiv_f64(ne, sRq[nat]);
             qv[u] = c < 0 ? -(int32_t)qm : (int32_t)qm;
         }
     }
@@ -139,8 +133,8 @@ __global__ __launch_bounds__(256) void k_fdct_quant(const tcg_qctx* __restrict__
                                                      uint32_t nplanes, int32_t* __restrict__ qbuf)
 {
     __shared__ int32_t sA[FQ_BLOCKS * 64];
-    __shared__ uint32_t sQ[64], sHalf[64], sDz[64];
-    __shared__ uint64_t sMag[64];
+    __shared__ double sRq[64];
+    __shared__ uint32_t sHalf[64], sDz[64];
     const uint32_t tile = blockIdx.x;
     const uint32_t pi = find_plane(planes, nplanes, tile);
     const tcg_eplane P = planes[pi];
@@ -151,9 +145,9 @@ __global__ __launch_bounds__(256) void k_fdct_quant(const tcg_qctx* __restrict__
     const uint32_t bx0 = (local % tiles_per_row) * FQ_BLOCKS;
     const uint32_t nb = min((uint32_t)FQ_BLOCKS, P.bcols - bx0);
     if (qctxs[P.qctx].bit_depth > 12u) {
-        fdct_quant_tile<true>(qctxs, P, brow, bx0, nb, qbuf, sA, sQ, sHalf, sDz, sMag);
+        fdct_quant_tile<true>(qctxs, P, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
     } else {
-        fdct_quant_tile<false>(qctxs, P, brow, bx0, nb, qbuf, sA, sQ, sHalf, sDz, sMag);
+        fdct_quant_tile<false>(qctxs, P, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
     }
 }
 
@@ -198,7 +192,7 @@ __device__ static inline void emit_byte(EncEmit& E, uint32_t b, uint32_t lane)
 {
     E.acc = (E.acc << 8) | (b & 0xFFu);
     E.n += 1u;
-    if ((E.n & 3u) == 0u) {
+    if (__builtin_expect((E.n & 3u) == 0u, 0)) {
         const uint32_t k = (E.n >> 2) - 1u;  // dword index counted from the end
         const uint32_t slot = 63u - (k & 63u);
         E.obuf = lane == slot ? E.acc : E.obuf;
@@ -217,7 +211,7 @@ __device__ static inline void emit_byte(EncEmit& E, uint32_t b, uint32_t lane)
 __device__ static inline uint32_t put_sym(uint32_t x, uint32_t A, uint32_t B, uint32_t X, EncEmit& E,
                                           uint32_t lane)
 {
-    while (x >= X) { emit_byte(E, x, lane); x >>= 8; }
+    while (__builtin_expect(x >= X, 0)) { emit_byte(E, x, lane); x >>= 8; }
     const uint32_t q = __umulhi(x << 1, A) >> (B >> 24);
     return x + q * ((B >> 12) & 0xFFFu) + (B & 0xFFFu);
 }
@@ -225,7 +219,7 @@ __device__ static inline uint32_t put_sym(uint32_t x, uint32_t A, uint32_t B, ui
 __device__ static inline uint32_t put_chunk(uint32_t x, uint32_t w, uint32_t c, EncEmit& E, uint32_t lane)
 {
     const uint32_t thr = 1u << (31u - w);
-    while (x >= thr) { emit_byte(E, x, lane); x >>= 8; }
+    while (__builtin_expect(x >= thr, 0)) { emit_byte(E, x, lane); x >>= 8; }
     return (x << w) | c;
 }
 
@@ -466,7 +460,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
             m &= ~(1ull << p);
             const uint32_t s = __builtin_amdgcn_readlane(sv, p);
             const uint32_t sw = s >> 24;
-            if (sw <= 23u) {
+            if (__builtin_expect(sw <= 23u, 1)) {
                 x = put_chunk(x, sw, s & 0xFFFFFFu, E, lane);
             } else {
                 x = put_suffix_wide(x, sw, __builtin_amdgcn_readlane(lm, p), E, lane);
@@ -585,9 +579,18 @@ __global__ __launch_bounds__(256) void k_frame_scan(const TcgConsts* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// K4: per-slice payload copy into the packet + CRC32 (chunked, combined in
-// GF(2)) + 17-byte slice header.
+// K4: per-slice payload copy into the packet + CRC32 + 17-byte slice header.
+// The payload moves in 4 KiB tiles aligned to the destination: the source
+// bytes of a tile are staged in LDS with coalesced 16-byte loads, then thread
+// t owns the t-th aligned 16-byte word of the tile in the packet: it writes
+// the word (one 16-byte store when the word is whole) and computes the raw
+// CRC of its bytes, shifted to the tile end by one GF(2) multiplication
+// (x^(128 j) table; the last tile uses the general shift). The XOR of the
+// shifted word CRCs is the tile's raw CRC; tiles chain by a shift over their
+// length. The standard init / final XOR is applied once at the end.
 // ---------------------------------------------------------------------------
+#define EMIT_TILE 4096u
+
 __global__ __launch_bounds__(256) void k_slice_emit(const TcgConsts* __restrict__ K,
                                                      const tcg_eframe* __restrict__ frames,
                                                      const tcg_eslice* __restrict__ slices,
@@ -596,35 +599,81 @@ __global__ __launch_bounds__(256) void k_slice_emit(const TcgConsts* __restrict_
                                                      const tcg_eframe_out* __restrict__ fout)
 {
     __shared__ uint32_t sTab[256];
-    __shared__ uint32_t sRed[256];
+    __shared__ uint32_t sX16[256];
     __shared__ uint32_t sX2n[32];
+    __shared__ uint4 sBuf[EMIT_TILE / 16u + 2u];
+    __shared__ uint32_t sRed[4];
     const uint32_t s = blockIdx.x, t = threadIdx.x;
     const tcg_eslice S = slices[s];
     const tcg_eresult R = results[s];
     const tcg_eframe F = frames[S.frame];
     sTab[t] = K->crc[t];
+    sX16[t] = K->x16[t];
     if (t < 32u) { sX2n[t] = K->x2n[t]; }
     __syncthreads();
     if (fout[S.frame].status != 0) { return; }
     const uint8_t* src = scratch + S.scratch + R.start;
     uint8_t* dst = (uint8_t*)F.packet + R.offset;
     const uint32_t n = R.size;
-    const uint32_t per = (n + 255u) / 256u;
-    const uint32_t a = min(n, t * per), b = min(n, (t + 1u) * per);
-    uint32_t c = 0u;
-    for (uint32_t i = a; i < b; ++i) {
-        const uint8_t v = src[i];
-        dst[i] = v;
-        c = sTab[(c ^ v) & 0xFFu] ^ (c >> 8);
-    }
-    sRed[t] = a < b ? tc_crc_shift(sX2n, c, (uint64_t)(n - b)) : 0u;
-    __syncthreads();
-    for (uint32_t w = 128u; w > 0u; w >>= 1) {
-        if (t < w) { sRed[t] ^= sRed[t + w]; }
+    const uintptr_t d0 = (uintptr_t)dst, dend = d0 + n;
+    uint32_t craw = 0u;  // raw CRC of the bytes emitted so far (same in every thread)
+    for (uintptr_t T = d0 & ~(uintptr_t)(EMIT_TILE - 1u); T < dend; T += EMIT_TILE) {
+        const uintptr_t ts = T > d0 ? T : d0;
+        const uintptr_t te = T + EMIT_TILE < dend ? T + EMIT_TILE : dend;
+        const uint32_t L = (uint32_t)(te - ts);
+        // stage source bytes [ts - d0, te - d0)
+        const uintptr_t s0 = (uintptr_t)(src + (ts - d0));
+        const uintptr_t sa = s0 & ~(uintptr_t)15u;
+        const uint32_t soff = (uint32_t)(s0 - sa);
+        const uint32_t nvec = (soff + L + 15u) >> 4;
+        for (uint32_t i = t; i < nvec; i += 256u) { sBuf[i] = ((const uint4*)sa)[i]; }
+        __syncthreads();
+        const uint8_t* sb = (const uint8_t*)sBuf + soff;
+        const uintptr_t W = (ts & ~(uintptr_t)15u) + 16u * t;
+        uint32_t part = 0u;
+        if (W < te) {
+            const uintptr_t a = W > ts ? W : ts;
+            const uintptr_t b = W + 16u < te ? W + 16u : te;
+            const uint32_t o = (uint32_t)(a - ts), len = (uint32_t)(b - a);
+            uint32_t c = 0u;
+            if (len == 16u) {
+                uint32_t w4[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t v = 0u;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t byte = sb[o + 4u * k + j];
+                        c = sTab[(c ^ byte) & 0xFFu] ^ (c >> 8);
+                        v |= byte << (8 * j);
+                    }
+                    w4[k] = v;
+                }
+                *(uint4*)W = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+            } else {
+                for (uint32_t j = 0; j < len; ++j) {
+                    const uint8_t byte = sb[o + j];
+                    c = sTab[(c ^ byte) & 0xFFu] ^ (c >> 8);
+                    ((uint8_t*)a)[j] = byte;
+                }
+            }
+            const uint32_t dist = (uint32_t)(te - b);
+            if ((te & 15u) == 0u) {
+                part = tc_gf2_mulmod(sX16[dist >> 4], c);
+            } else {
+                part = tc_crc_shift(sX2n, c, dist);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) { part ^= (uint32_t)__shfl_xor((int)part, off); }
+        if ((t & 63u) == 0u) { sRed[t >> 6] = part; }
+        __syncthreads();
+        const uint32_t tcrc = sRed[0] ^ sRed[1] ^ sRed[2] ^ sRed[3];
+        craw = (L == EMIT_TILE ? tc_gf2_mulmod(sX2n[15], craw) : tc_crc_shift(sX2n, craw, L)) ^ tcrc;
         __syncthreads();
     }
     if (t == 0u) {
-        const uint32_t crc = sRed[0] ^ tc_crc_shift(sX2n, 0xFFFFFFFFu, n) ^ 0xFFFFFFFFu;
+        const uint32_t crc = craw ^ tc_crc_shift(sX2n, 0xFFFFFFFFu, n) ^ 0xFFFFFFFFu;
         results[s].crc = crc;
         uint8_t* h = dst - 17;
         h[0] = (uint8_t)(n >> 24); h[1] = (uint8_t)(n >> 16); h[2] = (uint8_t)(n >> 8); h[3] = (uint8_t)n;

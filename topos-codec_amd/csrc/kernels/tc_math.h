@@ -6,6 +6,7 @@
 #ifndef TC_MATH_H
 #define TC_MATH_H
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -84,6 +85,20 @@ TC_HD uint32_t tc_rans_enc_update(uint32_t x, uint32_t A, uint32_t B)
 {
     const uint32_t q = (uint32_t)(((uint64_t)(x << 1) * A) >> 32) >> (B >> 24);
     return x + q * ((B >> 12) & 0xFFFu) + (B & 0xFFFu);
+}
+
+/* Exact floor(n / Q) for n < 2^31, Q >= 1, from r = fl(1/Q) (IEEE division)
+ * with one fused multiply-add: q = trunc(fma(n, r, r * 2^-20)).
+ * r = (1/Q)(1 + e1) and the FMA rounds once, |e1|, |e2| <= 2^-53, so the
+ * result lies within t * 2^-52 + 2^-20/Q of t + 2^-20/Q where t = n/Q < 2^31/Q:
+ *  - t = k integer: the bias 2^-20/Q exceeds the error t * 2^-52 < 2^-21/Q,
+ *    so the result is >= k;
+ *  - t <= k + 1 - 1/Q: bias + error < 1.25 * 2^-20/Q + 2^-53 < 1/Q, so the
+ *    result stays below k + 1.
+ * (the quantiser's divide on the GPU; fp64 FMA runs at full rate there) */
+TC_HD uint32_t tc_qdiv_f64(uint32_t n, double r)
+{
+    return (uint32_t)fma((double)n, r, r * 0x1p-20);
 }
 
 /* Exact floor(n / d) for n * d < 2^51 via m = floor(2^51 / d) + 1:
@@ -207,6 +222,18 @@ TC_HD uint32_t tc_ctx_prev(uint32_t cat, uint32_t has_prev)
 TC_HD uint32_t tc_ctx_dc(uint32_t cat, uint32_t has_prev)
 {
     return has_prev == 0u ? 0u : 1u + (cat == 0u ? 0u : cat == 1u ? 1u : cat <= 3u ? 2u : 3u);
+}
+
+/* The same buckets as packed-nibble lookups (the decode kernel's form):
+ * position buckets by idx >> 3 -> (0, 1, 2, 2, 3, 3, 3, 3); previous-category
+ * buckets from the decoder's entry fields w = max(cat - 1, 0), nz = (cat != 0):
+ * cat = w + nz, 0 for cat 0, else by min(w, 7) -> (1, 2, 2, 3, 3, 3, 3, 3). */
+#define TC_CTX_POS_NIBBLES 0x33332210u
+#define TC_CTX_PREV_NIBBLES 0x33333221u
+TC_HD uint32_t tc_ctx_pos_nib(uint32_t idx) { return (TC_CTX_POS_NIBBLES >> ((idx >> 1) & 0x1Cu)) & 0xFu; }
+TC_HD uint32_t tc_ctx_prev_wnz(uint32_t w, uint32_t nz)
+{
+    return nz ? (TC_CTX_PREV_NIBBLES >> ((w < 7u ? w : 7u) * 4u)) & 0xFu : 0u;
 }
 
 TC_HD int32_t tc_dc_predict(uint32_t has_left, int32_t left, uint32_t has_top, int32_t top)

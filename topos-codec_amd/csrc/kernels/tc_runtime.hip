@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include "tcg.h"
+#include "tc_math.h"
 #include "tc_dev.h"
 
 namespace {
@@ -193,6 +194,7 @@ int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uin
     memcpy(h.cost, cost4097, sizeof(h.cost));
     memcpy(h.crc, crc256, sizeof(h.crc));
     memcpy(h.x2n, x2n32, sizeof(h.x2n));
+    for (uint32_t j = 0; j < 256u; ++j) { h.x16[j] = tc_gf2_xpow8n(x2n32, 16u * (uint64_t)j); }
     const hipError_t e = hipMemcpy(g_consts, &h, sizeof(h), hipMemcpyHostToDevice);
     return e == hipSuccess ? 0 : fail("upload tables", e);
 }

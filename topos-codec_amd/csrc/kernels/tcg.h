@@ -29,11 +29,10 @@ extern "C" {
  * zone and the 51-bit exact reciprocal m = floor(2^51/Q)+1 split in two words
  * (q = (n * m) >> 51, exact for bit depth <= 12). */
 typedef struct tcg_qctx {
+    double rq[64];       /* fl(1 / Q): the quantiser divide (tc_qdiv_f64) */
     uint32_t Q[64];
     uint32_t half[64];
     uint32_t dz[64];
-    uint32_t mlo[64];
-    uint32_t mhi[64];
     int32_t fclamp;
     uint32_t bit_depth;
     int32_t mid;
