@@ -180,10 +180,11 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
         for (uint32_t p = 0; p < P->fh.plane_count; ++p) {
             const tc_fhdr* fh = &P->fh;
             nplanes++;
+            nqctx++;
             if (!tc_is_alpha(fh, p)) {
                 P->qp_eff[p] = tc_qp_eff(fh, r->cfg, p);
-                nqctx++;
-                ntiles += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                ntiles = pt > ntiles ? pt : ntiles;
             }
             nblocks += (uint32_t)fh->bcols[p] * fh->brows[p];
             if (r->dev_planes == NULL) {
@@ -262,18 +263,18 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
             E->blk0 = blk;
             E->coded_w = fh->pcw[p];
             E->tile0 = tile;
+            /* quantiser context i belongs to plane i of the batch */
+            E->qctx = iq;
             if (tc_is_alpha(fh, p)) {
                 E->kind = 1u;
-                E->qctx = 0u;
                 E->alpha_shift = fh->alpha_mode == 2u ? 16u - fh->alpha_bit_depth : 0u;
             } else {
                 E->kind = 0u;
-                E->qctx = iq;
                 tc_qctx_build(&hq[iq], tc_qm_tab[fh->qmatrix_id][tc_is_chroma(fh, p) ? 1 : 0],
                               (uint32_t)P->qp_eff[p], fh->bit_depth, tbl);
-                iq++;
                 tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
             }
+            iq++;
             for (uint32_t y0 = 0; y0 < fh->brows[p]; y0 += P->slice_rows) {
                 const uint32_t h = fh->brows[p] - y0 < P->slice_rows ? fh->brows[p] - y0 : P->slice_rows;
                 tcg_eslice* S = &hs[is++];
@@ -315,7 +316,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
     if (tcg_encode(c->g, (const tcg_qctx*)(uintptr_t)(c->desc.p + o_q), nplanes,
                    (const tcg_eplane*)(uintptr_t)(c->desc.p + o_pl), nslices,
                    (const tcg_eslice*)(uintptr_t)(c->desc.p + o_sl), n,
-                   (const tcg_eframe*)(uintptr_t)(c->desc.p + o_fr), c->q.p, c->dcm.p, tile, c->scratch.p,
+                   (const tcg_eframe*)(uintptr_t)(c->desc.p + o_fr), c->q.p, c->dcm.p, ntiles, c->scratch.p,
                    (tcg_eresult*)(uintptr_t)(c->res.p + o_res),
                    (tcg_eframe_out*)(uintptr_t)(c->res.p + o_fo)) != 0) {
         free(plan);
@@ -398,7 +399,7 @@ int32_t tc_engine_encode(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
 int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
 {
     c->out_dev = NULL;
-    uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, nrows = 0, ndb = 0;
+    uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, maxrows = 1, ndb = 0;
     size_t pkt_bytes = 0, out_bytes = 0, tmp_bytes = 0;
     for (uint32_t f = 0; f < n; ++f) {
         const tc_fhdr* fh = &reqs[f].view->fh;
@@ -408,8 +409,11 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         for (uint32_t p = 0; p < fh->plane_count; ++p) {
             nplanes++;
             nblocks += (uint32_t)fh->bcols[p] * fh->brows[p];
-            nrows += fh->brows[p];
-            if (!tc_is_alpha(fh, p)) { ntiles += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u); }
+            maxrows = fh->brows[p] > maxrows ? fh->brows[p] : maxrows;
+            if (!tc_is_alpha(fh, p)) {
+                const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                ntiles = pt > ntiles ? pt : ntiles;
+            }
             const size_t pb = al256((size_t)fh->pvw[p] * fh->pvh[p] * 2u);
             if (reqs[f].dev_planes == NULL) { out_bytes += pb; }
             if (db && p < 3u) { tmp_bytes += pb; }
@@ -419,8 +423,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     const size_t o_pl = al256(o_q + (size_t)nslices * sizeof(tcg_qctx));
     const size_t o_sl = al256(o_pl + (size_t)nplanes * sizeof(tcg_dplane));
     const size_t o_rs = al256(o_sl + (size_t)nslices * sizeof(tcg_dslice));
-    const size_t o_db = al256(o_rs + (size_t)nrows * sizeof(uint16_t));
-    const size_t desc_bytes = al256(o_db + (size_t)(nplanes + 1u) * sizeof(uint32_t));
+    const size_t desc_bytes = al256(o_rs + (size_t)nplanes * maxrows * sizeof(tcg_rowrec));
     const size_t res_bytes = al256((size_t)nslices * sizeof(int32_t));
     int32_t rc;
     if ((rc = dbuf_need(c, &c->q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
@@ -438,8 +441,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     tcg_qctx* hq = (tcg_qctx*)(c->hdesc + o_q);
     tcg_dplane* hp = (tcg_dplane*)(c->hdesc + o_pl);
     tcg_dslice* hs = (tcg_dslice*)(c->hdesc + o_sl);
-    uint16_t* hrs = (uint16_t*)(c->hdesc + o_rs);
-    uint32_t* hdb = (uint32_t*)(c->hdesc + o_db);
+    tcg_rowrec* hrs = (tcg_rowrec*)(c->hdesc + o_rs);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0;
     size_t pkt_off = 0, out_off = 0, tmp_off = 0;
     uint64_t* out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
@@ -538,18 +540,23 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                     S->qctx = qidx[k];
                 }
             }
-            for (uint32_t y = 0; y < h->h; ++y) { hrs[D->row_slice + h->y0 + y] = (uint16_t)is; }
+            for (uint32_t y = 0; y < h->h; ++y) {
+                tcg_rowrec* R = &hrs[(size_t)(ip0 + p) * maxrows + h->y0 + y];
+                R->qctx = S->qctx;
+                R->slice = is;
+            }
             is++;
         }
         ip += fh->plane_count;
     }
-    /* deblock tile table: strips x 256-column tiles per filtered plane */
+    /* deblock grid extent: strips x 256-column tiles of the largest filtered plane */
     uint32_t dbt = 0;
     for (uint32_t i = 0; i < ip; ++i) {
-        hdb[i] = dbt;
-        if (hp[i].deblock) { dbt += (hp[i].vis_h / 8u + 1u) * ((hp[i].vis_w + 255u) / 256u); }
+        if (hp[i].deblock) {
+            const uint32_t t = (hp[i].vis_h / 8u + 1u) * ((hp[i].vis_w + 255u) / 256u);
+            dbt = t > dbt ? t : dbt;
+        }
     }
-    hdb[ip] = dbt;
     (void)ndb;
     if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(out_dev);
@@ -558,8 +565,8 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     if (tcg_decode(c->g, (const tcg_qctx*)(uintptr_t)(c->desc.p + o_q), nplanes,
                    (const tcg_dplane*)(uintptr_t)(c->desc.p + o_pl), nslices,
                    (const tcg_dslice*)(uintptr_t)(c->desc.p + o_sl),
-                   (const uint16_t*)(uintptr_t)(c->desc.p + o_rs), c->q.p, tile,
-                   (int32_t*)(uintptr_t)c->res.p, dbt, (const uint32_t*)(uintptr_t)(c->desc.p + o_db)) != 0) {
+                   (const tcg_rowrec*)(uintptr_t)(c->desc.p + o_rs), maxrows, c->q.p, ntiles,
+                   (int32_t*)(uintptr_t)c->res.p, dbt) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode launch failed: %s", tcg_last_error());
     }

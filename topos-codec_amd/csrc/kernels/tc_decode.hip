@@ -338,16 +338,6 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
 // ---------------------------------------------------------------------------
 #define IQ_BLOCKS 32
 
-__device__ static inline uint32_t find_dplane(const tcg_dplane* planes, uint32_t nplanes, uint32_t tile)
-{
-    uint32_t lo = 0, hi = nplanes - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (planes[mid].tile0 <= tile) { lo = mid; } else { hi = mid - 1; }
-    }
-    return lo;
-}
-
 template <bool WIDE>
 __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __restrict__ qc,
                                          uint32_t brow, uint32_t bx0, uint32_t nb, bool conceal,
@@ -429,28 +419,30 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
     }
 }
 
+// grid: x = tile (32 blocks of a block row) within the plane, y = plane of the
+// batch. The row record (quantiser context, slice) loads beside the plane
+// descriptor; the context and the slice status then load together.
 __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict__ qctxs,
                                                        const tcg_dplane* __restrict__ planes,
-                                                       uint32_t nplanes,
-                                                       const tcg_dslice* __restrict__ slices,
-                                                       const uint16_t* __restrict__ row_slice,
+                                                       const tcg_rowrec* __restrict__ rowrec,
+                                                       uint32_t max_rows,
                                                        const int32_t* __restrict__ status,
                                                        const int32_t* __restrict__ qbuf)
 {
     __shared__ uint64_t sB[IQ_BLOCKS * 65];
     __shared__ uint32_t sQ[64];
-    const uint32_t tile = blockIdx.x;
-    const uint32_t pi = find_dplane(planes, nplanes, tile);
+    const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (P.kind != 0u) { return; }
     const uint32_t tiles_per_row = (P.bcols + IQ_BLOCKS - 1u) / IQ_BLOCKS;
-    const uint32_t local = tile - P.tile0;
+    const uint32_t local = blockIdx.x;
+    if (local >= P.brows * tiles_per_row) { return; }
     const uint32_t brow = local / tiles_per_row;
     const uint32_t bx0 = (local % tiles_per_row) * IQ_BLOCKS;
     const uint32_t nb = min((uint32_t)IQ_BLOCKS, P.bcols - bx0);
-    const uint32_t si = row_slice[P.row_slice + brow];
-    const tcg_qctx* qc = &qctxs[slices[si].qctx];
-    const bool conceal = status[si] != 0;
+    const tcg_rowrec rr = rowrec[(size_t)pi * max_rows + brow];
+    const tcg_qctx* qc = &qctxs[rr.qctx];
+    const bool conceal = status[rr.slice] != 0;
     if (qc->bit_depth > 12u) {
         dequant_idct_tile<true>(P, qc, brow, bx0, nb, conceal, qbuf, sB, sQ);
     } else {
@@ -490,25 +482,17 @@ __device__ static inline void db_edge(int32_t& p1, int32_t& p0, int32_t& q0, int
     p0 = np0; q0 = nq0; p1 = np1; q1 = nq1;
 }
 
-__global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ planes, uint32_t nplanes,
-                                                  const uint32_t* __restrict__ tile0)
+// grid: x = tile (strip x 256 columns) within the plane, y = plane of the batch
+__global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ planes)
 {
     __shared__ int32_t sP[8][DB_COLS + 4];
-    const uint32_t tile = blockIdx.x;
-    uint32_t pi = 0;
-    {
-        uint32_t lo = 0, hi = nplanes - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (tile0[mid] <= tile) { lo = mid; } else { hi = mid - 1; }
-        }
-        pi = lo;
-    }
+    const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (!P.deblock) { return; }
     const uint32_t w = P.vis_w, h = P.vis_h;
     const uint32_t col_tiles = (w + DB_COLS - 1u) / DB_COLS;
-    const uint32_t local = tile - tile0[pi];
+    const uint32_t local = blockIdx.x;
+    if (local >= (h / 8u + 1u) * col_tiles) { return; }
     const uint32_t r = local / col_tiles;
     const uint32_t c0 = (local % col_tiles) * DB_COLS;
     const int32_t ys = (int32_t)(8u * r) - 2 < 0 ? 0 : (int32_t)(8u * r) - 2;
@@ -562,9 +546,9 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
 // host launch
 // ---------------------------------------------------------------------------
 extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
-                              uint32_t nslices, const tcg_dslice* slices_dev,
-                              const uint16_t* row_slice_dev, uint64_t qbuf, uint32_t total_tiles,
-                              int32_t* slice_status_dev, uint32_t db_tiles, const uint32_t* db_tile0_dev)
+                              uint32_t nslices, const tcg_dslice* slices_dev, const tcg_rowrec* rowrec_dev,
+                              uint32_t max_rows, uint64_t qbuf, uint32_t max_plane_tiles,
+                              int32_t* slice_status_dev, uint32_t max_db_tiles)
 {
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
@@ -574,13 +558,13 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
                        (int32_t*)qbuf, slice_status_dev);
     tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
     tcg_mark(c, 1);
-    if (total_tiles > 0u) {
-        hipLaunchKernelGGL(k_dequant_idct, dim3(total_tiles), dim3(256), 0, st, qctx_dev, planes_dev,
-                           nplanes, slices_dev, row_slice_dev, slice_status_dev, (const int32_t*)qbuf);
+    if (max_plane_tiles > 0u) {
+        hipLaunchKernelGGL(k_dequant_idct, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev, planes_dev,
+                           rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
     }
     tcg_mark(c, 2);
-    if (db_tiles > 0u) {
-        hipLaunchKernelGGL(k_deblock, dim3(db_tiles), dim3(256), 0, st, planes_dev, nplanes, db_tile0_dev);
+    if (max_db_tiles > 0u) {
+        hipLaunchKernelGGL(k_deblock, dim3(max_db_tiles, nplanes), dim3(256), 0, st, planes_dev);
     }
     tcg_mark(c, 3);
     return tcg_check("decode launch");

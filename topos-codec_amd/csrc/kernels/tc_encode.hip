@@ -24,33 +24,17 @@
 // and written out as one contiguous 8 KB span.
 // ---------------------------------------------------------------------------
 #define FQ_BLOCKS 32
-
-__device__ static inline uint32_t find_plane(const tcg_eplane* planes, uint32_t nplanes, uint32_t tile)
-{
-    // planes[i].tile0 holds the first tile index of plane i (ascending)
-    uint32_t lo = 0, hi = nplanes - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (planes[mid].tile0 <= tile) { lo = mid; } else { hi = mid - 1; }
-    }
-    return lo;
-}
+// LDS pitch of one block's 64 words: 65 puts the blocks of a half-wave in
+// different banks for the row/column transposes
+#define FQ_PITCH 65u
 
 template <bool WIDE>
-__device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const tcg_eplane& P,
+__device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t fclamp,
                                        uint32_t brow, uint32_t bx0, uint32_t nb,
-                                       int32_t* __restrict__ qbuf, int32_t* sA, double* sRq,
-                                       uint32_t* sHalf, uint32_t* sDz)
+                                       int32_t* __restrict__ qbuf, int32_t* sA, const double* sRq,
+                                       const uint32_t* sHalf, const uint32_t* sDz)
 {
-    const tcg_qctx* qc = &qctxs[P.qctx];
     const uint32_t t = threadIdx.x;
-    if (t < 64u) {
-        sRq[t] = qc->rq[t];
-        sHalf[t] = qc->half[t];
-        sDz[t] = qc->dz[t];
-    }
-    const int32_t mid = qc->mid;
-    const int32_t fclamp = qc->fclamp;
 
     // ---- pass 1: rows ----
     {
@@ -87,7 +71,7 @@ __device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const
                 int32_t acc = 0;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) { acc += __mul24(x[k], (int32_t)kTM[v * 8 + k]); }
-                sA[j * 64 + r * 8 + v] = acc;
+                sA[j * FQ_PITCH + r * 8 + v] = acc;
             }
         }
     }
@@ -98,7 +82,7 @@ __device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const
     if (j < nb) {
         int32_t a[8];
 #pragma unroll
-        for (int y = 0; y < 8; ++y) { a[y] = sA[j * 64 + y * 8 + v]; }
+        for (int y = 0; y < 8; ++y) { a[y] = sA[j * FQ_PITCH + y * 8 + v]; }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             int32_t F = 0;
@@ -117,37 +101,49 @@ __device__ static void fdct_quant_tile(const tcg_qctx* __restrict__ qctxs, const
     __syncthreads();
     if (j < nb) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) { sA[j * 64 + kZigzagInv[u * 8 + v]] = qv[u]; }
+        for (int u = 0; u < 8; ++u) { sA[j * FQ_PITCH + kZigzagInv[u * 8 + v]] = qv[u]; }
     }
     __syncthreads();
     // contiguous store of nb * 64 coefficients (scan order per block)
     int32_t* dst = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0) * 64u;
     const uint32_t total = nb * 64u;
     for (uint32_t i = t * 4u; i < total; i += 256u * 4u) {
-        *(int4*)(dst + i) = *(const int4*)(sA + i);
+        const int32_t* s4 = sA + (i >> 6) * FQ_PITCH + (i & 63u);
+        *(int4*)(dst + i) = make_int4(s4[0], s4[1], s4[2], s4[3]);
     }
 }
 
+// grid: x = tile (32 blocks of a block row) within the plane, y = plane of the
+// batch; quantiser contexts are indexed by plane, so the context and the plane
+// descriptor load in parallel and no tile -> plane search is needed
 __global__ __launch_bounds__(256) void k_fdct_quant(const tcg_qctx* __restrict__ qctxs,
                                                      const tcg_eplane* __restrict__ planes,
-                                                     uint32_t nplanes, int32_t* __restrict__ qbuf)
+                                                     int32_t* __restrict__ qbuf)
 {
-    __shared__ int32_t sA[FQ_BLOCKS * 64];
+    __shared__ int32_t sA[FQ_BLOCKS * FQ_PITCH];
     __shared__ double sRq[64];
     __shared__ uint32_t sHalf[64], sDz[64];
-    const uint32_t tile = blockIdx.x;
-    const uint32_t pi = find_plane(planes, nplanes, tile);
+    const uint32_t pi = blockIdx.y, t = threadIdx.x;
+    const tcg_qctx* qc = &qctxs[pi];
+    if (t < 64u) {
+        sRq[t] = qc->rq[t];
+        sHalf[t] = qc->half[t];
+        sDz[t] = qc->dz[t];
+    }
+    const uint32_t bd = qc->bit_depth;
+    const int32_t mid = qc->mid, fclamp = qc->fclamp;
     const tcg_eplane P = planes[pi];
     if (P.kind != 0u) { return; }
     const uint32_t tiles_per_row = (P.bcols + FQ_BLOCKS - 1u) / FQ_BLOCKS;
-    const uint32_t local = tile - P.tile0;
+    const uint32_t local = blockIdx.x;
+    if (local >= P.brows * tiles_per_row) { return; }
     const uint32_t brow = local / tiles_per_row;
     const uint32_t bx0 = (local % tiles_per_row) * FQ_BLOCKS;
     const uint32_t nb = min((uint32_t)FQ_BLOCKS, P.bcols - bx0);
-    if (qctxs[P.qctx].bit_depth > 12u) {
-        fdct_quant_tile<true>(qctxs, P, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
+    if (bd > 12u) {
+        fdct_quant_tile<true>(P, mid, fclamp, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
     } else {
-        fdct_quant_tile<false>(qctxs, P, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
+        fdct_quant_tile<false>(P, mid, fclamp, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
     }
 }
 
@@ -693,16 +689,16 @@ __global__ __launch_bounds__(256) void k_slice_emit(const TcgConsts* __restrict_
 extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
                               uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
                               const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
-                              uint32_t total_tiles, uint64_t scratch, tcg_eresult* results_dev,
+                              uint32_t max_plane_tiles, uint64_t scratch, tcg_eresult* results_dev,
                               tcg_eframe_out* frame_out_dev)
 {
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 1;
     tcg_mark(c, 0);
-    if (total_tiles > 0u) {
-        hipLaunchKernelGGL(k_fdct_quant, dim3(total_tiles), dim3(256), 0, st, qctx_dev, planes_dev,
-                           nplanes, (int32_t*)qbuf);
+    if (max_plane_tiles > 0u) {
+        hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev, planes_dev,
+                           (int32_t*)qbuf);
     }
     tcg_mark(c, 1);
     hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,

@@ -125,6 +125,13 @@ typedef struct tcg_dplane {
     uint32_t pad_;
 } tcg_dplane;
 
+/* per (plane of the batch, block row): the quantiser context and the slice
+ * that codes the row (decode) */
+typedef struct tcg_rowrec {
+    uint32_t qctx;
+    uint32_t slice;
+} tcg_rowrec;
+
 /* ---- runtime ---- */
 /* A context is one default-priority stream plus its timing events; work on
  * different contexts runs concurrently on the device. */
@@ -156,14 +163,16 @@ int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uin
 int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
                    uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
                    const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
-                   uint32_t total_tiles, uint64_t scratch, tcg_eresult* results_dev,
+                   uint32_t max_plane_tiles, uint64_t scratch, tcg_eresult* results_dev,
                    tcg_eframe_out* frame_out_dev);
 
 /* ---- decode ---- */
+/* rowrec_dev: nplanes x max_rows records; max_plane_tiles / max_db_tiles:
+ * largest transform / deblock tile count of one plane (grid x extents) */
 int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
-                   uint32_t nslices, const tcg_dslice* slices_dev, const uint16_t* row_slice_dev,
-                   uint64_t qbuf, uint32_t total_tiles, int32_t* slice_status_dev,
-                   uint32_t db_tiles, const uint32_t* db_tile0_dev);
+                   uint32_t nslices, const tcg_dslice* slices_dev, const tcg_rowrec* rowrec_dev,
+                   uint32_t max_rows, uint64_t qbuf, uint32_t max_plane_tiles, int32_t* slice_status_dev,
+                   uint32_t max_db_tiles);
 
 /* per-kernel times in milliseconds (HIP events on the launch stream) of the
  * context's last launch sequence; valid after tcg_sync when timing is on */
