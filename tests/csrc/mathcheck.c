@@ -109,3 +109,45 @@ uint32_t mc_ctx_tables_check(void)
     }
     return bad;
 }
+
+/* butterfly 8-point transforms against the direct products with the matrix
+ * given in m[64] (row-major), for n pseudo-random vectors bounded by lim_f
+ * (forward, int32) and lim_i (inverse); returns mismatches */
+static const int mc_M[64] = {
+    13, 13, 13, 13, 13, 13, 13, 13, 13, 11, 7, 1, -1, -7, -11, -13,
+    12, 5, -5, -12, -12, -5, 5, 12, 11, -4, -13, -8, 8, 13, 4, -11,
+    9, -9, -9, 9, 9, -9, -9, 9, 8, -13, 4, 11, -11, -4, 13, -8,
+    5, -12, 12, -5, -5, 12, -12, 5, 1, -7, 11, -13, 13, -11, 7, -1,
+};
+uint32_t mc_dct8_check(const int32_t* m, uint32_t n, int32_t lim_f, int64_t lim_i)
+{
+    uint32_t bad = 0u;
+    uint64_t s = 0x853C49E6748FEA9Bull;
+    for (int i = 0; i < 64; ++i) { bad += m[i] != mc_M[i]; }
+    for (uint32_t it = 0; it < n; ++it) {
+        int32_t x[8], f[8];
+        int64_t gi[8], oi[8];
+        double gd[8], od[8];
+        for (int k = 0; k < 8; ++k) {
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            const int32_t e = it < 4u ? (it & 1u ? lim_f : -lim_f) : (int32_t)(s % (2u * (uint64_t)lim_f + 1u)) - lim_f;
+            x[k] = (it == 2u || it == 3u) ? ((k & 1) ? -e : e) : e;
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            const int64_t g = it < 4u ? (it & 1u ? lim_i : -lim_i) : (int64_t)(s % (2u * (uint64_t)lim_i + 1u)) - lim_i;
+            gi[k] = (it == 2u || it == 3u) ? ((k & 1) ? -g : g) : g;
+            gd[k] = (double)gi[k];
+        }
+        tc_fdct8_i32(x, f);
+        tc_idct8_i64(gi, oi);
+        tc_idct8_f64(gd, od);
+        for (int v = 0; v < 8; ++v) {
+            int64_t acc = 0, acci = 0;
+            for (int k = 0; k < 8; ++k) { acc += (int64_t)mc_M[v * 8 + k] * x[k]; }
+            for (int k = 0; k < 8; ++k) { acci += (int64_t)mc_M[k * 8 + v] * gi[k]; }
+            bad += (int64_t)f[v] != acc;
+            bad += oi[v] != acci;
+            bad += od[v] != (double)acci;
+        }
+    }
+    return bad;
+}

@@ -30,6 +30,8 @@ def mc(tmp_path_factory):
     lib.mc_zigzag_pos.restype = ctypes.c_uint32
     lib.mc_qdiv_f64_check.restype = ctypes.c_uint32
     lib.mc_ctx_tables_check.restype = ctypes.c_uint32
+    lib.mc_dct8_check.restype = ctypes.c_uint32
+    lib.mc_dct8_check.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int64]
     return lib
 
 
@@ -48,6 +50,15 @@ def test_rans_encoder_update_exact(mc):
     # x' = (x / f) * 4096 + x % f + cum through the reciprocal constants the
     # encode kernel uses, for every frequency and states up to the bound
     assert mc.mc_enc_update_check(2000) == 0
+
+
+def test_butterfly_transforms_exact(mc, oracle):
+    # the kernels' even/odd 8-point transforms equal the direct matrix products
+    # (the oracle's matrix), forward in int32 for inputs up to 2^22 and inverse
+    # in int64 and f64 for inputs up to 2^45 (all sums stay below 2^53)
+    m = np.ctypeslib.as_array((ctypes.c_int16 * 64).in_dll(oracle, "orc_M")).astype(np.int32)
+    arr = (ctypes.c_int32 * 64)(*m.tolist())
+    assert mc.mc_dct8_check(arr, 200000, (1 << 22), ctypes.c_int64(1 << 45)) == 0
 
 
 def test_context_lookup_tables(mc):

@@ -183,7 +183,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
             nqctx++;
             if (!tc_is_alpha(fh, p)) {
                 P->qp_eff[p] = tc_qp_eff(fh, r->cfg, p);
-                const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + TCG_FQ_BLOCKS - 1u) / TCG_FQ_BLOCKS);
                 ntiles = pt > ntiles ? pt : ntiles;
             }
             nblocks += (uint32_t)fh->bcols[p] * fh->brows[p];

@@ -338,6 +338,10 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
 // ---------------------------------------------------------------------------
 #define IQ_BLOCKS 32
 
+// 8-point inverse transform (exact even/odd form of out[x] = sum_v in[v] M[v][x])
+__device__ static inline void idct8(const double* in, double* out) { tc_idct8_f64(in, out); }
+__device__ static inline void idct8(const int64_t* in, int64_t* out) { tc_idct8_i64(in, out); }
+
 template <bool WIDE>
 __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __restrict__ qc,
                                          uint32_t brow, uint32_t bx0, uint32_t nb, bool conceal,
@@ -378,13 +382,10 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
                 }
                 G[v] = (acc_t)d * (acc_t)kTW[u * 8 + v];
             }
+            acc_t Bv[8];
+            idct8(G, Bv);
 #pragma unroll
-            for (int x = 0; x < 8; ++x) {
-                acc_t acc = 0;
-#pragma unroll
-                for (int v = 0; v < 8; ++v) { acc += G[v] * (acc_t)kTM[v * 8 + x]; }
-                sB[j * 65 + u * 8 + x] = acc;
-            }
+            for (int x = 0; x < 8; ++x) { sB[j * 65 + u * 8 + x] = Bv[x]; }
         }
     }
     __syncthreads();
@@ -395,11 +396,11 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
 #pragma unroll
             for (int u = 0; u < 8; ++u) { B[u] = sB[j * 65 + u * 8 + x]; }
             const uint32_t px = (bx0 + j) * 8u + x;
+            acc_t Xv[8];
+            idct8(B, Xv);
 #pragma unroll
             for (int y = 0; y < 8; ++y) {
-                acc_t acc = 0;
-#pragma unroll
-                for (int u = 0; u < 8; ++u) { acc += (acc_t)kTM[u * 8 + y] * B[u]; }
+                const acc_t acc = Xv[y];
                 int32_t r;
                 if (WIDE) {
                     const int64_t v = (int64_t)acc;

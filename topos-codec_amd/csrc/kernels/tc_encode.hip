@@ -16,14 +16,21 @@
 
 // ---------------------------------------------------------------------------
 // K1: fused FDCT + quantisation.
-// One workgroup = 256 threads = one tile of 32 blocks along a block row.
-// Pass 1 (rows): thread (r, j) loads row r of block j (8 u16 = 16 B, the 32
-// threads of a row read 512 contiguous bytes), computes A[r][v].
-// Pass 2 (columns) through LDS: thread (v, j) computes F[u][v], u = 0..7, and
-// quantises; the 32 x 64 coefficients are staged in LDS in scan (zigzag) order
-// and written out as one contiguous 8 KB span.
+// One workgroup = 8 x FQ_BLOCKS threads = one tile of FQ_BLOCKS blocks along a
+// block row. Pass 1 (rows): thread (r, j) loads row r of block j (8 u16 =
+// 16 B; the threads of a row read 16 x FQ_BLOCKS contiguous bytes) and
+// computes A[r][v]. Pass 2 (columns) through LDS: thread (v, j) computes
+// F[u][v], u = 0..7, and quantises; the coefficients are staged in LDS in
+// scan (zigzag) order and leave as one contiguous span of 256 B per block.
 // ---------------------------------------------------------------------------
-#define FQ_BLOCKS 32
+#define FQ_BLOCKS TCG_FQ_BLOCKS
+#define FQ_THREADS (8u * FQ_BLOCKS)
+#ifndef FQ_NT_STORE
+#define FQ_NT_STORE 0
+#endif
+#ifndef FQ_ZZ_ARITH
+#define FQ_ZZ_ARITH 0
+#endif
 // LDS pitch of one block's 64 words: 65 puts the blocks of a half-wave in
 // different banks for the row/column transposes
 #define FQ_PITCH 65u
@@ -38,7 +45,7 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t
 
     // ---- pass 1: rows ----
     {
-        const uint32_t r = t >> 5, j = t & 31u;
+        const uint32_t r = t / FQ_BLOCKS, j = t % FQ_BLOCKS;
         if (j < nb) {
             const uint16_t* src = (const uint16_t*)P.src;
             uint32_t y = brow * 8u + r;
@@ -66,28 +73,25 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t
 #pragma unroll
                 for (int k = 0; k < 8; ++k) { x[k] = (int32_t)(int16_t)x[k]; }
             }
+            int32_t A[8];
+            tc_fdct8_i32(x, A);
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                int32_t acc = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) { acc += __mul24(x[k], (int32_t)kTM[v * 8 + k]); }
-                sA[j * FQ_PITCH + r * 8 + v] = acc;
-            }
+            for (int v = 0; v < 8; ++v) { sA[j * FQ_PITCH + r * 8 + v] = A[v]; }
         }
     }
     __syncthreads();
     // ---- pass 2: columns + quantisation ----
     int32_t qv[8];
-    const uint32_t v = t >> 5, j = t & 31u;
+    const uint32_t v = t / FQ_BLOCKS, j = t % FQ_BLOCKS;
     if (j < nb) {
         int32_t a[8];
 #pragma unroll
         for (int y = 0; y < 8; ++y) { a[y] = sA[j * FQ_PITCH + y * 8 + v]; }
+        int32_t Fc[8];
+        tc_fdct8_i32(a, Fc);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            int32_t F = 0;
-#pragma unroll
-            for (int y = 0; y < 8; ++y) { F += __mul24((int32_t)kTM[u * 8 + y], a[y]); }
+            int32_t F = Fc[u];
             const uint32_t nat = (uint32_t)u * 8u + v;
             if (WIDE) { F = predistort(F, nat); }
             int32_t c = F > fclamp ? fclamp : (F < -fclamp ? -fclamp : F);
@@ -101,22 +105,34 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t
     __syncthreads();
     if (j < nb) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) { sA[j * FQ_PITCH + kZigzagInv[u * 8 + v]] = qv[u]; }
+        for (int u = 0; u < 8; ++u) {
+#if FQ_ZZ_ARITH
+            sA[j * FQ_PITCH + tc_zigzag_pos((uint32_t)u * 8u + v)] = qv[u];
+#else
+            sA[j * FQ_PITCH + kZigzagInv[u * 8 + v]] = qv[u];
+#endif
+        }
     }
     __syncthreads();
     // contiguous store of nb * 64 coefficients (scan order per block)
     int32_t* dst = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0) * 64u;
     const uint32_t total = nb * 64u;
-    for (uint32_t i = t * 4u; i < total; i += 256u * 4u) {
+    for (uint32_t i = t * 4u; i < total; i += FQ_THREADS * 4u) {
         const int32_t* s4 = sA + (i >> 6) * FQ_PITCH + (i & 63u);
+#if FQ_NT_STORE
+        typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+        const i32x4 v4 = { s4[0], s4[1], s4[2], s4[3] };
+        __builtin_nontemporal_store(v4, (i32x4*)(dst + i));
+#else
         *(int4*)(dst + i) = make_int4(s4[0], s4[1], s4[2], s4[3]);
+#endif
     }
 }
 
 // grid: x = tile (32 blocks of a block row) within the plane, y = plane of the
 // batch; quantiser contexts are indexed by plane, so the context and the plane
 // descriptor load in parallel and no tile -> plane search is needed
-__global__ __launch_bounds__(256) void k_fdct_quant(const tcg_qctx* __restrict__ qctxs,
+__global__ __launch_bounds__(FQ_THREADS) void k_fdct_quant(const tcg_qctx* __restrict__ qctxs,
                                                      const tcg_eplane* __restrict__ planes,
                                                      int32_t* __restrict__ qbuf)
 {
@@ -697,7 +713,7 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     c->kind = 1;
     tcg_mark(c, 0);
     if (max_plane_tiles > 0u) {
-        hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev, planes_dev,
+        hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0, st, qctx_dev, planes_dev,
                            (int32_t*)qbuf);
     }
     tcg_mark(c, 1);

@@ -202,6 +202,48 @@ TC_HD int32_t tc_unmap_signed(uint32_t m)
     return (m & 1u) == 0u ? (int32_t)(m >> 1) : -(int32_t)(((uint64_t)m + 1u) >> 1);
 }
 
+/* 8-point transform with the integer matrix M of the codec (rows: 13 13 13 13
+ * 13 13 13 13 / 13 11 7 1 -1 -7 -11 -13 / 12 5 -5 -12 -12 -5 5 12 / 11 -4 -13
+ * -8 8 13 4 -11 / 9 -9 -9 9 9 -9 -9 9 / 8 -13 4 11 -11 -4 13 -8 / 5 -12 12 -5
+ * -5 12 -12 5 / 1 -7 11 -13 13 -11 7 -1): even rows are symmetric and odd rows
+ * antisymmetric, so out = M * in splits into sums s_k = in_k + in_{7-k} (even
+ * rows) and differences d_k = in_k - in_{7-k} (odd rows); rows 0/4 and 2/6
+ * split once more. Every step is an exact integer operation, so the result is
+ * identical to the direct product (36 operations instead of 64).
+ * Forward: out[v] = sum_k M[v][k] in[k]. */
+TC_HD void tc_fdct8_i32(const int32_t* in, int32_t* out)
+{
+    const int32_t s0 = in[0] + in[7], s1 = in[1] + in[6], s2 = in[2] + in[5], s3 = in[3] + in[4];
+    const int32_t d0 = in[0] - in[7], d1 = in[1] - in[6], d2 = in[2] - in[5], d3 = in[3] - in[4];
+    const int32_t e0 = s0 + s3, e1 = s1 + s2, f0 = s0 - s3, f1 = s1 - s2;
+    out[0] = 13 * (e0 + e1);
+    out[4] = 9 * (e0 - e1);
+    out[2] = 12 * f0 + 5 * f1;
+    out[6] = 5 * f0 - 12 * f1;
+    out[1] = 13 * d0 + 11 * d1 + 7 * d2 + 1 * d3;
+    out[3] = 11 * d0 - 4 * d1 - 13 * d2 - 8 * d3;
+    out[5] = 8 * d0 - 13 * d1 + 4 * d2 + 11 * d3;
+    out[7] = 1 * d0 - 7 * d1 + 11 * d2 - 13 * d3;
+}
+
+/* Inverse direction: out[x] = sum_v in[v] M[v][x] (the transpose), with
+ * out[x] = E_x + O_x and out[7-x] = E_x - O_x. T is double (all values are
+ * integers below 2^53, so every sum is exact) or int64_t. */
+#define TC_IDCT8_BODY(T)                                                          \
+    const T p = (T)13 * in[0] + (T)9 * in[4], q = (T)13 * in[0] - (T)9 * in[4];    \
+    const T r = (T)12 * in[2] + (T)5 * in[6], t = (T)5 * in[2] - (T)12 * in[6];    \
+    const T e0 = p + r, e3 = p - r, e1 = q + t, e2 = q - t;                         \
+    const T o0 = (T)13 * in[1] + (T)11 * in[3] + (T)8 * in[5] + (T)1 * in[7];       \
+    const T o1 = (T)11 * in[1] - (T)4 * in[3] - (T)13 * in[5] - (T)7 * in[7];       \
+    const T o2 = (T)7 * in[1] - (T)13 * in[3] + (T)4 * in[5] + (T)11 * in[7];       \
+    const T o3 = (T)1 * in[1] - (T)8 * in[3] + (T)11 * in[5] - (T)13 * in[7];       \
+    out[0] = e0 + o0; out[7] = e0 - o0;                                             \
+    out[1] = e1 + o1; out[6] = e1 - o1;                                             \
+    out[2] = e2 + o2; out[5] = e2 - o2;                                             \
+    out[3] = e3 + o3; out[4] = e3 - o3;
+TC_HD void tc_idct8_f64(const double* in, double* out) { TC_IDCT8_BODY(double) }
+TC_HD void tc_idct8_i64(const int64_t* in, int64_t* out) { TC_IDCT8_BODY(int64_t) }
+
 /* Scan (zigzag) position of natural coefficient index n = 8r + c in the
  * 8x8 zigzag order that starts along the first row: diagonal d = r + c holds
  * positions from d(d+1)/2 (d < 8) or 64 - (15-d)(16-d)/2 (d >= 8); even

@@ -28,6 +28,12 @@ extern "C" {
 /* quantiser context of one (plane, qp): natural-order step, half step, dead
  * zone and the 51-bit exact reciprocal m = floor(2^51/Q)+1 split in two words
  * (q = (n * m) >> 51, exact for bit depth <= 12). */
+/* encode transform tile: blocks of one block row per workgroup (the engine
+ * sizes the grid with it) */
+#ifndef TCG_FQ_BLOCKS
+#define TCG_FQ_BLOCKS 32u
+#endif
+
 typedef struct tcg_qctx {
     double rq[64];       /* fl(1 / Q): the quantiser divide (tc_qdiv_f64) */
     uint32_t Q[64];
