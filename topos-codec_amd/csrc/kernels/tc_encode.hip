@@ -28,47 +28,67 @@
 #ifndef FQ_NT_STORE
 #define FQ_NT_STORE 0
 #endif
-#ifndef FQ_ZZ_ARITH
-#define FQ_ZZ_ARITH 0
-#endif
 // LDS pitch of one block's 64 words: 65 puts the blocks of a half-wave in
 // different banks for the row/column transposes
 #define FQ_PITCH 65u
 
+struct FqTile {
+    uint32_t brow, bx0, nb;
+    bool live;
+};
+
+__device__ static inline FqTile fq_tile(const tcg_eplane& P, uint32_t local)
+{
+    FqTile T;
+    const uint32_t tiles_per_row = (P.bcols + FQ_BLOCKS - 1u) / FQ_BLOCKS;
+    T.live = local < P.brows * tiles_per_row;
+    T.brow = T.live ? local / tiles_per_row : 0u;
+    T.bx0 = T.live ? (local % tiles_per_row) * FQ_BLOCKS : 0u;
+    T.nb = T.live ? min((uint32_t)FQ_BLOCKS, P.bcols - T.bx0) : 0u;
+    return T;
+}
+
+// level-shifted samples of row r of block j of the tile (rows past the
+// visible height repeat the last row, columns past the width the last column)
+__device__ static inline void fq_load(const tcg_eplane& P, const FqTile& T, int32_t mid, uint32_t t,
+                                      int32_t* x)
+{
+    const uint32_t r = t / FQ_BLOCKS, j = t % FQ_BLOCKS;
+    if (!T.live || j >= T.nb) { return; }
+    const uint16_t* src = (const uint16_t*)P.src;
+    uint32_t y = T.brow * 8u + r;
+    if (y >= P.vis_h) { y = P.vis_h - 1u; }
+    const uint32_t x0 = (T.bx0 + j) * 8u;
+    const uint16_t* row = src + (size_t)y * P.stride;
+    if (x0 + 8u <= P.vis_w && ((((uintptr_t)(row + x0)) & 15u) == 0u)) {
+        const uint4 v = *(const uint4*)(row + x0);
+        const uint32_t w[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[2 * k] = (int32_t)(w[k] & 0xFFFFu) - mid;
+            x[2 * k + 1] = (int32_t)(w[k] >> 16) - mid;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t xx = x0 + (uint32_t)k;
+            if (xx >= P.vis_w) { xx = P.vis_w - 1u; }
+            x[k] = (int32_t)row[xx] - mid;
+        }
+    }
+}
+
 template <bool WIDE>
-__device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t fclamp,
-                                       uint32_t brow, uint32_t bx0, uint32_t nb,
+__device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int32_t fclamp, int32_t* x,
                                        int32_t* __restrict__ qbuf, int32_t* sA, const double* sRq,
-                                       const uint32_t* sHalf, const uint32_t* sDz)
+                                       const uint32_t* sHalf, const uint32_t* sDz, const uint8_t* sZz)
 {
     const uint32_t t = threadIdx.x;
-
+    const uint32_t nb = T.nb;
     // ---- pass 1: rows ----
     {
         const uint32_t r = t / FQ_BLOCKS, j = t % FQ_BLOCKS;
         if (j < nb) {
-            const uint16_t* src = (const uint16_t*)P.src;
-            uint32_t y = brow * 8u + r;
-            if (y >= P.vis_h) { y = P.vis_h - 1u; }
-            const uint32_t x0 = (bx0 + j) * 8u;
-            const uint16_t* row = src + (size_t)y * P.stride;
-            int32_t x[8];
-            if (x0 + 8u <= P.vis_w && ((((uintptr_t)(row + x0)) & 15u) == 0u)) {
-                const uint4 v = *(const uint4*)(row + x0);
-                const uint32_t w[4] = { v.x, v.y, v.z, v.w };
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    x[2 * k] = (int32_t)(w[k] & 0xFFFFu) - mid;
-                    x[2 * k + 1] = (int32_t)(w[k] >> 16) - mid;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    uint32_t xx = x0 + (uint32_t)k;
-                    if (xx >= P.vis_w) { xx = P.vis_w - 1u; }
-                    x[k] = (int32_t)row[xx] - mid;
-                }
-            }
             if (!WIDE) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) { x[k] = (int32_t)(int16_t)x[k]; }
@@ -106,16 +126,12 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t
     if (j < nb) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-#if FQ_ZZ_ARITH
-            sA[j * FQ_PITCH + tc_zigzag_pos((uint32_t)u * 8u + v)] = qv[u];
-#else
-            sA[j * FQ_PITCH + kZigzagInv[u * 8 + v]] = qv[u];
-#endif
+            sA[j * FQ_PITCH + sZz[u * 8 + v]] = qv[u];
         }
     }
     __syncthreads();
     // contiguous store of nb * 64 coefficients (scan order per block)
-    int32_t* dst = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0) * 64u;
+    int32_t* dst = qbuf + ((size_t)P.blk0 + (size_t)T.brow * P.bcols + T.bx0) * 64u;
     const uint32_t total = nb * 64u;
     for (uint32_t i = t * 4u; i < total; i += FQ_THREADS * 4u) {
         const int32_t* s4 = sA + (i >> 6) * FQ_PITCH + (i & 63u);
@@ -129,9 +145,11 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, int32_t mid, int32_t
     }
 }
 
-// grid: x = tile (32 blocks of a block row) within the plane, y = plane of the
-// batch; quantiser contexts are indexed by plane, so the context and the plane
-// descriptor load in parallel and no tile -> plane search is needed
+// grid: x = tile (FQ_BLOCKS blocks of a block row) within the plane, y = plane
+// of the batch; quantiser contexts are indexed by plane, so the context and
+// the plane descriptor load in parallel and no tile -> plane search is needed.
+// The zigzag table is staged in LDS with the quantiser context (a per-lane
+// LDS read is cheaper than a constant-memory gather).
 __global__ __launch_bounds__(FQ_THREADS) void k_fdct_quant(const tcg_qctx* __restrict__ qctxs,
                                                      const tcg_eplane* __restrict__ planes,
                                                      int32_t* __restrict__ qbuf)
@@ -139,27 +157,27 @@ __global__ __launch_bounds__(FQ_THREADS) void k_fdct_quant(const tcg_qctx* __res
     __shared__ int32_t sA[FQ_BLOCKS * FQ_PITCH];
     __shared__ double sRq[64];
     __shared__ uint32_t sHalf[64], sDz[64];
+    __shared__ uint8_t sZz[64];
     const uint32_t pi = blockIdx.y, t = threadIdx.x;
     const tcg_qctx* qc = &qctxs[pi];
     if (t < 64u) {
         sRq[t] = qc->rq[t];
         sHalf[t] = qc->half[t];
         sDz[t] = qc->dz[t];
+        sZz[t] = kZigzagInv[t];
     }
     const uint32_t bd = qc->bit_depth;
     const int32_t mid = qc->mid, fclamp = qc->fclamp;
     const tcg_eplane P = planes[pi];
     if (P.kind != 0u) { return; }
-    const uint32_t tiles_per_row = (P.bcols + FQ_BLOCKS - 1u) / FQ_BLOCKS;
-    const uint32_t local = blockIdx.x;
-    if (local >= P.brows * tiles_per_row) { return; }
-    const uint32_t brow = local / tiles_per_row;
-    const uint32_t bx0 = (local % tiles_per_row) * FQ_BLOCKS;
-    const uint32_t nb = min((uint32_t)FQ_BLOCKS, P.bcols - bx0);
+    const FqTile T = fq_tile(P, blockIdx.x);
+    if (!T.live) { return; }
+    int32_t x[8];
+    fq_load(P, T, mid, t, x);
     if (bd > 12u) {
-        fdct_quant_tile<true>(P, mid, fclamp, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
+        fdct_quant_tile<true>(P, T, fclamp, x, qbuf, sA, sRq, sHalf, sDz, sZz);
     } else {
-        fdct_quant_tile<false>(P, mid, fclamp, brow, bx0, nb, qbuf, sA, sRq, sHalf, sDz);
+        fdct_quant_tile<false>(P, T, fclamp, x, qbuf, sA, sRq, sHalf, sDz, sZz);
     }
 }
 
@@ -713,7 +731,8 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     c->kind = 1;
     tcg_mark(c, 0);
     if (max_plane_tiles > 0u) {
-        hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0, st, qctx_dev, planes_dev,
+        hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0,
+                           st, qctx_dev, planes_dev,
                            (int32_t*)qbuf);
     }
     tcg_mark(c, 1);
