@@ -755,6 +755,193 @@ static int rans2_payload(const toks* t, bytes_t* out)
     return em.oom || out->oom ? ORC_ENOMEM : ORC_OK;
 }
 
+/* ------------------------------------------ V1 Rice / V2 VLC colour slices */
+
+/* V2 frozen code lengths per book (codebook_version 1): DC_CAT 29 symbols,
+ * RUN 64, LEVEL_CAT 28 with category 0 outside the alphabet
+ * (native/src/entropy/vlc_tables.h:28-67). */
+static const uint8_t vlc_dc_len[4][29] = {
+    { 3, 13, 13, 13, 13, 13, 13, 12, 8, 6, 6, 5, 4, 3, 3, 3, 4, 4, 4, 4, 5, 5, 6, 6, 7, 7, 7, 9, 10 },
+    { 3, 10, 15, 15, 14, 14, 11, 7, 6, 6, 5, 4, 3, 3, 3, 3, 4, 4, 5, 5, 6, 7, 7, 7, 7, 8, 9, 12, 14 },
+    { 3, 10, 13, 11, 8, 7, 6, 5, 4, 3, 3, 3, 3, 4, 4, 5, 5, 6, 6, 7, 7, 7, 7, 9, 12, 15, 15, 15, 15 },
+    { 3, 6, 5, 4, 4, 3, 3, 3, 4, 4, 4, 5, 5, 5, 6, 7, 7, 7, 8, 9, 10, 11, 14, 14, 14, 14, 14, 14, 13 },
+};
+static const uint8_t vlc_run_len[4][64] = {
+    { 1, 2, 5, 6, 7, 6, 7, 7, 8, 7, 7, 8, 9, 7, 8, 13, 15, 10, 16, 20, 20, 12, 12, 14, 13, 20, 17, 20, 15, 20, 20, 20,
+      14, 20, 20, 20, 20, 20, 18, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20,
+      20, 20, 20, 3 },
+    { 1, 2, 5, 6, 7, 6, 7, 8, 7, 7, 7, 8, 9, 7, 8, 13, 15, 10, 16, 20, 20, 12, 12, 13, 13, 20, 17, 20, 16, 20, 20, 20,
+      16, 20, 20, 20, 20, 20, 18, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20,
+      20, 20, 20, 3 },
+    { 1, 2, 4, 5, 7, 7, 8, 9, 10, 9, 9, 10, 11, 9, 10, 15, 16, 12, 18, 19, 19, 14, 14, 16, 16, 20, 20, 20, 19, 20, 20,
+      20, 19, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20,
+      20, 20, 20, 20, 3 },
+    { 1, 2, 4, 5, 5, 6, 6, 7, 8, 8, 8, 9, 9, 9, 9, 11, 11, 10, 11, 12, 12, 12, 13, 13, 13, 13, 14, 15, 15, 16, 16, 15,
+      16, 17, 17, 18, 19, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20, 20,
+      20, 20, 20, 4 },
+};
+static const uint8_t vlc_lvl_len[4][28] = {
+    { 0, 10, 9, 8, 7, 5, 4, 4, 3, 3, 3, 3, 3, 4, 5, 5, 6, 6, 7, 8, 8, 9, 8, 7, 7, 7, 9, 10 },
+    { 0, 6, 5, 4, 4, 3, 3, 3, 3, 4, 4, 4, 5, 5, 6, 6, 7, 8, 8, 8, 7, 7, 7, 9, 10, 11, 12, 12 },
+    { 0, 4, 3, 3, 3, 3, 3, 4, 4, 5, 5, 5, 6, 6, 7, 8, 7, 7, 6, 7, 7, 9, 10, 13, 13, 12, 12, 12 },
+    { 0, 3, 2, 3, 3, 3, 4, 5, 5, 6, 6, 6, 6, 6, 6, 6, 7, 8, 9, 10, 13, 13, 13, 13, 13, 13, 13, 13 },
+};
+
+const uint8_t* orc_vlc_lengths(uint32_t family, uint32_t book)
+{
+    if (book > 3u) { return NULL; }
+    return family == 0u ? vlc_dc_len[book] : family == 1u ? vlc_run_len[book]
+         : family == 2u ? vlc_lvl_len[book] : NULL;
+}
+
+/* canonical code: lengths ascending, symbols ascending within one length,
+ * consecutive code values (native/src/entropy/vlc.c:37-53) */
+typedef struct {
+    uint32_t nsym, code[64], first[21], count[21], idx[21];
+    uint8_t len[64], order[64];
+} vbook;
+
+static void vbook_make(vbook* b, const uint8_t* len, uint32_t nsym)
+{
+    memset(b, 0, sizeof(*b));
+    b->nsym = nsym;
+    uint32_t code = 0, n = 0;
+    for (uint32_t l = 1; l <= 20u; ++l) {
+        b->first[l] = code;
+        b->idx[l] = n;
+        for (uint32_t s = 0; s < nsym; ++s) {
+            if (len[s] != l) { continue; }
+            b->len[s] = (uint8_t)l;
+            b->code[s] = code++;
+            b->order[n++] = (uint8_t)s;
+            b->count[l]++;
+        }
+        code <<= 1;
+    }
+}
+
+uint32_t orc_vlc_code(uint32_t family, uint32_t book, uint32_t sym)
+{
+    const uint8_t* len = orc_vlc_lengths(family, book);
+    vbook b;
+    vbook_make(&b, len, family == 0u ? 29u : family == 1u ? 64u : 28u);
+    return b.code[sym];
+}
+
+/* one symbol, bit by bit (native/src/entropy/vlc.c:108-126) */
+static int vbook_get(bitr* r, const vbook* b, uint32_t* sym)
+{
+    uint32_t code = 0, bit;
+    for (uint32_t l = 1; l <= 20u; ++l) {
+        if (br_bit(r, &bit) != 0) { return ORC_ETRUNC; }
+        code = (code << 1) | bit;
+        if (b->count[l] && code >= b->first[l] && code - b->first[l] < b->count[l]) {
+            *sym = b->order[b->idx[l] + code - b->first[l]];
+            return ORC_OK;
+        }
+    }
+    return ORC_EMALFORMED;
+}
+
+/* category symbol + (cat - 1) suffix bits, m = 1 << (cat-1) | suffix
+ * (native/src/entropy/vlc.c:130-151) */
+static int vbook_get_cat(bitr* r, const vbook* b, uint32_t mmax, uint32_t* m)
+{
+    uint32_t cat;
+    int rc = vbook_get(r, b, &cat);
+    if (rc) { return rc; }
+    uint32_t v = 0;
+    if (cat > 1u) {
+        uint32_t sfx;
+        if (br_bits(r, cat - 1u, &sfx) != 0) { return ORC_ETRUNC; }
+        v = (1u << (cat - 1u)) | sfx;
+    } else if (cat == 1u) {
+        v = 1u;
+    }
+    if (v > mmax) { return ORC_EMALFORMED; }
+    *m = v;
+    return ORC_OK;
+}
+
+static void vbook_put_cat(bitw* w, const vbook* b, uint32_t m)
+{
+    const uint32_t cat = bitlen(m);
+    bw_put(w, b->len[cat], b->code[cat]);
+    if (cat > 1u) { bw_put(w, cat - 1u, m & ((1u << (cat - 1u)) - 1u)); }
+}
+
+/* book with the fewest total bits (code length + category suffix), ties to
+ * the lowest id (native/src/codec/codec.c:1896-1922) */
+static uint32_t vlc_pick(const uint8_t* base, uint32_t stride, uint32_t nsym, const uint32_t* hist, int is_cat)
+{
+    uint64_t best = UINT64_MAX;
+    uint32_t id = 0;
+    for (uint32_t bk = 0; bk < 4u; ++bk) {
+        const uint8_t* len = base + (size_t)bk * stride;
+        uint64_t total = 0;
+        for (uint32_t s = 0; s < nsym; ++s) {
+            if (hist[s]) { total += (uint64_t)hist[s] * (len[s] + (is_cat && s > 1u ? s - 1u : 0u)); }
+        }
+        if (total < best) { best = total; id = bk; }
+    }
+    return id;
+}
+
+/* V1 colour payload: Rice k per family from the band means, then per block
+ * DC (k1), (run k3, level k2) pairs and the end-of-block run 63
+ * (native/src/codec/codec.c:1837-1839,1854-1881) */
+static int v1_payload(const toks* t, bytes_t* out, uint8_t* k1, uint8_t* k2, uint8_t* k3)
+{
+    uint64_t sdc = 0, slv = 0, srun = 0;
+    for (size_t b = 0; b < t->blocks; ++b) { sdc += t->dc_m[b]; }
+    for (size_t i = 0; i < t->pairs; ++i) { slv += t->lvl_m[i]; srun += t->run[i]; }
+    srun += 63u * (uint64_t)t->blocks;
+    const uint32_t a = k_estimate(sdc, t->blocks), l = k_estimate(slv, t->pairs);
+    const uint32_t r = k_estimate(srun, t->pairs + t->blocks);
+    bitw w = { { 0 }, 0, 0 };
+    size_t pi = 0;
+    for (size_t b = 0; b < t->blocks; ++b) {
+        rice_put(&w, a, t->dc_m[b]);
+        for (uint32_t k = 0; k < t->npair[b]; ++k, ++pi) {
+            rice_put(&w, r, t->run[pi]);
+            rice_put(&w, l, t->lvl_m[pi]);
+        }
+        rice_put(&w, r, 63u);
+    }
+    bw_flush(&w);
+    *out = w.b;
+    *k1 = (uint8_t)a; *k2 = (uint8_t)l; *k3 = (uint8_t)r;
+    return out->oom ? ORC_ENOMEM : ORC_OK;
+}
+
+/* V2 colour payload: same symbols as V1 with the frozen canonical books,
+ * one book per family chosen by exact bit count
+ * (native/src/codec/codec.c:1950-2015) */
+static int v2_payload(const toks* t, bytes_t* out, uint8_t* k1, uint8_t* k2, uint8_t* k3)
+{
+    const uint32_t dbk = vlc_pick(&vlc_dc_len[0][0], 29, 29, t->dc_hist, 1);
+    const uint32_t lbk = vlc_pick(&vlc_lvl_len[0][0], 28, 28, t->lvl_hist, 1);
+    const uint32_t rbk = vlc_pick(&vlc_run_len[0][0], 64, 64, t->run_hist, 0);
+    vbook db, lb, rb;
+    vbook_make(&db, vlc_dc_len[dbk], 29);
+    vbook_make(&lb, vlc_lvl_len[lbk], 28);
+    vbook_make(&rb, vlc_run_len[rbk], 64);
+    bitw w = { { 0 }, 0, 0 };
+    size_t pi = 0;
+    for (size_t b = 0; b < t->blocks; ++b) {
+        vbook_put_cat(&w, &db, t->dc_m[b]);
+        for (uint32_t k = 0; k < t->npair[b]; ++k, ++pi) {
+            bw_put(&w, rb.len[t->run[pi]], rb.code[t->run[pi]]);
+            vbook_put_cat(&w, &lb, t->lvl_m[pi]);
+        }
+        bw_put(&w, rb.len[63], rb.code[63]);
+    }
+    bw_flush(&w);
+    *out = w.b;
+    *k1 = (uint8_t)dbk; *k2 = (uint8_t)lbk; *k3 = (uint8_t)rbk;
+    return out->oom ? ORC_ENOMEM : ORC_OK;
+}
+
 /* -------------------------------------------------------- encode: alpha */
 
 static int32_t med(int32_t a, int32_t b, int32_t c)
@@ -819,7 +1006,7 @@ int32_t orc_frame_encode(const orc_cfg* cfg, const uint16_t* const planes[4],
                          const size_t strides[4], uint8_t* out, size_t cap, size_t* size_out)
 {
     const uint32_t em = cfg->reserved[0] & 0xFFu;
-    if (em != 8u) { return ORC_ENOTIMPL; }
+    if (em != 8u && em != 0u && em != 1u) { return ORC_ENOTIMPL; }
     orc_hdr h;
     cfg_to_hdr(cfg, &h);
     derive_geometry(&h);
@@ -864,7 +1051,11 @@ int32_t orc_frame_encode(const orc_cfg* cfg, const uint16_t* const planes[4],
                 toks t;
                 s.qpd = (uint8_t)(64 + qe - (int32_t)h.qp_base);
                 rc = band_tokens(&h, coded, p, (uint32_t)qe, y0, bh, &t);
-                if (rc == ORC_OK) { rc = rans2_payload(&t, &pay); }
+                if (rc == ORC_OK) {
+                    rc = em == 8u ? rans2_payload(&t, &pay)
+                       : em == 1u ? v2_payload(&t, &pay, &s.k1, &s.k2, &s.k3)
+                                  : v1_payload(&t, &pay, &s.k1, &s.k2, &s.k3);
+                }
                 toks_free(&t);
             }
             if (rc == ORC_OK) {
@@ -898,6 +1089,89 @@ typedef struct {
     uint16_t* dst; size_t stride; uint32_t vis_w, vis_h;
 } plane_out;
 
+/* dequantise + inverse transform one block and store its visible samples
+ * (native/src/codec/color_store.h:454-718) */
+static void store_block(const plane_out* po, const int32_t blk[64], const uint16_t* qm, uint32_t qp, uint32_t tbl,
+                        uint32_t bd, uint32_t bx, uint32_t brow)
+{
+    const int32_t mid = 1 << (bd - 1), maxv = (1 << bd) - 1;
+    int32_t xh[64];
+    orc_dequant_idct(blk, qm, qp, tbl, bd, xh);
+    const uint32_t px0 = bx * 8u, py0 = brow * 8u;
+    for (uint32_t yy = 0; yy < 8u && py0 + yy < po->vis_h; ++yy) {
+        for (uint32_t xx = 0; xx < 8u && px0 + xx < po->vis_w; ++xx) {
+            int32_t v = xh[yy * 8 + xx] + mid;
+            v = v < 0 ? 0 : v > maxv ? maxv : v;
+            po->dst[(size_t)(py0 + yy) * po->stride + px0 + xx] = (uint16_t)v;
+        }
+    }
+}
+
+/* V1 Rice (major 1) or V2 canonical VLC (major 2, entropy mode 1) colour
+ * slice: DC residual, (run, level) pairs up to the end-of-block run 63, the
+ * 12-bit symbol domains, then byte alignment inside the payload and exact
+ * consumption (native/src/bitstream/slice_codec.c:233-341,
+ * native/src/entropy/block_coding.c:78-111,146-184). */
+static int decode_colour_bits(const orc_hdr* h, const orc_slice* s, const uint8_t* pl, const plane_out* po)
+{
+    const int vlc = h->major == 2u;
+    vbook db, lb, rb;
+    if (vlc) {
+        if (s->k1 > 3u || s->k2 > 3u || s->k3 > 3u) { return ORC_EMALFORMED; }
+        vbook_make(&db, vlc_dc_len[s->k1], 29);
+        vbook_make(&lb, vlc_lvl_len[s->k2], 28);
+        vbook_make(&rb, vlc_run_len[s->k3], 64);
+    } else if (s->k1 > 14u || s->k2 > 14u || s->k3 > 14u) {
+        return ORC_EINVAL;
+    }
+    bitr br = { pl, s->size, 0 };
+    const uint32_t p = s->plane, cols = h->bcols[p];
+    const uint16_t* qm = orc_qm[h->qmatrix_id][is_chroma(h, p) ? 1 : 0];
+    const uint32_t tbl = (h->flags & 8u) ? 1u : 0u;
+    const uint32_t qp = (uint32_t)((int32_t)h->qp_base + (int32_t)s->qpd - 64);
+    const int64_t dlim = (int64_t)1 << 25;
+    int32_t* prow = (int32_t*)calloc(cols, 4);
+    int32_t* crow = (int32_t*)calloc(cols, 4);
+    int rc = 0;
+    for (uint32_t by = 0; by < s->h && rc == 0; ++by) {
+        for (uint32_t bx = 0; bx < cols && rc == 0; ++bx) {
+            int32_t blk[64];
+            memset(blk, 0, sizeof(blk));
+            uint32_t m;
+            rc = vlc ? vbook_get_cat(&br, &db, 0x08000000u, &m) : rice_get(&br, s->k1, 0x08000000u, &m);
+            if (rc) { break; }
+            const int32_t pred = dc_predict(bx != 0, bx ? crow[bx - 1] : 0, by != 0, prow[bx]);
+            const int64_t dv = (int64_t)pred + unmap_signed(m);
+            if (dv > dlim || dv < -dlim) { rc = ORC_EMALFORMED; break; }
+            blk[0] = (int32_t)dv;
+            uint32_t pos = 1;
+            for (;;) {
+                uint32_t run;
+                rc = vlc ? vbook_get(&br, &rb, &run) : rice_get(&br, s->k3, 63u, &run);
+                if (rc || run == 63u) { break; }
+                const uint32_t idx = pos + run;
+                if (idx > 63u) { rc = ORC_EMALFORMED; break; }
+                rc = vlc ? vbook_get_cat(&br, &lb, 0x04000000u, &m) : rice_get(&br, s->k2, 0x04000000u, &m);
+                if (rc) { break; }
+                blk[orc_zigzag[idx]] = unmap_signed(m);
+                pos = idx + 1u;
+            }
+            if (rc) { break; }
+            crow[bx] = blk[0];
+            store_block(po, blk, qm, qp, tbl, h->bit_depth, bx, s->y0 + by);
+        }
+        int32_t* tmp = prow; prow = crow; crow = tmp;
+    }
+    free(prow);
+    free(crow);
+    if (rc == 0) {
+        const uint64_t aligned = (br.pos + 7u) & ~(uint64_t)7u;
+        if (aligned > (uint64_t)s->size * 8u) { rc = ORC_ETRUNC; }
+        else if (aligned != (uint64_t)s->size * 8u) { rc = ORC_EMALFORMED; }
+    }
+    return rc;
+}
+
 /* rANS2 colour slice decode + reconstruction into the caller plane
  * (native/src/bitstream/slice_codec.c:554-736; codec/color_store.h:454-718). */
 static int decode_colour(const orc_hdr* h, const orc_slice* s, const uint8_t* pl, const plane_out* po)
@@ -925,7 +1199,6 @@ static int decode_colour(const orc_hdr* h, const orc_slice* s, const uint8_t* pl
     const uint32_t tbl = (h->flags & 8u) ? 1u : 0u;
     const uint32_t qp = (uint32_t)((int32_t)h->qp_base + (int32_t)s->qpd - 64);
     const uint32_t bd = h->bit_depth;
-    const int32_t mid = 1 << (bd - 1), maxv = (1 << bd) - 1;
     const uint32_t dc_max = bd > 12u ? 0x0FFFFFFFu : 0x08000000u;
     const uint32_t lv_max = bd > 12u ? 0x07FFFFFFu : 0x04000000u;
     const int64_t dlim = bd > 12u ? ((int64_t)1 << 27) : ((int64_t)1 << 25);
@@ -963,16 +1236,7 @@ static int decode_colour(const orc_hdr* h, const orc_slice* s, const uint8_t* pl
             }
             if (rc) { break; }
             crow[bx] = blk[0];
-            int32_t xh[64];
-            orc_dequant_idct(blk, qm, qp, tbl, bd, xh);
-            const uint32_t px0 = bx * 8u, py0 = (s->y0 + by) * 8u;
-            for (uint32_t yy = 0; yy < 8u && py0 + yy < po->vis_h; ++yy) {
-                for (uint32_t xx = 0; xx < 8u && px0 + xx < po->vis_w; ++xx) {
-                    int32_t v = xh[yy * 8 + xx] + mid;
-                    v = v < 0 ? 0 : v > maxv ? maxv : v;
-                    po->dst[(size_t)(py0 + yy) * po->stride + px0 + xx] = (uint16_t)v;
-                }
-            }
+            store_block(po, blk, qm, qp, tbl, bd, bx, s->y0 + by);
         }
         int32_t* tmp = prow; prow = crow; crow = tmp;
     }
@@ -1047,7 +1311,8 @@ int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes
     orc_hdr h;
     int rc = hdr_read(pkt, size, &h);
     if (rc) { return rc; }
-    if (!(h.major == 7u && h.entropy_mode == 7u)) { return ORC_ENOTIMPL; }
+    const int bits = h.major == 1u || (h.major == 2u && h.entropy_mode == 1u);
+    if (!(h.major == 7u && h.entropy_mode == 7u) && !bits) { return ORC_ENOTIMPL; }
     if (h.packet_size != size) { return ORC_EMALFORMED; }
     uint32_t nconc = 0;
     size_t off = 53;
@@ -1064,7 +1329,10 @@ int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes
         const uint32_t p = s.plane;
         plane_out po = { planes[p], strides && strides[p] ? strides[p] : h.pvw[p], h.pvw[p], h.pvh[p] };
         int src = orc_crc32(pl, s.size) == s.crc ? 0 : ORC_ECRC;
-        if (src == 0) { src = is_alpha(&h, p) ? decode_alpha(&h, &s, pl, &po) : decode_colour(&h, &s, pl, &po); }
+        if (src == 0) {
+            src = is_alpha(&h, p) ? decode_alpha(&h, &s, pl, &po)
+                : bits ? decode_colour_bits(&h, &s, pl, &po) : decode_colour(&h, &s, pl, &po);
+        }
         if (slice_status) { slice_status[i] = src ? 1u : 0u; }
         if (src) {
             nconc++;

@@ -69,8 +69,9 @@ static int32_t enc_check(const topos_frame_config* cfg, const topos_frame_input*
     if (cfg->reserved[1] == 1u || cfg->reserved[2] == 1u) {
         return tc_fail(TC_ERR_NOT_IMPLEMENTED, "adaptive quantisation / RDO encode is not provided by the GPU backend");
     }
-    if (em != 8u) {
-        return tc_fail(TC_ERR_NOT_IMPLEMENTED, "entropy mode %u is not provided by the GPU backend (rANS2 = 8)",
+    if (em != 8u && em != 0u && em != 1u) {
+        return tc_fail(TC_ERR_NOT_IMPLEMENTED,
+                       "entropy mode %u is not provided by the GPU backend (V1 Rice = 0, V2 VLC = 1, rANS2 = 8)",
                        (unsigned)em);
     }
     if (cfg->alpha_mode != 0u && !tcx_alpha_supported()) {
@@ -130,7 +131,8 @@ static int32_t dec_supported(const tc_view* v)
     if (fh->major == 7u && fh->entropy_mode == 8u && fh->frame_type != 0u) {
         return tc_fail(TC_ERR_STATE, "inter frame decode requires a GOP context");
     }
-    if (!(fh->major == 7u && fh->entropy_mode == 7u)) {
+    if (!(fh->major == 7u && fh->entropy_mode == 7u) && fh->major != 1u &&
+        !(fh->major == 2u && fh->entropy_mode == 1u)) {
         return tc_fail(TC_ERR_NOT_IMPLEMENTED,
                        "bitstream generation (%u, entropy %u) is not decoded by the GPU backend",
                        (unsigned)fh->major, (unsigned)fh->entropy_mode);

@@ -79,6 +79,21 @@ static int32_t host_need(tc_ectx* c, uint8_t** p, size_t* cap, size_t bytes)
     return TC_OK;
 }
 
+/* colour slice entropy coding of a frame generation */
+static uint32_t slice_mode(const tc_fhdr* fh)
+{
+    if (fh->major == 1u) { return TCG_MODE_RICE; }
+    if (fh->major == 2u) { return TCG_MODE_VLC; }
+    return TCG_MODE_RANS2;
+}
+
+/* encode scratch bound per 8x8 block of a slice */
+static size_t slice_bytes_per_block(const tc_fhdr* fh, uint32_t p)
+{
+    if (tc_is_alpha(fh, p)) { return 2u * TCG_SLICE_BYTES_PER_BLOCK; }
+    return slice_mode(fh) == TCG_MODE_RANS2 ? TCG_SLICE_BYTES_PER_BLOCK : TCG_BITS_BYTES_PER_BLOCK;
+}
+
 static int32_t engine_init_locked(void)
 {
     if (g_state == 0) {
@@ -90,6 +105,9 @@ static int32_t engine_init_locked(void)
             for (uint32_t f = 1; f <= 4096u; ++f) { cost[f] = 12.0 - tc_ctx_log2((double)f); }
             tc_crc_tables(crc, x2n);
             rc = tcg_set_tables(cost, crc, x2n);
+            static tcg_vlc_books books;
+            if (rc == 0 && tc_vlc_books_build(&books) != TC_OK) { rc = -2; }
+            if (rc == 0) { rc = tcg_set_vlc(&books); }
         }
         if (rc != 0) {
             g_state = -1;
@@ -193,8 +211,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
             }
             for (uint32_t y0 = 0; y0 < fh->brows[p]; y0 += P->slice_rows) {
                 const uint32_t h = fh->brows[p] - y0 < P->slice_rows ? fh->brows[p] - y0 : P->slice_rows;
-                const size_t per_blk = tc_is_alpha(fh, p) ? 2u * TCG_SLICE_BYTES_PER_BLOCK : TCG_SLICE_BYTES_PER_BLOCK;
-                const size_t cap = (size_t)h * fh->bcols[p] * per_blk + TCG_SLICE_BYTES_EXTRA;
+                const size_t cap = (size_t)h * fh->bcols[p] * slice_bytes_per_block(fh, p) + TCG_SLICE_BYTES_EXTRA;
                 scratch_bytes += al256(cap);
                 pkt_bytes += TC_SLICE_HDR_SIZE + cap;
             }
@@ -282,9 +299,8 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
                 S->blk0 = blk + y0 * fh->bcols[p];
                 S->nblk = h * fh->bcols[p];
                 S->scratch = scr_off;
-                S->scratch_cap = (uint32_t)(S->nblk * (tc_is_alpha(fh, p) ? 2u * TCG_SLICE_BYTES_PER_BLOCK
-                                                                        : TCG_SLICE_BYTES_PER_BLOCK)
-                                            + TCG_SLICE_BYTES_EXTRA);
+                S->scratch_cap = (uint32_t)(S->nblk * slice_bytes_per_block(fh, p) + TCG_SLICE_BYTES_EXTRA);
+                S->mode = slice_mode(fh);
                 S->qpd = tc_is_alpha(fh, p) ? 64u : (uint32_t)(64 + P->qp_eff[p] - (int32_t)fh->qp_base);
                 S->hdr_plane = p;
                 scr_off += al256(S->scratch_cap);
@@ -519,7 +535,8 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             S->plane = ip0 + p;
             S->y0 = h->y0; S->h = h->h;
             S->blk0 = D->blk0 + (uint32_t)h->y0 * fh->bcols[p];
-            S->k1 = h->k1; S->k2 = h->k2;
+            S->k1 = h->k1; S->k2 = h->k2; S->k3 = h->k3;
+            S->mode = slice_mode(fh);
             S->dc_max = fh->bit_depth > 12u ? 0x0FFFFFFFu : 0x08000000u;
             S->lv_max = fh->bit_depth > 12u ? 0x07FFFFFFu : 0x04000000u;
             if (tc_is_alpha(fh, p)) {

@@ -62,6 +62,9 @@ extern const uint8_t tc_zigzag_inv[64];
 
 uint32_t tc_crc32(const uint8_t* p, size_t n);
 void tc_crc_tables(uint32_t crc256[256], uint32_t x2n32[32]);
+/* V2 VLC books (tc_vlc.c) */
+const uint8_t* tc_vlc_lengths(uint32_t family, uint32_t book, uint32_t* nsym);
+int32_t tc_vlc_books_build(tcg_vlc_books* b);
 double tc_ctx_log2(double x);
 
 int32_t tc_cfg_validate(const topos_frame_config* cfg);

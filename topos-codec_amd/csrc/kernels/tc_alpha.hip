@@ -47,11 +47,6 @@ __device__ static inline uint32_t rice_len(uint32_t k, uint32_t m)
     return q <= 30u ? q + 1u + k : 63u;
 }
 
-__device__ static inline uint32_t bswap32(uint32_t v)
-{
-    return (v >> 24) | ((v >> 8) & 0xFF00u) | ((v << 8) & 0xFF0000u) | (v << 24);
-}
-
 // OR an L-bit code (value in the low L bits of v) into a big-endian bit
 // stream held in little-endian 32-bit words, starting at bit position pos.
 __device__ static inline void put_code(uint32_t* words, uint64_t pos, uint32_t L, uint64_t v)

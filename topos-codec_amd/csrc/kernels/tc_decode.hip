@@ -15,25 +15,6 @@
 #include "tc_dev.h"
 
 // ---------------------------------------------------------------------------
-// wave-wide CRC32 of a byte range: each lane folds a contiguous chunk with the
-// bytewise table, the partial registers are shifted over the bytes that follow
-// them (multiplication by x^(8k) mod P) and XOR-reduced across the wave.
-// ---------------------------------------------------------------------------
-__device__ static uint32_t wave_crc32(const uint8_t* p, uint32_t n, const uint32_t* tab,
-                                      const uint32_t* x2n)
-{
-    const uint32_t lane = __lane_id();
-    const uint32_t per = (n + 63u) >> 6;
-    const uint32_t a = min(n, lane * per), b = min(n, a + per);
-    uint32_t c = 0u;
-    for (uint32_t i = a; i < b; ++i) { c = tab[(c ^ p[i]) & 0xFFu] ^ (c >> 8); }
-    uint32_t part = a < b ? tc_crc_shift(x2n, c, (uint64_t)(n - b)) : 0u;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) { part ^= (uint32_t)__shfl_xor((int)part, off); }
-    return part ^ tc_crc_shift(x2n, 0xFFFFFFFFu, n) ^ 0xFFFFFFFFu;
-}
-
-// ---------------------------------------------------------------------------
 // K5: rANS2 colour slice decoder, one wavefront per slice.
 // The decode chain is serial (one 32-bit state per slice) and a wavefront
 // issues at most one instruction per four cycles, so the loop is written as a
@@ -52,9 +33,6 @@ __device__ static uint32_t wave_crc32(const uint8_t* p, uint32_t n, const uint32
 //    loop through one compare; range errors are collected per block.
 // The payload must be readable 768 bytes past its end (window prefetch).
 // ---------------------------------------------------------------------------
-typedef __attribute__((address_space(1))) const uint32_t gcu32;
-
-__device__ static inline uint32_t wuni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 struct RansRd {
     uint32_t x;       // rANS state
@@ -68,11 +46,6 @@ struct RansRd {
     uint32_t nxt;     // lane i: dword i of the next window, as stored
     gcu32* wp;        // address of the window after `nxt`
 };
-
-__device__ static inline uint32_t bswap32(uint32_t v)
-{
-    return (v >> 24) | ((v >> 8) & 0xFF00u) | ((v << 8) & 0xFF0000u) | (v << 24);
-}
 
 __device__ static inline void rr_refill(RansRd& D, uint32_t lane)
 {
@@ -254,7 +227,7 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
 
     const uint32_t lane = threadIdx.x;
     const tcg_dslice S = slices[blockIdx.x];
-    if (S.kind != 0u) { return; }
+    if (S.kind != 0u || S.mode != TCG_MODE_RANS2) { return; }
     const tcg_dplane P = planes[S.plane];
     for (uint32_t i = lane; i < 256u; i += 64u) { sTab[i] = K->crc[i]; }
     if (lane < 32u) { sX2n[lane] = K->x2n[lane]; }
@@ -557,6 +530,7 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     tcg_mark(c, 0);
     hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
                        (int32_t*)qbuf, slice_status_dev);
+    tcg_bits_decode_launch(st, K, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
     tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
     tcg_mark(c, 1);
     if (max_plane_tiles > 0u) {

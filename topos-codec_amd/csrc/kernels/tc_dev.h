@@ -37,12 +37,41 @@ static __device__ const uint8_t kZigzagInv[64] = {
     21, 34, 37, 47, 50, 56, 59, 61, 35, 36, 48, 49, 57, 58, 62, 63,
 };
 
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+
+__device__ static inline uint32_t wuni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ static inline uint32_t bswap32(uint32_t v)
+{
+    return (v >> 24) | ((v >> 8) & 0xFF00u) | ((v << 8) & 0xFF0000u) | (v << 24);
+}
+
+// ---------------------------------------------------------------------------
+// wave-wide CRC32 of a byte range: each lane folds a contiguous chunk with the
+// bytewise table, the partial registers are shifted over the bytes that follow
+// them (multiplication by x^(8k) mod P) and XOR-reduced across the wave.
+// ---------------------------------------------------------------------------
+__device__ static inline uint32_t wave_crc32(const uint8_t* p, uint32_t n, const uint32_t* tab,
+                                      const uint32_t* x2n)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t per = (n + 63u) >> 6;
+    const uint32_t a = min(n, lane * per), b = min(n, a + per);
+    uint32_t c = 0u;
+    for (uint32_t i = a; i < b; ++i) { c = tab[(c ^ p[i]) & 0xFFu] ^ (c >> 8); }
+    uint32_t part = a < b ? tc_crc_shift(x2n, c, (uint64_t)(n - b)) : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) { part ^= (uint32_t)__shfl_xor((int)part, off); }
+    return part ^ tc_crc_shift(x2n, 0xFFFFFFFFu, n) ^ 0xFFFFFFFFu;
+}
+
 // Run-time tables uploaded once by tcg_set_tables.
 struct TcgConsts {
     double cost[4097];      // 12 - log2(f), the model-choice cost per symbol
     uint32_t crc[256];      // bytewise CRC32 table (reflected 0xEDB88320)
     uint32_t x2n[32];       // x^(2^k) mod P
     uint32_t x16[256];      // x^(128 j) mod P: shift of a raw CRC over 16 j zero bytes
+    tcg_vlc_books vlc;      // V2 canonical VLC books (tcg_set_vlc)
 };
 
 // host-side runtime services (tc_runtime.hip)
@@ -95,5 +124,12 @@ void tcg_alpha_encode_launch(hipStream_t st, const tcg_eplane* planes, const tcg
                              tcg_eresult* results);
 void tcg_alpha_decode_launch(hipStream_t st, const tcg_dplane* planes, const tcg_dslice* slices,
                              uint32_t nslices, int32_t* qbuf, int32_t* status);
+
+// V1 Rice / V2 VLC colour slice kernels (tc_bits.hip)
+void tcg_bits_encode_launch(hipStream_t st, const TcgConsts* K, const tcg_eplane* planes,
+                            const tcg_eslice* slices, uint32_t nslices, const int32_t* qbuf, uint32_t* dcmbuf,
+                            uint8_t* scratch, tcg_eresult* results);
+void tcg_bits_decode_launch(hipStream_t st, const TcgConsts* K, const tcg_dplane* planes,
+                            const tcg_dslice* slices, uint32_t nslices, int32_t* qbuf, int32_t* status);
 
 #endif

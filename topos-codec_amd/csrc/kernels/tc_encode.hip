@@ -287,6 +287,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
 
     const uint32_t lane = threadIdx.x;
     const tcg_eslice S = slices[blockIdx.x];
+    if (S.mode != TCG_MODE_RANS2) { return; }
     const tcg_eplane P = planes[S.plane];
     if (P.kind != 0u) { return; }
     const uint32_t cols = P.bcols;
@@ -738,6 +739,8 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     tcg_mark(c, 1);
     hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
                        (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
+    tcg_bits_encode_launch(st, K, planes_dev, slices_dev, nslices, (const int32_t*)qbuf, (uint32_t*)dcmbuf,
+                           (uint8_t*)scratch, results_dev);
     tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch,
                             results_dev);
     tcg_mark(c, 2);

@@ -4,6 +4,7 @@
 // timing with HIP events on the launch stream.
 #include <hip/hip_runtime.h>
 
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -197,6 +198,13 @@ int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uin
     for (uint32_t j = 0; j < 256u; ++j) { h.x16[j] = tc_gf2_xpow8n(x2n32, 16u * (uint64_t)j); }
     const hipError_t e = hipMemcpy(g_consts, &h, sizeof(h), hipMemcpyHostToDevice);
     return e == hipSuccess ? 0 : fail("upload tables", e);
+}
+
+int32_t tcg_set_vlc(const tcg_vlc_books* books)
+{
+    const hipError_t e = hipMemcpy((char*)g_consts + offsetof(TcgConsts, vlc), books, sizeof(*books),
+                                   hipMemcpyHostToDevice);
+    return e == hipSuccess ? 0 : fail("upload VLC books", e);
 }
 
 void tcg_set_timing(int on) { g_timing = on; }

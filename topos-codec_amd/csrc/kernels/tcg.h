@@ -24,6 +24,25 @@ extern "C" {
 #define TCG_SLICE_BYTES_PER_BLOCK 512u /* entropy scratch bound per 8x8 block */
 #define TCG_SLICE_BYTES_EXTRA 1024u    /* prefix tables + state + slack */
 #define TCG_PREFIX_MAX 352u            /* flags + 5x29 + 64 + 5x28 = 350 */
+/* V1 Rice / V2 VLC slices: worst case per block is 1008 B (Rice: a 63-bit
+ * escape for the DC, each of 63 runs and levels, and the end-of-block run)
+ * and 529 B (VLC) */
+#define TCG_BITS_BYTES_PER_BLOCK 1024u
+
+/* entropy coding of a colour slice */
+#define TCG_MODE_RANS2 0u /* rANS2 with context models (major 7, entropy mode 7) */
+#define TCG_MODE_RICE 1u  /* V1 bounded Rice (major 1) */
+#define TCG_MODE_VLC 2u   /* V2 canonical VLC, codebook version 1 (major 2, entropy mode 1) */
+
+/* the twelve frozen V2 books, index family * 4 + book (family 0 DC
+ * category, 1 run, 2 level category), in the lane layouts the kernels use */
+typedef struct tcg_vlc_books {
+    uint32_t lim[12][64];  /* lane l = 1..20: end of the length-l codes, left-aligned to 20 bits; else 0 */
+    uint32_t info[12][64]; /* lane l: first code of length l << 8 | its index in (length, symbol) order */
+    uint32_t sym[12][64];  /* (length, symbol) order position -> symbol */
+    uint32_t code[12][64]; /* symbol -> code */
+    uint32_t len[12][64];  /* symbol -> code length (0: not in the alphabet) */
+} tcg_vlc_books;
 
 /* quantiser context of one (plane, qp): natural-order step, half step, dead
  * zone and the 51-bit exact reciprocal m = floor(2^51/Q)+1 split in two words
@@ -67,7 +86,7 @@ typedef struct tcg_eslice {
     uint32_t scratch_cap;
     uint32_t qpd;        /* qp_delta_biased written to the slice header */
     uint32_t hdr_plane;  /* plane number written to the slice header */
-    uint32_t pad_;
+    uint32_t mode;       /* TCG_MODE_* of a colour slice */
 } tcg_eslice;
 
 /* per-slice encode result */
@@ -105,10 +124,12 @@ typedef struct tcg_dslice {
     uint32_t y0, h;      /* band in block rows */
     uint32_t blk0;       /* first block (flat) of the band */
     uint32_t qctx;       /* index into the tcg_qctx table */
-    uint32_t kind;       /* 0 colour (rANS2), 1 alpha (Rice/MED) */
-    uint32_t k1, k2;     /* alpha Rice parameters */
+    uint32_t kind;       /* 0 colour, 1 alpha (Rice/MED) */
+    uint32_t k1, k2;     /* slice header k fields: alpha / V1 Rice parameters, V2 books */
     uint32_t dc_max;     /* colour: largest legal mapped DC residual */
     uint32_t lv_max;     /* colour: largest legal mapped AC level */
+    uint32_t k3;
+    uint32_t mode;       /* TCG_MODE_* of a colour slice */
     uint32_t pad_;
 } tcg_dslice;
 
@@ -164,6 +185,8 @@ int32_t tcg_copy(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes);
 /* upload the (12 - log2 f) cost table, f = 0..4096 (entry 0 unused), and
  * the CRC32 tables (bytewise table, x^(2^k) mod P) */
 int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uint32_t* x2n32);
+/* upload the V2 VLC books */
+int32_t tcg_set_vlc(const tcg_vlc_books* books);
 
 /* ---- encode ---- */
 int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
