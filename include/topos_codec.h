@@ -148,6 +148,24 @@ typedef struct topos_plane_view {
     size_t stride;
 } topos_plane_view;
 
+/* caller byte source: read exactly len bytes at an absolute offset, TC_OK or
+ * TC_ERR_IO (reference topos_codec.h:646-669; the write callbacks are for the
+ * reference's container writers and are not called here) */
+typedef int32_t (*tc_io_read_fn)(void* ctx, uint64_t offset, void* buf, size_t len);
+typedef int32_t (*tc_io_write_fn)(void* ctx, const void* data, size_t len);
+typedef int32_t (*tc_io_seek_write_fn)(void* ctx, uint64_t offset, const void* data, size_t len);
+
+typedef struct topos_io {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    void* ctx;
+    tc_io_read_fn read;
+    tc_io_write_fn write;
+    tc_io_seek_write_fn seek_write;
+    uint64_t length;
+    uint32_t reserved[4];
+} topos_io;
+
 /* decoder context config (reference topos_codec.h:502-510) */
 typedef struct topos_decoder_config {
     uint32_t struct_size;

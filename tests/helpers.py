@@ -55,3 +55,90 @@ def frame_for(pkg, syn, kind, w, h, seed):
     if kind == "raw":
         return pkg.synth_raw12(syn, w, h, seed)
     raise ValueError(kind)
+
+
+# ---------------------------------------------------------------- .toos files
+# A minimal TPIM envelope writer for tests: preamble, two-entry directory,
+# IDSC and PIXL, following docs/image/topos_image_file_spec_v0.md sections
+# 3, 4 and 6 (the reference writer is native/src/image/image_writer.c).
+import struct as _struct
+import zlib as _zlib
+
+
+def idsc_for_packet(packet, image_profile):
+    """IDSC bytes derived from a packet header as the reference derives them
+    (native/src/image/image_reader.c:736-815)."""
+    h = packet
+    pf, bd, profile = h[11], h[12], h[10]
+    am, abd, flags = h[13], h[14], int.from_bytes(h[8:10], "big")
+    vw, vh = int.from_bytes(h[23:25], "big"), int.from_bytes(h[25:27], "big")
+    sar_num, sar_den = int.from_bytes(h[37:39], "big"), int.from_bytes(h[39:41], "big")
+    sar_num, sar_den = sar_num or 1, sar_den or 1
+    model = 0 if pf == 2 else (3 if pf == 3 else 1)
+    container = bd if bd >= 12 else 10
+    presence = 0 if am == 0 else (2 if flags & 1 else 1)
+    sub = 1 if pf == 0 else (2 if pf == 3 else 0)
+    b = bytearray(128)
+    b[0:4] = b"IDSC"
+    _struct.pack_into(">HHII", b, 4, 1, 0, 128, 1)  # version 1.0, size, orientation normalised
+    _struct.pack_into(">iiiiiiii", b, 16, 0, 0, vw - 1, vh - 1, 0, 0, vw - 1, vh - 1)
+    _struct.pack_into(">III", b, 48, 1, sar_num, sar_den)
+    fields = [model, h[27], 0, bd, container, 1, sub, h[36], presence, am, abd, 0, 0,
+              h[6], h[7], image_profile, profile, pf, h[33], h[34], h[35], h[32], 0, 0]
+    b[60:84] = bytes(fields)
+    b[84:88] = h[49:53]
+    return bytes(b)
+
+
+def toos_wrap(packet, image_profile, idsc=None):
+    """.toos bytes: preamble (64) | directory (2 x 32) | IDSC (128) | PIXL."""
+    idsc = idsc if idsc is not None else idsc_for_packet(packet, image_profile)
+    dir_off = 64
+    idsc_off = dir_off + 2 * 32
+    pixl_off = idsc_off + len(idsc)
+    total = pixl_off + len(packet)
+    d = bytearray()
+    for typ, off, data in ((b"IDSC", idsc_off, idsc), (b"PIXL", pixl_off, packet)):
+        d += typ + _struct.pack(">IQQII", 1, off, len(data), _zlib.crc32(data), 0)
+    p = bytearray(64)
+    p[0:4] = b"TPIM"
+    _struct.pack_into(">HHIIQQIIIIII", p, 4, 1, 0, 64, 0, total, dir_off, 32, 2, 0, 0, 0, _zlib.crc32(bytes(d)))
+    _struct.pack_into(">I", p, 48, _zlib.crc32(bytes(p)))
+    return bytes(p) + bytes(d) + idsc + bytes(packet)
+
+
+def toos_layout(data):
+    """(directory offset, count, [(type, flags, offset, size)])"""
+    dir_off, = _struct.unpack(">Q", data[24:32])
+    count, = _struct.unpack(">I", data[36:40])
+    if count > 64 or dir_off + 32 * count > len(data):
+        return dir_off, 0, []
+    ents = []
+    for i in range(count):
+        e = data[dir_off + 32 * i: dir_off + 32 * (i + 1)]
+        typ, fl, off, size, _crc, _r = _struct.unpack(">IIQQII", e)
+        ents.append((typ, fl, off, size))
+    return dir_off, count, ents
+
+
+def toos_repair(data, fix_packet=True):
+    """Recompute, in dependency order, the packet header CRC and the IDSC
+    binding digest, every chunk CRC, the directory CRC and the preamble CRC."""
+    b = bytearray(data)
+    dir_off, count, ents = toos_layout(bytes(b))
+    pix = [e for e in ents if e[0] == 0x5049584C]
+    ids = [e for e in ents if e[0] == 0x49445343]
+    if fix_packet and pix and ids and pix[0][3] >= 53 and ids[0][3] == 128:
+        po, io_ = pix[0][2], ids[0][2]
+        if po + 53 <= len(b) and io_ + 88 <= len(b):
+            b[po + 49:po + 53] = _zlib.crc32(bytes(b[po:po + 49])).to_bytes(4, "big")
+            b[io_ + 84:io_ + 88] = b[po + 49:po + 53]
+    for i, (typ, fl, off, size) in enumerate(ents):
+        if off + size <= len(b):
+            e = dir_off + 32 * i
+            b[e + 24:e + 28] = _zlib.crc32(bytes(b[off:off + size])).to_bytes(4, "big")
+    if dir_off + 32 * count <= len(b):
+        b[52:56] = _zlib.crc32(bytes(b[dir_off:dir_off + 32 * count])).to_bytes(4, "big")
+    b[48:52] = b"\0\0\0\0"
+    b[48:52] = _zlib.crc32(bytes(b[:64])).to_bytes(4, "big")
+    return bytes(b)

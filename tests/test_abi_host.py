@@ -22,7 +22,7 @@ def declared_functions():
             continue
         text = open(path).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**([a-z_][a-z0-9_]*)\s*\(",
+        for m in re.finditer(r"^\s*(?!typedef\b)(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**([a-z_][a-z0-9_]*)\s*\(",
                              text, flags=re.M):
             names.add(m.group(1))
     return sorted(names)

@@ -58,6 +58,13 @@ def load_library(path=LIB_PATH):
     lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
+    lib.tc_image_status_message.restype = ctypes.c_char_p
+    lib.tc_image_status_message.argtypes = [ctypes.c_int32]
+    lib.tc_image_probe.argtypes = [vp, vp]
+    lib.tc_image_validate.argtypes = [vp, ctypes.c_uint32, vp]
+    lib.tc_image_decode.argtypes = [vp, vp, vp]
+    lib.tc_image_decode_preview.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp]
+    lib.tc_image_query_decode_buffer.argtypes = [vp, ctypes.c_uint32, vp, vp]
     del u8p
     return lib
 
@@ -162,6 +169,114 @@ class ToposCodec:
                 ptrs[4 * i + p] = a.ctypes.data
         status = int(self._lib.tc_frame_decode_batch(pk, n, ptrs, None, infos))
         return status, outs, infos
+
+
+class ToposImageError(RuntimeError):
+    def __init__(self, status, detail=""):
+        super().__init__(f"[{status}] {detail}")
+        self.status = status
+
+
+class MemoryIo:
+    """topos_io over bytes held in memory (read callback only)."""
+
+    def __init__(self, data):
+        self.data = bytes(data)
+        self.last_error = ""
+
+        def _read(_ctx, offset, buf, length):
+            if offset + length > len(self.data):
+                self.last_error = f"short read at {offset}+{length}"
+                return abi.TC_ERR_IO
+            ctypes.memmove(buf, self.data[offset:offset + length], length)
+            return abi.TC_OK
+
+        self._cb = abi.IO_READ_FN(_read)
+        self.io = abi.TposIo()
+        self.io.struct_size = ctypes.sizeof(abi.TposIo)
+        self.io.abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        self.io.read = self._cb
+        self.io.length = len(self.data)
+
+    def ref(self):
+        return ctypes.cast(ctypes.byref(self.io), ctypes.c_void_p)
+
+
+def _image_source(src):
+    if isinstance(src, (bytes, bytearray, memoryview)):
+        return MemoryIo(src)
+    with open(src, "rb") as f:
+        return MemoryIo(f.read())
+
+
+class ToposImageCodec:
+    """The .toos read path (reference python/topos_codec/topos_image_binding.py:
+    ToposImageCodec.probe :463, .validate :476, .decode :523), over
+    include/topos_image.h; a source is a file path or the file's bytes."""
+
+    def __init__(self, path=LIB_PATH):
+        self._lib = load_library(path)
+
+    def status_message(self, status):
+        return self._lib.tc_image_status_message(status).decode(errors="replace")
+
+    def _detail(self, status):
+        return f"{self.status_message(status)} ({self._lib.tc_last_error().decode(errors='replace')})"
+
+    def probe_status(self, src):
+        io, info = _image_source(src), abi.ImageInfo()
+        rc = self._lib.tc_image_probe(io.ref(), ctypes.byref(info))
+        return rc, info
+
+    def validate_status(self, src, deep=False):
+        io, info = _image_source(src), abi.ImageInfo()
+        rc = self._lib.tc_image_validate(io.ref(), abi.TC_IMG_VALIDATE_DEEP if deep else 0, ctypes.byref(info))
+        return rc, info
+
+    def probe(self, src):
+        rc, info = self.probe_status(src)
+        if rc != abi.TC_OK:
+            raise ToposImageError(rc, self._detail(rc))
+        return info
+
+    def validate(self, src, deep=False):
+        rc, info = self.validate_status(src, deep)
+        if rc != abi.TC_OK:
+            raise ToposImageError(rc, self._detail(rc))
+        return info
+
+    def decode_status(self, src):
+        """(status, planes, frame output); planes are allocated from the probe."""
+        io = _image_source(src)
+        info = abi.ImageInfo()
+        rc = self._lib.tc_image_probe(io.ref(), ctypes.byref(info))
+        if rc != abi.TC_OK:
+            return rc, [], None
+        views = (abi.PlaneView * abi.TC_FRAME_MAX_PLANES)()
+        arrays = []
+        for p in range(abi.TC_FRAME_MAX_PLANES):
+            views[p].struct_size = ctypes.sizeof(abi.PlaneView)
+            views[p].abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        for p in range(int(info.plane_count)):
+            w, h = ctypes.c_uint32(0), ctypes.c_uint32(0)
+            rc = self._lib.tc_image_query_decode_buffer(ctypes.byref(info), p, ctypes.byref(w), ctypes.byref(h))
+            if rc != abi.TC_OK:
+                return rc, [], None
+            arr = np.zeros((h.value, w.value), np.uint16)
+            arrays.append(arr)
+            views[p].pixels = arr.ctypes.data
+            views[p].stride = 0
+        out = abi.FrameOutput()
+        out.struct_size = ctypes.sizeof(abi.FrameOutput)
+        out.abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        rc = self._lib.tc_image_decode(io.ref(), ctypes.cast(views, ctypes.c_void_p), ctypes.byref(out))
+        return rc, arrays, out
+
+    def decode(self, src):
+        rc, arrays, out = self.decode_status(src)
+        if rc not in (abi.TC_OK, abi.TC_WARN_CONCEALED):
+            raise ToposImageError(rc, self._detail(rc))
+        return arrays, out
 
 
 class DeviceBuffer:

@@ -130,6 +130,75 @@ class PlaneView(ctypes.Structure):
     ]
 
 
+# ---- .toos envelope (include/topos_image.h) ----
+TOPOS_IMAGE_ABI_VERSION = 1
+TC_IMG_VALIDATE_DEEP = 1
+TC_IMG_ERR_BAD_MAGIC = -100
+TC_IMG_ERR_BAD_PREAMBLE = -101
+TC_IMG_ERR_BAD_DIRECTORY = -102
+TC_IMG_ERR_UNKNOWN_CRITICAL = -103
+TC_IMG_ERR_CHUNK_CONFLICT = -104
+TC_IMG_ERR_METADATA_CONFLICT = -105
+TC_IMG_ERR_LIMIT = -106
+
+IO_READ_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t)
+
+
+class TposIo(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("abi_version", ctypes.c_uint32),
+        ("ctx", ctypes.c_void_p),
+        ("read", IO_READ_FN),
+        ("write", ctypes.c_void_p),
+        ("seek_write", ctypes.c_void_p),
+        ("length", ctypes.c_uint64),
+        ("reserved", ctypes.c_uint32 * 4),
+    ]
+
+
+class ImagePreamble(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+        ("file_version_major", ctypes.c_uint16), ("file_version_minor", ctypes.c_uint16),
+        ("flags", ctypes.c_uint32), ("file_size", ctypes.c_uint64), ("directory_offset", ctypes.c_uint64),
+        ("directory_entry_size", ctypes.c_uint32), ("directory_count", ctypes.c_uint32),
+        ("primary_image_index", ctypes.c_uint32), ("compatibility_flags", ctypes.c_uint32),
+        ("header_crc32", ctypes.c_uint32), ("directory_crc32", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 4),
+    ]
+
+
+class ImageIdsc(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+        ("idsc_version_major", ctypes.c_uint16), ("idsc_version_minor", ctypes.c_uint16),
+        ("flags", ctypes.c_uint32),
+        ("display_x_min", ctypes.c_int32), ("display_y_min", ctypes.c_int32),
+        ("display_x_max", ctypes.c_int32), ("display_y_max", ctypes.c_int32),
+        ("data_x_min", ctypes.c_int32), ("data_y_min", ctypes.c_int32),
+        ("data_x_max", ctypes.c_int32), ("data_y_max", ctypes.c_int32),
+        ("orientation", ctypes.c_uint32), ("pixel_aspect_num", ctypes.c_uint32),
+        ("pixel_aspect_den", ctypes.c_uint32),
+    ] + [(n, ctypes.c_uint8) for n in (
+        "channel_model", "channel_count", "sample_kind", "valid_bit_depth", "container_bit_depth",
+        "storage_layout", "subsampling", "chroma_siting", "alpha_presence", "alpha_mode", "alpha_bit_depth",
+        "alpha_max_abs_err", "codec_id", "payload_major", "payload_minor", "image_profile", "codec_profile",
+        "pixel_format", "color_primaries", "color_transfer", "color_matrix", "color_range", "iccp_ref",
+        "ocio_ref")] + [("payload_header_crc32", ctypes.c_uint32)]
+
+
+class ImageInfo(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+        ("preamble", ImagePreamble), ("idsc", ImageIdsc),
+        ("pixl_offset", ctypes.c_uint64), ("pixl_size", ctypes.c_uint64),
+        ("visible_width", ctypes.c_uint16), ("visible_height", ctypes.c_uint16),
+    ] + [(n, ctypes.c_uint8) for n in (
+        "plane_count", "has_alpha", "bit_depth", "profile", "pixel_format", "alpha_mode", "alpha_bit_depth",
+        "alpha_premultiplied")] + [("reserved", ctypes.c_uint32 * 8)]
+
+
 class DecoderConfig(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32),

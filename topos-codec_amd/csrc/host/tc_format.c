@@ -101,13 +101,15 @@ static void crc_init(void)
     g_crc_ready = 1;
 }
 
-uint32_t tc_crc32(const uint8_t* p, size_t n)
+uint32_t tc_crc32_update(uint32_t crc, const uint8_t* p, size_t n)
 {
     crc_init();
-    uint32_t c = 0xFFFFFFFFu;
+    uint32_t c = crc ^ 0xFFFFFFFFu;
     for (size_t i = 0; i < n; ++i) { c = g_crc_tab[(c ^ p[i]) & 0xFFu] ^ (c >> 8); }
     return c ^ 0xFFFFFFFFu;
 }
+
+uint32_t tc_crc32(const uint8_t* p, size_t n) { return tc_crc32_update(0u, p, n); }
 
 void tc_crc_tables(uint32_t crc256[256], uint32_t x2n32[32])
 {

@@ -61,6 +61,8 @@ extern const uint16_t tc_qm_tab[5][2][64];
 extern const uint8_t tc_zigzag_inv[64];
 
 uint32_t tc_crc32(const uint8_t* p, size_t n);
+/* running CRC32: tc_crc32_update(0, ...) over consecutive pieces */
+uint32_t tc_crc32_update(uint32_t crc, const uint8_t* p, size_t n);
 void tc_crc_tables(uint32_t crc256[256], uint32_t x2n32[32]);
 /* V2 VLC books (tc_vlc.c) */
 const uint8_t* tc_vlc_lengths(uint32_t family, uint32_t book, uint32_t* nsym);
