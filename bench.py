@@ -8,18 +8,21 @@ slice) at fixed qp 62. Inputs are resident in HBM before the timed region.
 
 A step is one pass of the batch through the hot path: 8 frame encodes and 8
 frame decodes, all data in HBM.
- - served-batched path (the reported value): each step is one serving step
-   (tcx_serve_step): the 8 frames are encoded with one launch sequence on one
-   stream while the 8 packets of the previous step are decoded with one
-   launch sequence on a second stream;
- - batched encode-then-decode (same batch, no overlap) and the per-frame path
-   (one launch sequence per frame and direction, with per-frame latency
-   percentiles) are reported beside it; per-kernel times and the roofline come
-   from the isolated batched steps.
-Every step is checked: packet sizes per step, both packet sets byte-identical
-to the first pass at the end, frame 0's packet equal to the reference
-encoder's (compiled reference, else the oracle) and its decode equal to the
-oracle's decode of the same packet.
+ - served-batched path (the reported value): the serving loop (tcx_server_*,
+   continuous batching over four engine lanes, HBM-budget admission) is given
+   K steps' worth of requests (8K encodes of the resident frames and 8K decodes
+   of their resident packets, interleaved) and timed until the last completes;
+ - latency under load: Poisson arrivals (half encodes, half decodes) at 50 %,
+   70 % and 90 % of that rate, p50 / p99 from submission to completion;
+ - the two-stream pipelined step (tcx_serve_step: 8 encodes on one stream
+   while the previous 8 packets decode on another), batched encode-then-decode
+   and the per-frame path (one launch sequence per frame and direction, with
+   per-frame latency percentiles) are reported beside it; per-kernel times and
+   the roofline come from the isolated batched steps.
+Every step is checked: packet sizes of every completion, both packet sets
+byte-identical to the first pass at the end, frame 0's packet equal to the
+reference encoder's (compiled reference, else the oracle) and its decode equal
+to the oracle's decode of the same packet.
 """
 import argparse
 import ctypes
@@ -119,6 +122,73 @@ def cpu_baseline(abi, frames, cfg, budget_s):
             "sample": f"{done} frames of the 4K 422 HQ qp{QP} workload (encode+decode each), {what}"}, pk0
 
 
+def server_run(pkg, srv, enc_req, dec_req, nsteps, nf, sizes):
+    """Submits nsteps x (nf encodes + nf decodes) at once, interleaved, and
+    waits for all; returns (elapsed s, completions)."""
+    t0 = time.perf_counter()
+    kinds = {}
+    for k in range(nsteps):
+        for i in range(nf):
+            for req in (enc_req(i), dec_req(i)):
+                st, rid = req()
+                assert st == 0, st
+                kinds[rid] = i
+    done = []
+    while len(done) < len(kinds):
+        for c in srv.poll(256, timeout_s=30.0):
+            assert c.status == 0, c.status
+            if c.kind == pkg.abi.TCX_REQ_ENCODE:
+                assert c.packet_size == sizes[kinds[c.id]]
+            done.append(c)
+            srv.release(c.id)
+    return time.perf_counter() - t0, done
+
+
+def server_poisson(pkg, srv, enc_req, dec_req, nf, sizes, rate_fps, seconds, seed):
+    """Poisson arrivals at rate_fps frames/s enc+dec (2 x rate_fps requests/s,
+    each an encode or a decode with probability 1/2) for `seconds`; returns
+    per-request latencies (ms) by kind and the achieved frames/s."""
+    import threading
+    rng = np.random.default_rng(seed)
+    n = max(8, int(2 * rate_fps * seconds))
+    gaps = rng.exponential(1.0 / (2 * rate_fps), size=n)
+    which = rng.integers(0, nf, size=n)
+    is_dec = rng.random(n) < 0.5
+    kinds, err = {}, []
+
+    def arrivals():
+        t = time.perf_counter()
+        for g, i, d in zip(gaps, which, is_dec):
+            t += float(g)
+            dt = t - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            st, rid = (dec_req(int(i)) if d else enc_req(int(i)))()
+            if st != 0:
+                err.append(st)
+                return
+            kinds[rid] = (int(i), bool(d))
+
+    th = threading.Thread(target=arrivals)
+    t0 = time.perf_counter()
+    th.start()
+    lat = {"enc": [], "dec": []}
+    done = 0
+    while done < n and not err:
+        for c in srv.poll(256, timeout_s=1.0):
+            assert c.status == 0, c.status
+            i, d = kinds.get(c.id, (None, None))
+            if c.kind == pkg.abi.TCX_REQ_ENCODE and i is not None:
+                assert c.packet_size == sizes[i]
+            lat["dec" if c.kind == pkg.abi.TCX_REQ_DECODE else "enc"].append((c.t_done_ns - c.t_submit_ns) / 1e6)
+            srv.release(c.id)
+            done += 1
+    th.join()
+    assert not err, err
+    elapsed = time.perf_counter() - t0
+    return lat, n / 2 / elapsed
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +198,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
+    ap.add_argument("--hbm-budget", type=int, default=24, help="serving loop HBM budget (GiB)")
+    ap.add_argument("--max-batch", type=int, default=8, help="serving loop: most frames per launch sequence")
+    ap.add_argument("--lanes", type=int, default=4, help="serving loop: launch sequences in flight")
+    ap.add_argument("--load-seconds", type=float, default=4.0, help="length of each Poisson load run")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -206,19 +280,78 @@ def main():
         assert st == 0, (er, dr, codec.last_error())
         return er[0]
 
-    for k in range(a.warmup):
-        serve_step(k)
+    # served-batched: the serving loop over requests on resident frames and packets
+    srv = pkg.Server(codec, a.hbm_budget << 30, max_batch=a.max_batch, max_inflight=a.lanes)
 
+    def enc_req(i):
+        return lambda: srv.submit_encode(cfg, dev_planes=in_addr[i])
+
+    def dec_req(i):
+        return lambda: srv.submit_decode(host_pkts[i], dev_packet=pkt_bufs[i].addr)
+
+    # warm-up (untimed), which also checks every packet the server produces and keeps frame 0's decode
+    t0 = time.perf_counter()
+    served_dec0 = None
+    kinds = {}
+    for i in all_idx:
+        kinds[enc_req(i)()[1]] = i
+        kinds[dec_req(i)()[1]] = i
+    got = 0
+    while got < 2 * nf:
+        for c in srv.poll(64, timeout_s=30.0):
+            assert c.status == 0, c.status
+            i = kinds[c.id]
+            if c.kind == abi.TCX_REQ_ENCODE:
+                buf = np.empty(c.packet_size, np.uint8)
+                assert codec._lib.tcx_d2h(pkg._ptr(buf), c.dev_packet, c.packet_size) == 0
+                assert hashlib.sha256(buf.tobytes()).hexdigest() == ref_hash[i], f"served packet {i}"
+            elif i == 0:
+                served_dec0 = []
+                for p, (w, h) in enumerate(dims):
+                    arr = np.empty((h, w), np.uint16)
+                    assert codec._lib.tcx_d2h(pkg._ptr(arr), c.dev_planes[p], w * h * 2) == 0
+                    served_dec0.append(arr)
+            srv.release(c.id)
+            got += 1
+    if a.warmup > 1:
+        server_run(pkg, srv, enc_req, dec_req, a.warmup - 1, nf, sizes)
+    print(f"# server warm-up {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    elapsed, comps = server_run(pkg, srv, enc_req, dec_req, a.steps, nf, sizes)
+    fps_batched = nf * a.steps / elapsed
+    mean_batch = float(np.mean([c.batch_frames for c in comps]))
+    sst = srv.stats()
+    print(f"# served-batched (serving loop, {a.lanes} lanes, batches up to {a.max_batch}): {fps_batched:.1f} "
+          f"frames/s enc+dec, mean batch {mean_batch:.2f}, peak HBM held {sst.peak_bytes / 2**30:.2f} GiB",
+          file=sys.stderr, flush=True)
+
+    # latency under Poisson load at 50 / 70 / 90 % of the served rate
+    loads = []
+    for load in (0.5, 0.7, 0.9):
+        lat, ach = server_poisson(pkg, srv, enc_req, dec_req, nf, sizes, load * fps_batched, a.load_seconds,
+                                  seed=int(load * 100))
+        allv = lat["enc"] + lat["dec"]
+        rec = {"load": load, "offered_fps": round(load * fps_batched, 2), "achieved_fps": round(ach, 2),
+               "requests": len(allv), "p50_ms": round(pct(allv, 50), 3), "p99_ms": round(pct(allv, 99), 3),
+               "encode_p99_ms": round(pct(lat["enc"], 99), 3) if lat["enc"] else None,
+               "decode_p99_ms": round(pct(lat["dec"], 99), 3) if lat["dec"] else None}
+        loads.append(rec)
+        print(f"# load {load:.0%}: {rec}", file=sys.stderr, flush=True)
+    srv.close()
+
+    # two-stream pipelined step (encode batch || previous decode batch)
+    for k in range(2):
+        serve_step(k)
     times = []
     t_all = time.perf_counter()
-    for k in range(a.steps):
+    npipe = max(3, min(a.steps, 6))
+    for k in range(npipe):
         t0 = time.perf_counter()
-        s = serve_step(a.warmup + k)
+        s = serve_step(k)
         times.append(time.perf_counter() - t0)
         assert list(s) == list(sizes)
-    elapsed = time.perf_counter() - t_all
-    fps_batched = nf * a.steps / elapsed
-    print(f"# served-batched (encode || decode): {fps_batched:.1f} frames/s enc+dec, step p50 "
+    fps_pipe = nf * npipe / (time.perf_counter() - t_all)
+    print(f"# pipelined step (encode || decode): {fps_pipe:.1f} frames/s enc+dec, step p50 "
           f"{pct(times, 50) * 1e3:.2f} ms", file=sys.stderr, flush=True)
 
     # isolated batched steps (encode, then decode) with per-kernel HIP event timing
@@ -279,6 +412,9 @@ def main():
         bitexact["packet0_equals_oracle"] = rc == 0 and opk == host_pkts[0]
     rc, oplanes, _, _ = orc_decode(orc, host_pkts[0], dims)
     bitexact["decode0_equals_oracle"] = rc == 0 and all((x == y).all() for x, y in zip(oplanes, dec0))
+    bitexact["served_packets_equal_batched"] = True  # every warm-up packet hash checked above
+    bitexact["served_decode0_equals_oracle"] = served_dec0 is not None and all(
+        (x == y).all() for x, y in zip(oplanes, served_dec0))
 
     # roofline of the dominant HBM-bound kernel: fused FDCT + quantisation,
     # 2 B (u16 sample in) + 4 B (i32 coefficient out) per sample
@@ -288,8 +424,10 @@ def main():
     idct_ms = float(np.mean(kt["idct"]))
     db_ms = float(np.mean(kt["deblock"]))
     ms_step = elapsed / a.steps * 1e3
-    step_desc = ("served-batched step: the 8-frame batch is encoded on one stream while the previous "
-                 "step's 8 packets decode on another (tcx_serve_step), 8 encodes + 8 decodes per step")
+    step_desc = ("one step = the 8 resident frames encoded and their 8 resident packets decoded, as 16 "
+                 "requests to the serving loop (continuous batching, encode and decode batches in flight "
+                 "together); K steps are submitted at once and timed until the last completes")
+    at70 = next(r for r in loads if r["load"] == 0.7)
     out = {
         "metric": "4K 422HQ 10b frames/s enc+dec, served-batched vs per-frame, bit-exact; p99 ms",
         "value": round(fps_batched, 2),
@@ -306,15 +444,22 @@ def main():
         "config": {"workload": f"{nf} x 3840x2160 YUV 4:2:2 10-bit, 422 HQ (rANS2, qm0, deblock+QPT2, "
                                f"8 block rows/slice), fixed qp {QP}, encode+decode per frame",
                    "frames_per_step": nf, "packet_bytes_mean": int(np.mean(sizes)),
-                   "parallelism": "1 GPU, one launch sequence per batch and direction, encode and decode on two streams",
-                   "step": step_desc},
+                   "parallelism": f"1 GPU, serving loop with {a.lanes} lanes (launch sequences in flight), "
+                                  f"batches up to {a.max_batch} frames, HBM budget {a.hbm_budget} GiB",
+                   "step": step_desc, "mean_batch_frames": round(mean_batch, 2),
+                   "peak_hbm_held_bytes": int(sst.peak_bytes)},
         "batched_encode_then_decode": {"value": round(fps_seq, 2), "unit": "frames/s"},
         "per_frame_path": {"value": round(fps_pf, 2) if fps_pf else None, "unit": "frames/s",
                            "p50_ms": round(pct(pf_lat, 50) * 1e3, 3) if pf_lat else None,
                            "p99_ms": round(pct(pf_lat, 99) * 1e3, 3) if pf_lat else None},
         "batched_vs_per_frame": round(fps_batched / fps_pf, 3) if fps_pf else None,
-        "latency_ms": {"batch_step_p50": round(pct(times, 50) * 1e3, 3), "batch_step_p99": round(pct(times, 99) * 1e3, 3),
-                       "frame_p99_served": round(2 * pct(times, 99) * 1e3, 3)},
+        "latency_ms": {"load": 0.7, "p50": at70["p50_ms"], "p99": at70["p99_ms"],
+                       "what": "request latency (submission to completion) under Poisson arrivals at 70 % of the "
+                               "served rate, encodes and decodes together"},
+        "served_at_load": loads,
+        "pipelined_step": {"value": round(fps_pipe, 2), "unit": "frames/s",
+                           "step_p50_ms": round(pct(times, 50) * 1e3, 3), "step_p99_ms": round(pct(times, 99) * 1e3, 3),
+                           "what": "tcx_serve_step: 8 encodes on one stream while the previous 8 packets decode"},
         "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": "k_fdct_quant", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -79,6 +79,84 @@ typedef struct tcx_dec_batch {
 
 int32_t tcx_serve_step(const tcx_enc_batch* enc, const tcx_dec_batch* dec);
 
+/* ---- serving loop ------------------------------------------------------
+ * A request loop over the batch engine. Encode and decode requests of any
+ * size, profile and format are accepted as they arrive (tcx_server_submit
+ * never blocks), admitted in arrival order against a budget of HBM, and
+ * grouped into launch sequences whenever an engine lane is free (continuous
+ * batching): a request that arrives while others run joins the next batch
+ * and its work overlaps theirs on the device. Inputs may be resident in HBM
+ * or host buffers staged into the budget; results stay in HBM until the
+ * caller releases them. All times are tcx_now_ns() values. */
+typedef struct tcx_server tcx_server;
+
+typedef struct tcx_server_config {
+    size_t hbm_budget;      /* bytes of HBM for admitted requests (staged inputs, packets, planes) */
+    uint32_t max_batch;     /* most frames per launch sequence, 1..64 (0: 8) */
+    uint32_t max_inflight;  /* launch sequences in flight at once, 1..4 (0: 4) */
+} tcx_server_config;
+
+#define TCX_REQ_ENCODE 1u
+#define TCX_REQ_DECODE 2u
+
+typedef struct tcx_request {
+    uint32_t kind;                    /* TCX_REQ_ENCODE or TCX_REQ_DECODE */
+    uint64_t user;                    /* caller's tag, returned in the completion */
+    /* encode: the frame config and its input planes, resident (dev_planes) or
+     * host (host_planes, staged into the budget at launch); strides in
+     * elements, 0 = tight */
+    topos_frame_config cfg;
+    uint64_t dev_planes[4];
+    const uint16_t* host_planes[4];
+    size_t strides[4];
+    /* decode: host bytes of the packet (structure parse, and the staged copy
+     * unless dev_packet holds a resident one); must stay valid until the
+     * request completes */
+    const uint8_t* packet;
+    size_t packet_size;
+    uint64_t dev_packet;
+    /* outputs; 0 = allocated by the server from its budget */
+    uint64_t dev_out;                 /* encode: packet buffer */
+    size_t dev_out_cap;
+    uint64_t dev_out_planes[4];       /* decode: output planes */
+    size_t out_strides[4];
+    topos_frame_output* info;         /* decode: slice status etc., filled by completion (optional) */
+    topos_frame_stats* stats;         /* encode: filled by completion (optional) */
+} tcx_request;
+
+typedef struct tcx_completion {
+    uint64_t id, user;
+    uint32_t kind;
+    int32_t status;                   /* TC_OK, TC_WARN_CONCEALED or a TC_ERR_* code */
+    uint32_t packet_size;             /* encode */
+    uint32_t batch_frames;            /* frames in the launch sequence that ran it */
+    uint64_t dev_packet;              /* encode: the packet in HBM */
+    uint64_t dev_planes[4];           /* decode: the planes in HBM */
+    size_t strides[4];                /* decode: their strides (elements) */
+    int64_t t_submit_ns, t_admit_ns, t_launch_ns, t_done_ns;
+} tcx_completion;
+
+typedef struct tcx_server_stats {
+    uint64_t submitted, completed, batches, batch_frames;
+    uint64_t budget_bytes, held_bytes, peak_bytes;
+    uint32_t queued, inflight;
+} tcx_server_stats;
+
+int64_t tcx_now_ns(void);
+int32_t tcx_server_create(const tcx_server_config* cfg, tcx_server** out);
+/* validates the request (config rules / packet structure) and queues it;
+ * a request larger than the whole budget fails with TC_ERR_OUT_OF_MEMORY */
+int32_t tcx_server_submit(tcx_server* s, const tcx_request* req, uint64_t* id);
+/* waits up to timeout_ns (0: no wait, < 0: forever) for completions and
+ * returns up to max of them in completion order (*n may be 0) */
+int32_t tcx_server_poll(tcx_server* s, tcx_completion* out, uint32_t max, uint32_t* n, int64_t timeout_ns);
+/* every completed request is released once: its server-owned HBM returns
+ * to the budget */
+int32_t tcx_server_release(tcx_server* s, uint64_t id);
+int32_t tcx_server_get_stats(tcx_server* s, tcx_server_stats* st);
+/* finishes queued and running work, then frees everything */
+void tcx_server_destroy(tcx_server* s);
+
 /* per-kernel times of the last finished encode / decode launch sequence (ms), measured
  * with HIP events on the launch stream when enabled: fdct, slice_enc, scan,
  * emit, slice_dec, idct, deblock */

@@ -58,6 +58,14 @@ def load_library(path=LIB_PATH):
     lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
+    lib.tcx_now_ns.restype = ctypes.c_int64
+    lib.tcx_server_create.argtypes = [vp, vp]
+    lib.tcx_server_submit.argtypes = [vp, vp, vp]
+    lib.tcx_server_poll.argtypes = [vp, vp, ctypes.c_uint32, vp, ctypes.c_int64]
+    lib.tcx_server_release.argtypes = [vp, ctypes.c_uint64]
+    lib.tcx_server_get_stats.argtypes = [vp, vp]
+    lib.tcx_server_destroy.argtypes = [vp]
+    lib.tcx_server_destroy.restype = None
     lib.tc_image_status_message.restype = ctypes.c_char_p
     lib.tc_image_status_message.argtypes = [ctypes.c_int32]
     lib.tc_image_probe.argtypes = [vp, vp]
@@ -406,6 +414,110 @@ class DeviceCodec(ToposCodec):
         er = (list(ea["sizes"]), list(ea["rcs"]), ea["stats"]) if ea else None
         dr = (list(da["rcs"]), da["infos"]) if da else None
         return st, er, dr
+
+
+class Server:
+    """The serving loop (tcx_server_*): submit encode / decode requests as they
+    arrive, poll completions, release results. Request buffers handed in as
+    numpy arrays or bytes are kept alive here until the request is released."""
+
+    def __init__(self, codec, hbm_budget, max_batch=8, max_inflight=4):
+        self._lib = codec._lib
+        self._codec = codec
+        cfg = abi.TcxServerConfig(hbm_budget, max_batch, max_inflight)
+        self._h = ctypes.c_void_p()
+        st = self._lib.tcx_server_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if st != 0:
+            raise ToposCodecError(st, codec.last_error())
+        self._keep = {}
+
+    def now_ns(self):
+        return int(self._lib.tcx_now_ns())
+
+    def _submit(self, req, keep):
+        rid = ctypes.c_uint64(0)
+        st = int(self._lib.tcx_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
+        if st != 0:
+            return st, None
+        self._keep[rid.value] = keep
+        return 0, rid.value
+
+    def submit_encode(self, cfg, dev_planes=None, host_planes=None, strides=None, user=0, stats=None):
+        """Returns (status, id). dev_planes: device addresses; host_planes: uint16 arrays."""
+        req = abi.TcxRequest()
+        req.kind, req.user, req.cfg = abi.TCX_REQ_ENCODE, user, cfg
+        keep = [req]
+        if host_planes is not None:
+            arrs = [np.ascontiguousarray(p, dtype=np.uint16) for p in host_planes]
+            keep += arrs
+            for i, a in enumerate(arrs):
+                req.host_planes[i] = a.ctypes.data
+        else:
+            for i, a in enumerate(dev_planes):
+                req.dev_planes[i] = a
+        if strides is not None:
+            for i, v in enumerate(strides):
+                req.strides[i] = v
+        if stats is not None:
+            req.stats = ctypes.cast(ctypes.byref(stats), ctypes.c_void_p)
+            keep.append(stats)
+        return self._submit(req, keep)
+
+    def submit_decode(self, packet, dev_packet=0, user=0, info=None, dev_out_planes=None, out_strides=None):
+        """Returns (status, id); packet: the host bytes (kept until release)."""
+        req = abi.TcxRequest()
+        req.kind, req.user = abi.TCX_REQ_DECODE, user
+        buf = np.frombuffer(packet, np.uint8)
+        req.packet, req.packet_size, req.dev_packet = buf.ctypes.data, len(packet), dev_packet
+        keep = [req, buf, packet]
+        if info is not None:
+            req.info = ctypes.cast(ctypes.byref(info), ctypes.c_void_p)
+            keep.append(info)
+        if dev_out_planes is not None:
+            for i, a in enumerate(dev_out_planes):
+                req.dev_out_planes[i] = a
+        if out_strides is not None:
+            for i, v in enumerate(out_strides):
+                req.out_strides[i] = v
+        return self._submit(req, keep)
+
+    def poll(self, max_n=64, timeout_s=None):
+        """Completions as TcxCompletion structs; timeout None waits for at least one."""
+        out = (abi.TcxCompletion * max_n)()
+        n = ctypes.c_uint32(0)
+        t = -1 if timeout_s is None else int(timeout_s * 1e9)
+        st = self._lib.tcx_server_poll(self._h, out, max_n, ctypes.byref(n), ctypes.c_int64(t))
+        if st != 0:
+            raise ToposCodecError(st, self._codec.last_error())
+        return [out[i] for i in range(n.value)]
+
+    def release(self, rid):
+        st = int(self._lib.tcx_server_release(self._h, ctypes.c_uint64(rid)))
+        self._keep.pop(rid, None)
+        return st
+
+    def stats(self):
+        s = abi.TcxServerStats()
+        self._lib.tcx_server_get_stats(self._h, ctypes.byref(s))
+        return s
+
+    def close(self):
+        if self._h:
+            self._lib.tcx_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class TcxEncBatch(ctypes.Structure):

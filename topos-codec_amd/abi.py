@@ -254,3 +254,55 @@ def plane_dims(width, height, pixel_format, alpha_mode):
     if alpha_mode:
         dims.append((width, height))
     return dims
+
+
+# ---- serving loop (include/topos_gpu.h; no counterpart in the reference ABI)
+TCX_REQ_ENCODE = 1
+TCX_REQ_DECODE = 2
+
+
+class TcxServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch", ctypes.c_uint32),
+                ("max_inflight", ctypes.c_uint32)]
+
+
+class TcxRequest(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32),
+        ("user", ctypes.c_uint64),
+        ("cfg", FrameConfig),
+        ("dev_planes", ctypes.c_uint64 * 4),
+        ("host_planes", ctypes.c_void_p * 4),
+        ("strides", ctypes.c_size_t * 4),
+        ("packet", ctypes.c_void_p),
+        ("packet_size", ctypes.c_size_t),
+        ("dev_packet", ctypes.c_uint64),
+        ("dev_out", ctypes.c_uint64),
+        ("dev_out_cap", ctypes.c_size_t),
+        ("dev_out_planes", ctypes.c_uint64 * 4),
+        ("out_strides", ctypes.c_size_t * 4),
+        ("info", ctypes.c_void_p),
+        ("stats", ctypes.c_void_p),
+    ]
+
+
+class TcxCompletion(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64), ("user", ctypes.c_uint64),
+        ("kind", ctypes.c_uint32), ("status", ctypes.c_int32),
+        ("packet_size", ctypes.c_uint32), ("batch_frames", ctypes.c_uint32),
+        ("dev_packet", ctypes.c_uint64),
+        ("dev_planes", ctypes.c_uint64 * 4),
+        ("strides", ctypes.c_size_t * 4),
+        ("t_submit_ns", ctypes.c_int64), ("t_admit_ns", ctypes.c_int64),
+        ("t_launch_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64),
+    ]
+
+
+class TcxServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64), ("batch_frames", ctypes.c_uint64),
+        ("budget_bytes", ctypes.c_uint64), ("held_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
+        ("queued", ctypes.c_uint32), ("inflight", ctypes.c_uint32),
+    ]

@@ -120,7 +120,7 @@ static int32_t engine_init_locked(void)
     return g_state == 1 ? TC_OK : tc_fail(g_init_rc, "GPU backend unavailable: %s", g_init_msg);
 }
 
-int32_t tc_engine_acquire(tc_ectx** out)
+int32_t tc_engine_try_acquire(tc_ectx** out)
 {
     pthread_mutex_lock(&g_mu);
     int32_t rc = engine_init_locked();
@@ -146,6 +146,17 @@ int32_t tc_engine_acquire(tc_ectx** out)
         *out = c;
         return TC_OK;
     }
+    pthread_mutex_unlock(&g_mu);
+    *out = NULL;
+    return 1;
+}
+
+int32_t tc_engine_acquire(tc_ectx** out)
+{
+    const int32_t rc = tc_engine_try_acquire(out);
+    if (rc != 1) { return rc; }
+    /* every context is in use: wait for one, round robin */
+    pthread_mutex_lock(&g_mu);
     tc_ectx* c = &g_ctx[g_rr++ % g_nctx];
     pthread_mutex_unlock(&g_mu);
     pthread_mutex_lock(&c->mu);
@@ -154,6 +165,24 @@ int32_t tc_engine_acquire(tc_ectx** out)
 }
 
 void tc_engine_release(tc_ectx* c) { pthread_mutex_unlock(&c->mu); }
+
+size_t tc_engine_encode_cap(const topos_frame_config* cfg)
+{
+    tc_fhdr fh;
+    tc_cfg_to_fhdr(cfg, &fh);
+    if (tc_fhdr_geometry(&fh) != TC_OK) { return 0u; }
+    const uint32_t sr = tc_slice_rows_of(cfg);
+    uint32_t bands[4], total = 0;
+    if (sr == 0u || tc_band_plan(&fh, sr, bands, &total) != TC_OK) { return 0u; }
+    size_t cap = TC_HDR_SIZE;
+    for (uint32_t p = 0; p < fh.plane_count; ++p) {
+        for (uint32_t y0 = 0; y0 < fh.brows[p]; y0 += sr) {
+            const uint32_t h = fh.brows[p] - y0 < sr ? fh.brows[p] - y0 : sr;
+            cap += TC_SLICE_HDR_SIZE + (size_t)h * fh.bcols[p] * slice_bytes_per_block(&fh, p) + TCG_SLICE_BYTES_EXTRA;
+        }
+    }
+    return cap < TC_MAX_PACKET ? cap : TC_MAX_PACKET;
+}
 
 tcg_ctx* tc_engine_stream(tc_ectx* c) { return c->g; }
 

@@ -89,7 +89,12 @@ void tc_fill_output_info(const tc_fhdr* fh, topos_frame_output* info);
 typedef struct tc_ectx tc_ectx;
 /* lazily initialises the GPU; returns a free context (locked until release) */
 int32_t tc_engine_acquire(tc_ectx** out);
+/* never blocks: 1 (and *out = NULL) when every context is in use */
+int32_t tc_engine_try_acquire(tc_ectx** out);
 void tc_engine_release(tc_ectx* c);
+/* device packet capacity the encoder needs for one frame of cfg (0 on a bad
+ * config): frame header plus every slice's header and entropy bound */
+size_t tc_engine_encode_cap(const topos_frame_config* cfg);
 tcg_ctx* tc_engine_stream(tc_ectx* c);
 /* per-kernel times of the last finished encode and decode (timing on) */
 void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec);

@@ -189,6 +189,14 @@ int32_t tcg_sync(tcg_ctx* c)
     return e == hipSuccess ? 0 : fail("hipStreamSynchronize", e);
 }
 
+int32_t tcg_query(tcg_ctx* c)
+{
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) { return 0; }
+    if (e == hipErrorNotReady) { return 1; }
+    return fail("hipStreamQuery", e);
+}
+
 int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uint32_t* x2n32)
 {
     TcgConsts h;

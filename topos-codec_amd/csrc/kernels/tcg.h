@@ -180,6 +180,9 @@ int32_t tcg_h2d_2d(tcg_ctx* c, uint64_t dst, size_t dpitch, const void* src, siz
                    size_t rows);
 int32_t tcg_d2h_2d(tcg_ctx* c, void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows);
 int32_t tcg_sync(tcg_ctx* c);
+/* 0 when everything queued on the context's stream has finished, 1 while
+ * work is pending, negative on a device error (never blocks) */
+int32_t tcg_query(tcg_ctx* c);
 /* device copy of bytes (multiple of 16) with a plain vector-copy kernel */
 int32_t tcg_copy(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes);
 /* upload the (12 - log2 f) cost table, f = 0..4096 (entry 0 unused), and
