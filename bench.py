@@ -9,7 +9,7 @@ slice) at fixed qp 62. Inputs are resident in HBM before the timed region.
 A step is one pass of the batch through the hot path: 8 frame encodes and 8
 frame decodes, all data in HBM.
  - served-batched path (the reported value): the serving loop (tcx_server_*,
-   continuous batching over four engine lanes, HBM-budget admission) is given
+   continuous batching over two engine lanes, batches of up to 32 frames, HBM-budget admission) is given
    K steps' worth of requests (8K encodes of the resident frames and 8K decodes
    of their resident packets, interleaved) and timed until the last completes;
  - latency under load: Poisson arrivals (half encodes, half decodes) at 50 %,
@@ -199,8 +199,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
     ap.add_argument("--hbm-budget", type=int, default=24, help="serving loop HBM budget (GiB)")
-    ap.add_argument("--max-batch", type=int, default=8, help="serving loop: most frames per launch sequence")
-    ap.add_argument("--lanes", type=int, default=4, help="serving loop: launch sequences in flight")
+    ap.add_argument("--max-batch", type=int, default=32, help="serving loop: most frames per launch sequence")
+    ap.add_argument("--lanes", type=int, default=2, help="serving loop: launch sequences in flight")
     ap.add_argument("--load-seconds", type=float, default=4.0, help="length of each Poisson load run")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration)")

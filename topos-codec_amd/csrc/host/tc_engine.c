@@ -282,6 +282,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
     tcg_eframe* hf = (tcg_eframe*)(c->hdesc + o_fr);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0;
     size_t in_off = 0, scr_off = 0, pkt_off = 0;
+    tcg_batch_info info = { 1u, 0u };
     for (uint32_t f = 0; f < n && rc == TC_OK; ++f) {
         tc_enc_req* r = &reqs[f];
         enc_frame_plan* P = &plan[f];
@@ -314,8 +315,11 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
             if (tc_is_alpha(fh, p)) {
                 E->kind = 1u;
                 E->alpha_shift = fh->alpha_mode == 2u ? 16u - fh->alpha_bit_depth : 0u;
+                info.kinds |= TCG_HAS_ALPHA;
             } else {
                 E->kind = 0u;
+                info.kinds |= slice_mode(fh) == TCG_MODE_RANS2 ? TCG_HAS_RANS2 : TCG_HAS_BITS;
+                info.max_cols = fh->bcols[p] > info.max_cols ? fh->bcols[p] : info.max_cols;
                 tc_qctx_build(&hq[iq], tc_qm_tab[fh->qmatrix_id][tc_is_chroma(fh, p) ? 1 : 0],
                               (uint32_t)P->qp_eff[p], fh->bit_depth, tbl);
                 tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
@@ -363,7 +367,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
                    (const tcg_eslice*)(uintptr_t)(c->desc.p + o_sl), n,
                    (const tcg_eframe*)(uintptr_t)(c->desc.p + o_fr), c->q.p, c->dcm.p, ntiles, c->scratch.p,
                    (tcg_eresult*)(uintptr_t)(c->res.p + o_res),
-                   (tcg_eframe_out*)(uintptr_t)(c->res.p + o_fo)) != 0) {
+                   (tcg_eframe_out*)(uintptr_t)(c->res.p + o_fo), &info) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "encode launch failed: %s", tcg_last_error());
     }
@@ -489,6 +493,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     tcg_rowrec* hrs = (tcg_rowrec*)(c->hdesc + o_rs);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0;
     size_t pkt_off = 0, out_off = 0, tmp_off = 0;
+    tcg_batch_info info = { 1u, 0u };
     uint64_t* out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
     if (out_dev == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode plan"); }
     for (uint32_t f = 0; f < n; ++f) {
@@ -527,11 +532,14 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             D->tile0 = tile;
             if (tc_is_alpha(fh, p)) {
                 D->kind = 1u;
+                info.kinds |= TCG_HAS_ALPHA;
                 D->fill = 0xFFFFu;
                 D->maxv = 0xFFFFu;
                 D->alpha_mask = fh->alpha_mode == 2u ? (1u << (16u - fh->alpha_bit_depth)) - 1u : 0u;
             } else {
                 D->kind = 0u;
+                info.kinds |= slice_mode(fh) == TCG_MODE_RANS2 ? TCG_HAS_RANS2 : TCG_HAS_BITS;
+                info.max_cols = fh->bcols[p] > info.max_cols ? fh->bcols[p] : info.max_cols;
                 D->fill = 1u << (fh->bit_depth - 1u);
                 D->maxv = (1u << fh->bit_depth) - 1u;
                 tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
@@ -612,7 +620,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                    (const tcg_dplane*)(uintptr_t)(c->desc.p + o_pl), nslices,
                    (const tcg_dslice*)(uintptr_t)(c->desc.p + o_sl),
                    (const tcg_rowrec*)(uintptr_t)(c->desc.p + o_rs), maxrows, c->q.p, ntiles,
-                   (int32_t*)(uintptr_t)c->res.p, dbt) != 0) {
+                   (int32_t*)(uintptr_t)c->res.p, dbt, &info) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode launch failed: %s", tcg_last_error());
     }

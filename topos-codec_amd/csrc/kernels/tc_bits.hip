@@ -114,7 +114,7 @@ __global__ __launch_bounds__(64) void k_slice_encode_bits(const TcgConsts* __res
 {
     __shared__ uint32_t sRing[BW_RING];
     __shared__ uint32_t sH[BH_TOTAL];
-    __shared__ int32_t sDcRow[2048];
+    extern __shared__ int32_t sDcRow[];  // one per block column of the batch's widest plane
     __shared__ uint32_t sCode[3][64], sLen[3][64];  // chosen books: 0 DC, 1 run, 2 level
 
     const uint32_t lane = threadIdx.x;
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(64) void k_slice_decode_bits(const TcgConsts* __res
 {
     __shared__ uint32_t sTab[256];
     __shared__ uint32_t sX2n[32];
-    __shared__ int32_t sDcRow[2048];
+    extern __shared__ int32_t sDcRow[];  // one per block column of the batch's widest plane
 
     const uint32_t lane = threadIdx.x;
     const tcg_dslice S = slices[blockIdx.x];
@@ -557,14 +557,16 @@ __global__ __launch_bounds__(64) void k_slice_decode_bits(const TcgConsts* __res
 // ---------------------------------------------------------------------------
 void tcg_bits_encode_launch(hipStream_t st, const TcgConsts* K, const tcg_eplane* planes,
                             const tcg_eslice* slices, uint32_t nslices, const int32_t* qbuf, uint32_t* dcmbuf,
-                            uint8_t* scratch, tcg_eresult* results)
+                            uint8_t* scratch, tcg_eresult* results, uint32_t max_cols)
 {
-    hipLaunchKernelGGL(k_slice_encode_bits, dim3(nslices), dim3(64), 0, st, K, planes, slices, qbuf, dcmbuf,
-                       scratch, results);
+    hipLaunchKernelGGL(k_slice_encode_bits, dim3(nslices), dim3(64), (size_t)max_cols * 4u, st, K, planes, slices,
+                       qbuf, dcmbuf, scratch, results);
 }
 
 void tcg_bits_decode_launch(hipStream_t st, const TcgConsts* K, const tcg_dplane* planes,
-                            const tcg_dslice* slices, uint32_t nslices, int32_t* qbuf, int32_t* status)
+                            const tcg_dslice* slices, uint32_t nslices, int32_t* qbuf, int32_t* status,
+                            uint32_t max_cols)
 {
-    hipLaunchKernelGGL(k_slice_decode_bits, dim3(nslices), dim3(64), 0, st, K, planes, slices, qbuf, status);
+    hipLaunchKernelGGL(k_slice_decode_bits, dim3(nslices), dim3(64), (size_t)max_cols * 4u, st, K, planes, slices,
+                       qbuf, status);
 }

@@ -222,8 +222,10 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
     __shared__ uint32_t sX2n[32];
     __shared__ uint16_t sFreq[11][64];
     __shared__ uint16_t sCum[11][65];
-    __shared__ int32_t sDcRow[2048];
     __shared__ int32_t sBad[1];
+    // DC values of the block row above, one per block column (dynamic: the
+    // batch's widest plane)
+    extern __shared__ int32_t sDcRow[];
 
     const uint32_t lane = threadIdx.x;
     const tcg_dslice S = slices[blockIdx.x];
@@ -522,16 +524,24 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
 extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
                               uint32_t nslices, const tcg_dslice* slices_dev, const tcg_rowrec* rowrec_dev,
                               uint32_t max_rows, uint64_t qbuf, uint32_t max_plane_tiles,
-                              int32_t* slice_status_dev, uint32_t max_db_tiles)
+                              int32_t* slice_status_dev, uint32_t max_db_tiles, const tcg_batch_info* info)
 {
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 2;
     tcg_mark(c, 0);
-    hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
-                       (int32_t*)qbuf, slice_status_dev);
-    tcg_bits_decode_launch(st, K, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
-    tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
+    const size_t dc_lds = (size_t)info->max_cols * sizeof(int32_t);
+    if (info->kinds & TCG_HAS_RANS2) {
+        hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), dc_lds, st, K, planes_dev, slices_dev,
+                           (int32_t*)qbuf, slice_status_dev);
+    }
+    if (info->kinds & TCG_HAS_BITS) {
+        tcg_bits_decode_launch(st, K, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev,
+                               info->max_cols);
+    }
+    if (info->kinds & TCG_HAS_ALPHA) {
+        tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
+    }
     tcg_mark(c, 1);
     if (max_plane_tiles > 0u) {
         hipLaunchKernelGGL(k_dequant_idct, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev, planes_dev,

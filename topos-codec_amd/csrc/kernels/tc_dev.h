@@ -128,8 +128,9 @@ void tcg_alpha_decode_launch(hipStream_t st, const tcg_dplane* planes, const tcg
 // V1 Rice / V2 VLC colour slice kernels (tc_bits.hip)
 void tcg_bits_encode_launch(hipStream_t st, const TcgConsts* K, const tcg_eplane* planes,
                             const tcg_eslice* slices, uint32_t nslices, const int32_t* qbuf, uint32_t* dcmbuf,
-                            uint8_t* scratch, tcg_eresult* results);
+                            uint8_t* scratch, tcg_eresult* results, uint32_t max_cols);
 void tcg_bits_decode_launch(hipStream_t st, const TcgConsts* K, const tcg_dplane* planes,
-                            const tcg_dslice* slices, uint32_t nslices, int32_t* qbuf, int32_t* status);
+                            const tcg_dslice* slices, uint32_t nslices, int32_t* qbuf, int32_t* status,
+                            uint32_t max_cols);
 
 #endif

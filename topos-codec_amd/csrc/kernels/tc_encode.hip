@@ -265,6 +265,8 @@ __device__ static inline uint32_t put_suffix_wide(uint32_t x, uint32_t b, uint32
 __device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 #define ENC_GROUP 8
+// words of the model tables: row bytes, frequencies and cumulative frequencies
+#define ENC_MODEL_WORDS ((NROWS * 64 + NROWS * 64 * 2 + NROWS * 65 * 2 + 3) / 4)
 
 __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict__ K,
                                                       const tcg_eplane* __restrict__ planes,
@@ -274,16 +276,25 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
                                                       uint8_t* __restrict__ scratch,
                                                       tcg_eresult* __restrict__ results)
 {
-    __shared__ uint32_t sH[H_TOTAL];
-    __shared__ int32_t sDcRow[2048];
-    __shared__ uint8_t sRowB[NROWS][64];
-    __shared__ uint16_t sFreq[NROWS][64];
-    __shared__ uint16_t sCum[NROWS][65];
     __shared__ double sCost[5];
-    __shared__ uint32_t sTabA[11 * 64];
-    __shared__ uint32_t sTabB[11 * 64];
-    __shared__ uint32_t sTabX[11 * 64];
     __shared__ uint8_t sPrefix[TCG_PREFIX_MAX];
+    // dynamic LDS (words), reused across the kernel's phases:
+    //  forward pass      [0, H_TOTAL) histograms, then the DC values of the
+    //                    block row above (one per block column of the batch's
+    //                    widest plane)
+    //  model choice      histograms, then the rows' bytes, frequencies and
+    //                    cumulative frequencies over the dead DC row
+    //  backward pass     [0, 3 x 11 x 64) per-symbol encoder constants, stored
+    //                    from registers after the model tables' last read
+    extern __shared__ uint32_t sDyn[];
+    uint32_t* const sH = sDyn;
+    int32_t* const sDcRow = (int32_t*)(sDyn + H_TOTAL);
+    uint8_t (*const sRowB)[64] = (uint8_t (*)[64])(sDyn + H_TOTAL);
+    uint16_t (*const sFreq)[64] = (uint16_t (*)[64])((uint8_t*)(sDyn + H_TOTAL) + NROWS * 64);
+    uint16_t (*const sCum)[65] = (uint16_t (*)[65])((uint8_t*)(sDyn + H_TOTAL) + NROWS * 64 * 3);
+    uint32_t* const sTabA = sDyn;
+    uint32_t* const sTabB = sDyn + 11 * 64;
+    uint32_t* const sTabX = sDyn + 2 * 11 * 64;
 
     const uint32_t lane = threadIdx.x;
     const tcg_eslice S = slices[blockIdx.x];
@@ -420,18 +431,30 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         for (uint32_t i = lane; i < lvl_n * 28u; i += 64u) {
             sPrefix[1u + dc_n * 29u + 64u + i] = sRowB[lvl_r0 + i / 28u][i % 28u];
         }
-        // per-symbol encoder constants
-        for (uint32_t slot = 0; slot < dc_n + 1u + lvl_n; ++slot) {
+    }
+    {
+        // per-symbol encoder constants: computed into registers while the
+        // model tables are live, stored over them after the barrier
+        uint32_t tA[11], tB[11], tX[11];
+        const uint32_t nslot = dc_n + 1u + lvl_n;
+#pragma unroll
+        for (uint32_t slot = 0; slot < 11u; ++slot) {
             uint32_t row, nsym;
             if (slot < dc_n) { row = dc_r0 + slot; nsym = 29u; }
             else if (slot == dc_n) { row = 16u; nsym = 64u; }
             else { row = lvl_r0 + (slot - dc_n - 1u); nsym = 28u; }
-            if (lane < nsym) {
-                uint32_t A, B, X;
-                tc_rans_enc_consts(sFreq[row][lane], sCum[row][lane], &A, &B, &X);
-                sTabA[slot * 64u + lane] = A;
-                sTabB[slot * 64u + lane] = B;
-                sTabX[slot * 64u + lane] = X;
+            tA[slot] = tB[slot] = tX[slot] = 0u;
+            if (slot < nslot && lane < nsym) {
+                tc_rans_enc_consts(sFreq[row][lane], sCum[row][lane], &tA[slot], &tB[slot], &tX[slot]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t slot = 0; slot < 11u; ++slot) {
+            if (slot < nslot) {
+                sTabA[slot * 64u + lane] = tA[slot];
+                sTabB[slot * 64u + lane] = tB[slot];
+                sTabX[slot * 64u + lane] = tX[slot];
             }
         }
     }
@@ -725,7 +748,7 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
                               uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
                               const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
                               uint32_t max_plane_tiles, uint64_t scratch, tcg_eresult* results_dev,
-                              tcg_eframe_out* frame_out_dev)
+                              tcg_eframe_out* frame_out_dev, const tcg_batch_info* info)
 {
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
@@ -737,12 +760,19 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
                            (int32_t*)qbuf);
     }
     tcg_mark(c, 1);
-    hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), 0, st, K, planes_dev, slices_dev,
-                       (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
-    tcg_bits_encode_launch(st, K, planes_dev, slices_dev, nslices, (const int32_t*)qbuf, (uint32_t*)dcmbuf,
-                           (uint8_t*)scratch, results_dev);
-    tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch,
-                            results_dev);
+    if (info->kinds & TCG_HAS_RANS2) {
+        const size_t words = max((size_t)H_TOTAL + max((size_t)info->max_cols, (size_t)ENC_MODEL_WORDS),
+                                 (size_t)3 * 11 * 64);
+        hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), words * 4u, st, K, planes_dev, slices_dev,
+                           (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
+    }
+    if (info->kinds & TCG_HAS_BITS) {
+        tcg_bits_encode_launch(st, K, planes_dev, slices_dev, nslices, (const int32_t*)qbuf, (uint32_t*)dcmbuf,
+                               (uint8_t*)scratch, results_dev, info->max_cols);
+    }
+    if (info->kinds & TCG_HAS_ALPHA) {
+        tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch, results_dev);
+    }
     tcg_mark(c, 2);
     hipLaunchKernelGGL(k_frame_scan, dim3(nframes), dim3(256), 0, st, K, frames_dev, results_dev,
                        slices_dev, planes_dev, frame_out_dev);

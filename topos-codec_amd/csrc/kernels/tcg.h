@@ -191,12 +191,23 @@ int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uin
 /* upload the V2 VLC books */
 int32_t tcg_set_vlc(const tcg_vlc_books* books);
 
+/* what a batch holds, for launch shapes: the widest plane (blocks) sizes the
+ * entropy kernels' DC-row staging in LDS; kernels for slice kinds the batch
+ * lacks are not launched */
+#define TCG_HAS_RANS2 1u
+#define TCG_HAS_BITS 2u
+#define TCG_HAS_ALPHA 4u
+typedef struct tcg_batch_info {
+    uint32_t max_cols;
+    uint32_t kinds;      /* TCG_HAS_* */
+} tcg_batch_info;
+
 /* ---- encode ---- */
 int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
                    uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
                    const tcg_eframe* frames_dev, uint64_t qbuf, uint64_t dcmbuf,
                    uint32_t max_plane_tiles, uint64_t scratch, tcg_eresult* results_dev,
-                   tcg_eframe_out* frame_out_dev);
+                   tcg_eframe_out* frame_out_dev, const tcg_batch_info* info);
 
 /* ---- decode ---- */
 /* rowrec_dev: nplanes x max_rows records; max_plane_tiles / max_db_tiles:
@@ -204,7 +215,7 @@ int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const
 int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
                    uint32_t nslices, const tcg_dslice* slices_dev, const tcg_rowrec* rowrec_dev,
                    uint32_t max_rows, uint64_t qbuf, uint32_t max_plane_tiles, int32_t* slice_status_dev,
-                   uint32_t max_db_tiles);
+                   uint32_t max_db_tiles, const tcg_batch_info* info);
 
 /* per-kernel times in milliseconds (HIP events on the launch stream) of the
  * context's last launch sequence; valid after tcg_sync when timing is on */
