@@ -157,6 +157,14 @@ int32_t tcx_server_get_stats(tcx_server* s, tcx_server_stats* st);
 /* finishes queued and running work, then frees everything */
 void tcx_server_destroy(tcx_server* s);
 
+/* launch sequences through captured HIP graphs: each distinct batch shape is
+ * captured once per engine context and replayed with one graph launch (on by
+ * default; TOPOS_HIP_GRAPHS=0 in the environment turns it off at start-up).
+ * Per-kernel timing (tcx_set_timing) runs the sequences eagerly. */
+void tcx_set_graphs(int on);
+/* graph replays and captures so far, summed over the engine contexts */
+void tcx_graph_stats(uint64_t* replays, uint64_t* captures);
+
 /* per-kernel times of the last finished encode / decode launch sequence (ms), measured
  * with HIP events on the launch stream when enabled: fdct, slice_enc, scan,
  * emit, slice_dec, idct, deblock */

@@ -58,6 +58,10 @@ def load_library(path=LIB_PATH):
     lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
+    lib.tcx_set_graphs.argtypes = [ctypes.c_int]
+    lib.tcx_set_graphs.restype = None
+    lib.tcx_graph_stats.argtypes = [vp, vp]
+    lib.tcx_graph_stats.restype = None
     lib.tcx_now_ns.restype = ctypes.c_int64
     lib.tcx_server_create.argtypes = [vp, vp]
     lib.tcx_server_submit.argtypes = [vp, vp, vp]
@@ -342,6 +346,15 @@ class DeviceCodec(ToposCodec):
 
     def set_timing(self, on):
         self._lib.tcx_set_timing(1 if on else 0)
+
+    def set_graphs(self, on):
+        self._lib.tcx_set_graphs(1 if on else 0)
+
+    def graph_stats(self):
+        """(replays, captures) of launch-sequence graphs so far."""
+        h, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        self._lib.tcx_graph_stats(ctypes.byref(h), ctypes.byref(b))
+        return h.value, b.value
 
     def timing(self):
         t = (ctypes.c_float * 7)()

@@ -186,6 +186,22 @@ size_t tc_engine_encode_cap(const topos_frame_config* cfg)
 
 tcg_ctx* tc_engine_stream(tc_ectx* c) { return c->g; }
 
+void tc_engine_graph_stats(uint64_t* hits, uint64_t* builds)
+{
+    uint64_t h = 0, b = 0;
+    pthread_mutex_lock(&g_mu);
+    const uint32_t n = g_nctx;
+    pthread_mutex_unlock(&g_mu);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t a = 0, c = 0;
+        tcg_graph_stats(g_ctx[i].g, &a, &c);
+        h += a;
+        b += c;
+    }
+    *hits = h;
+    *builds = b;
+}
+
 void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec)
 {
     pthread_mutex_lock(&g_tmu);

@@ -279,3 +279,13 @@ void tcx_get_timing(float out[7])
     out[0] = e.fdct_ms; out[1] = e.slice_enc_ms; out[2] = e.scan_ms; out[3] = e.emit_ms;
     out[4] = d.slice_dec_ms; out[5] = d.idct_ms; out[6] = d.deblock_ms;
 }
+
+void tcx_set_graphs(int on) { tcg_set_graphs(on); }
+
+void tcx_graph_stats(uint64_t* replays, uint64_t* captures)
+{
+    uint64_t h = 0, b = 0;
+    tc_engine_graph_stats(&h, &b);
+    if (replays) { *replays = h; }
+    if (captures) { *captures = b; }
+}

@@ -96,6 +96,8 @@ void tc_engine_release(tc_ectx* c);
  * config): frame header plus every slice's header and entropy bound */
 size_t tc_engine_encode_cap(const topos_frame_config* cfg);
 tcg_ctx* tc_engine_stream(tc_ectx* c);
+/* graph replays / captures summed over the contexts */
+void tc_engine_graph_stats(uint64_t* hits, uint64_t* builds);
 /* per-kernel times of the last finished encode and decode (timing on) */
 void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec);
 

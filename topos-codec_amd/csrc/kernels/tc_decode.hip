@@ -529,28 +529,35 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 2;
-    tcg_mark(c, 0);
     const size_t dc_lds = (size_t)info->max_cols * sizeof(int32_t);
-    if (info->kinds & TCG_HAS_RANS2) {
-        hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), dc_lds, st, K, planes_dev, slices_dev,
-                           (int32_t*)qbuf, slice_status_dev);
-    }
-    if (info->kinds & TCG_HAS_BITS) {
-        tcg_bits_decode_launch(st, K, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev,
-                               info->max_cols);
-    }
-    if (info->kinds & TCG_HAS_ALPHA) {
-        tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
-    }
-    tcg_mark(c, 1);
-    if (max_plane_tiles > 0u) {
-        hipLaunchKernelGGL(k_dequant_idct, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev, planes_dev,
-                           rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
-    }
-    tcg_mark(c, 2);
-    if (max_db_tiles > 0u) {
-        hipLaunchKernelGGL(k_deblock, dim3(max_db_tiles, nplanes), dim3(256), 0, st, planes_dev);
-    }
-    tcg_mark(c, 3);
-    return tcg_check("decode launch");
+    auto launch = [&]() {
+        tcg_mark(c, 0);
+        if (info->kinds & TCG_HAS_RANS2) {
+            hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), dc_lds, st, K, planes_dev, slices_dev,
+                               (int32_t*)qbuf, slice_status_dev);
+        }
+        if (info->kinds & TCG_HAS_BITS) {
+            tcg_bits_decode_launch(st, K, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev,
+                                   info->max_cols);
+        }
+        if (info->kinds & TCG_HAS_ALPHA) {
+            tcg_alpha_decode_launch(st, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev);
+        }
+        tcg_mark(c, 1);
+        if (max_plane_tiles > 0u) {
+            hipLaunchKernelGGL(k_dequant_idct, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev,
+                               planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
+        }
+        tcg_mark(c, 2);
+        if (max_db_tiles > 0u) {
+            hipLaunchKernelGGL(k_deblock, dim3(max_db_tiles, nplanes), dim3(256), 0, st, planes_dev);
+        }
+        tcg_mark(c, 3);
+    };
+    const uint64_t key[TCG_GRAPH_KEY] = {
+        2u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,
+        (uint64_t)(uintptr_t)slices_dev, (uint64_t)(uintptr_t)rowrec_dev, max_rows, qbuf, max_plane_tiles,
+        (uint64_t)(uintptr_t)slice_status_dev, max_db_tiles, info->max_cols, info->kinds, (uint64_t)(uintptr_t)K,
+        0u, 0u, 0u, 0u, 0u };
+    return tcg_run(c, key, launch);
 }

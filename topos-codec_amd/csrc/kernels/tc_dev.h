@@ -75,16 +75,64 @@ struct TcgConsts {
 };
 
 // host-side runtime services (tc_runtime.hip)
+#define TCG_GRAPH_KEY 20
+#define TCG_GRAPHS_PER_CTX 8
+struct tcg_graph_ent {
+    uint64_t key[TCG_GRAPH_KEY];
+    hipGraphExec_t exec;
+    uint64_t used;      // last use (LRU eviction)
+};
 struct tcg_ctx {
     hipStream_t stream;
     hipEvent_t ev[8];
     int nmarks;     // events recorded by the last launch sequence (0 = timing off)
     int kind;       // 1 encode, 2 decode
+    tcg_graph_ent graphs[TCG_GRAPHS_PER_CTX];
+    int ngraphs;
+    uint64_t clock;
+    uint64_t graph_hits, graph_builds;
 };
 const TcgConsts* tcg_consts_dev();
 int32_t tcg_check(const char* what);
 // records event i of the launch sequence when timing is on
 void tcg_mark(tcg_ctx* c, int i);
+// 1 when launch sequences go through captured graphs (graphs on, timing off)
+int tcg_use_graphs();
+// the cached executable graph for key, or null
+hipGraphExec_t tcg_graph_find(tcg_ctx* c, const uint64_t* key);
+// stores a graph captured for key (evicting the least recently used one)
+int32_t tcg_graph_store(tcg_ctx* c, const uint64_t* key, hipGraph_t g, hipGraphExec_t* out);
+
+// Runs a launch sequence on the context's stream. With graphs in use the
+// sequence is captured once per distinct key - every pointer, count and
+// launch size its kernels receive - into an executable graph and replayed
+// with one hipGraphLaunch from then on; a context keeps its
+// TCG_GRAPHS_PER_CTX most recently used graphs. Captured sequences contain
+// kernel nodes only (no memset or copy nodes).
+template <typename F>
+int32_t tcg_run(tcg_ctx* c, const uint64_t* key, F&& launch)
+{
+    if (!tcg_use_graphs()) {
+        launch();
+        return tcg_check("launch");
+    }
+    hipGraphExec_t exec = tcg_graph_find(c, key);
+    if (exec == nullptr) {
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) { return tcg_check("hipStreamBeginCapture"); }
+        launch();
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess || g == nullptr) { return tcg_check("hipStreamEndCapture"); }
+        const int32_t rc = tcg_graph_store(c, key, g, &exec);
+        if (rc != 0) { return rc; }
+        c->graph_builds++;
+    } else {
+        c->graph_hits++;
+    }
+    if (hipGraphLaunch(exec, c->stream) != hipSuccess) { return tcg_check("hipGraphLaunch"); }
+    return tcg_check("graph launch");
+}
 
 // f64 cost accumulation in the reference's order with no contraction: the
 // model choice compares these sums with strict '<', so every rounding must

@@ -753,32 +753,38 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 1;
-    tcg_mark(c, 0);
-    if (max_plane_tiles > 0u) {
-        hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0,
-                           st, qctx_dev, planes_dev,
-                           (int32_t*)qbuf);
-    }
-    tcg_mark(c, 1);
-    if (info->kinds & TCG_HAS_RANS2) {
-        const size_t words = max((size_t)H_TOTAL + max((size_t)info->max_cols, (size_t)ENC_MODEL_WORDS),
+    const size_t enc_words = max((size_t)H_TOTAL + max((size_t)info->max_cols, (size_t)ENC_MODEL_WORDS),
                                  (size_t)3 * 11 * 64);
-        hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), words * 4u, st, K, planes_dev, slices_dev,
-                           (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
-    }
-    if (info->kinds & TCG_HAS_BITS) {
-        tcg_bits_encode_launch(st, K, planes_dev, slices_dev, nslices, (const int32_t*)qbuf, (uint32_t*)dcmbuf,
-                               (uint8_t*)scratch, results_dev, info->max_cols);
-    }
-    if (info->kinds & TCG_HAS_ALPHA) {
-        tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch, results_dev);
-    }
-    tcg_mark(c, 2);
-    hipLaunchKernelGGL(k_frame_scan, dim3(nframes), dim3(256), 0, st, K, frames_dev, results_dev,
-                       slices_dev, planes_dev, frame_out_dev);
-    tcg_mark(c, 3);
-    hipLaunchKernelGGL(k_slice_emit, dim3(nslices), dim3(256), 0, st, K, frames_dev, slices_dev,
-                       (const uint8_t*)scratch, results_dev, frame_out_dev);
-    tcg_mark(c, 4);
-    return tcg_check("encode launch");
+    auto launch = [&]() {
+        tcg_mark(c, 0);
+        if (max_plane_tiles > 0u) {
+            hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0, st, qctx_dev,
+                               planes_dev, (int32_t*)qbuf);
+        }
+        tcg_mark(c, 1);
+        if (info->kinds & TCG_HAS_RANS2) {
+            hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), enc_words * 4u, st, K, planes_dev,
+                               slices_dev, (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
+        }
+        if (info->kinds & TCG_HAS_BITS) {
+            tcg_bits_encode_launch(st, K, planes_dev, slices_dev, nslices, (const int32_t*)qbuf,
+                                   (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev, info->max_cols);
+        }
+        if (info->kinds & TCG_HAS_ALPHA) {
+            tcg_alpha_encode_launch(st, planes_dev, slices_dev, nslices, nullptr, (uint8_t*)scratch, results_dev);
+        }
+        tcg_mark(c, 2);
+        hipLaunchKernelGGL(k_frame_scan, dim3(nframes), dim3(256), 0, st, K, frames_dev, results_dev,
+                           slices_dev, planes_dev, frame_out_dev);
+        tcg_mark(c, 3);
+        hipLaunchKernelGGL(k_slice_emit, dim3(nslices), dim3(256), 0, st, K, frames_dev, slices_dev,
+                           (const uint8_t*)scratch, results_dev, frame_out_dev);
+        tcg_mark(c, 4);
+    };
+    const uint64_t key[TCG_GRAPH_KEY] = {
+        1u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,
+        (uint64_t)(uintptr_t)slices_dev, nframes, (uint64_t)(uintptr_t)frames_dev, qbuf, dcmbuf, max_plane_tiles,
+        scratch, (uint64_t)(uintptr_t)results_dev, (uint64_t)(uintptr_t)frame_out_dev, info->max_cols, info->kinds,
+        (uint64_t)(uintptr_t)K, 0u, 0u, 0u };
+    return tcg_run(c, key, launch);
 }

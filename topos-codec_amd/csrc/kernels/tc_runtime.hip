@@ -17,6 +17,7 @@ namespace {
 int g_ready = 0;
 TcgConsts* g_consts = nullptr;
 int g_timing = 0;
+int g_graphs = 1;
 thread_local char g_err[256];
 
 int32_t fail(const char* what, hipError_t e)
@@ -32,6 +33,42 @@ int32_t tcg_check(const char* what)
 {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) { return fail(what, e); }
+    return 0;
+}
+
+int tcg_use_graphs() { return g_graphs && !g_timing; }
+
+hipGraphExec_t tcg_graph_find(tcg_ctx* c, const uint64_t* key)
+{
+    for (int i = 0; i < c->ngraphs; ++i) {
+        if (memcmp(c->graphs[i].key, key, sizeof(c->graphs[i].key)) == 0) {
+            c->graphs[i].used = ++c->clock;
+            return c->graphs[i].exec;
+        }
+    }
+    return nullptr;
+}
+
+int32_t tcg_graph_store(tcg_ctx* c, const uint64_t* key, hipGraph_t g, hipGraphExec_t* out)
+{
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { return fail("hipGraphInstantiate", e); }
+    int slot = c->ngraphs;
+    if (slot == TCG_GRAPHS_PER_CTX) {
+        slot = 0;
+        for (int i = 1; i < c->ngraphs; ++i) {
+            if (c->graphs[i].used < c->graphs[slot].used) { slot = i; }
+        }
+        (void)hipGraphExecDestroy(c->graphs[slot].exec);
+    } else {
+        c->ngraphs++;
+    }
+    memcpy(c->graphs[slot].key, key, sizeof(c->graphs[slot].key));
+    c->graphs[slot].exec = exec;
+    c->graphs[slot].used = ++c->clock;
+    *out = exec;
     return 0;
 }
 
@@ -63,6 +100,8 @@ int32_t tcg_init(void)
                  e != hipSuccess ? hipGetErrorString(e) : "count 0");
         return -16;
     }
+    const char* gv = getenv("TOPOS_HIP_GRAPHS");
+    if (gv != nullptr && gv[0] == '0') { g_graphs = 0; }
     e = hipSetDevice(0);
     if (e != hipSuccess) { return fail("hipSetDevice", e); }
     e = hipMalloc((void**)&g_consts, sizeof(TcgConsts));
@@ -100,6 +139,7 @@ void tcg_ctx_destroy(tcg_ctx* c)
 {
     if (c == nullptr) { return; }
     (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < c->ngraphs; ++i) { (void)hipGraphExecDestroy(c->graphs[i].exec); }
     for (int i = 0; i < 8; ++i) { (void)hipEventDestroy(c->ev[i]); }
     (void)hipStreamDestroy(c->stream);
     free(c);
@@ -216,6 +256,14 @@ int32_t tcg_set_vlc(const tcg_vlc_books* books)
 }
 
 void tcg_set_timing(int on) { g_timing = on; }
+
+void tcg_set_graphs(int on) { g_graphs = on; }
+
+void tcg_graph_stats(tcg_ctx* c, uint64_t* hits, uint64_t* builds)
+{
+    *hits = c->graph_hits;
+    *builds = c->graph_builds;
+}
 
 int tcg_timing_on(void) { return g_timing; }
 

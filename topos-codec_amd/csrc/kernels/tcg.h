@@ -224,6 +224,10 @@ typedef struct tcg_timing {
     float slice_dec_ms, idct_ms, deblock_ms;
 } tcg_timing;
 void tcg_set_timing(int on);
+/* launch sequences through captured HIP graphs (default on; the environment
+ * variable TOPOS_HIP_GRAPHS=0 turns them off at start-up) */
+void tcg_set_graphs(int on);
+void tcg_graph_stats(tcg_ctx* c, uint64_t* hits, uint64_t* builds);
 int tcg_timing_on(void);
 void tcg_ctx_timing(tcg_ctx* c, tcg_timing* t);
 
