@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--max-batch", type=int, default=32, help="serving loop: most frames per launch sequence")
     ap.add_argument("--lanes", type=int, default=2, help="serving loop: launch sequences in flight")
     ap.add_argument("--load-seconds", type=float, default=4.0, help="length of each Poisson load run")
+    ap.add_argument("--no-tune", action="store_true", help="skip the entropy launch-shape autotuning")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -279,6 +280,21 @@ def main():
                                       dec=(host_pkts, dec_set, out_addr))
         assert st == 0, (er, dr, codec.last_error())
         return er[0]
+
+    # autotune the persistent entropy launch shape on one full serving batch (untimed)
+    tune = None
+    if not a.no_tune:
+        nt = a.max_batch
+        tune_out = [codec.alloc(pkt_cap) for _ in range(nt)]
+        st, rep = codec.autotune_batch([cfg] * nt, [in_addr[i % nf] for i in range(nt)], tune_out)
+        assert st == 0, codec.last_error()
+        tune = {"batch_frames": nt, "encode": {"waves_per_workgroup": rep.enc_wpg, "persistent_grid": bool(rep.enc_persistent),
+                                               "ms": round(rep.enc_ms_best, 3), "default_ms": round(rep.enc_ms_default, 3)},
+                "decode": {"waves_per_workgroup": rep.dec_wpg, "persistent_grid": bool(rep.dec_persistent),
+                           "ms": round(rep.dec_ms_best, 3), "default_ms": round(rep.dec_ms_default, 3)}}
+        for b in tune_out:
+            b.close()
+        print(f"# autotune ({nt}-frame batch): {tune}", file=sys.stderr, flush=True)
 
     # served-batched: the serving loop over requests on resident frames and packets
     srv = pkg.Server(codec, a.hbm_budget << 30, max_batch=a.max_batch, max_inflight=a.lanes)
@@ -468,6 +484,7 @@ def main():
                      "idct_achieved": round(samples * 6 / (idct_ms * 1e-3) / 1e9, 1) if idct_ms > 0 else None,
                      "deblock_achieved": round(samples * 4 / (db_ms * 1e-3) / 1e9, 1) if db_ms > 0 else None},
         "cpu_baseline": cpu,
+        "autotune": tune,
         "bitexact": bitexact,
     }
     print(json.dumps(out), flush=True)

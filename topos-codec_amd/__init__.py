@@ -58,6 +58,11 @@ def load_library(path=LIB_PATH):
     lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
+    lib.tcx_autotune_batch.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_uint32, vp]
+    lib.tcx_tune_force.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    lib.tcx_tune_force.restype = None
+    lib.tcx_tune_reset.argtypes = []
+    lib.tcx_tune_reset.restype = None
     lib.tcx_set_graphs.argtypes = [ctypes.c_int]
     lib.tcx_set_graphs.restype = None
     lib.tcx_graph_stats.argtypes = [vp, vp]
@@ -403,6 +408,22 @@ class DeviceCodec(ToposCodec):
         st = int(self._lib.tcx_decode_batch_dev(a["n"], a["hp"], a["sizes"], a["dp"], a["planes"], a["strides"],
                                                 a["infos"], a["rcs"]))
         return st, list(a["rcs"]), a["infos"]
+
+    def autotune_batch(self, cfgs, plane_addrs, out_bufs, reps=2, strides=None):
+        """Times the persistent entropy launch shapes on this batch and keeps the
+        fastest; returns (status, TcxTuneReport)."""
+        a = self._enc_args(cfgs, plane_addrs, out_bufs, strides)
+        rep = abi.TcxTuneReport()
+        st = int(self._lib.tcx_autotune_batch(a["cfgs"], a["n"], a["planes"], a["strides"], a["outs"], a["caps"],
+                                              reps, ctypes.byref(rep)))
+        return st, rep
+
+    def tune_reset(self):
+        self._lib.tcx_tune_reset()
+
+    def tune_force(self, wpg, groups=0):
+        """Fixes the calling thread's rANS2 launch shape (wpg 0: back to the tuned shapes)."""
+        self._lib.tcx_tune_force(wpg, groups)
 
     def serve_step(self, enc=None, dec=None):
         """One serving step: enc = (cfgs, plane_addrs, out_bufs[, strides]) and

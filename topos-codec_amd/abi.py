@@ -306,3 +306,10 @@ class TcxServerStats(ctypes.Structure):
         ("budget_bytes", ctypes.c_uint64), ("held_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
         ("queued", ctypes.c_uint32), ("inflight", ctypes.c_uint32),
     ]
+
+
+class TcxTuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_uint32), ("enc_wpg", ctypes.c_uint32), ("enc_persistent", ctypes.c_uint32),
+                ("dec_wpg", ctypes.c_uint32), ("dec_persistent", ctypes.c_uint32),
+                ("enc_ms_best", ctypes.c_float), ("enc_ms_default", ctypes.c_float),
+                ("dec_ms_best", ctypes.c_float), ("dec_ms_default", ctypes.c_float)]

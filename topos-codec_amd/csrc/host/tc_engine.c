@@ -79,6 +79,33 @@ static int32_t host_need(tc_ectx* c, uint8_t** p, size_t* cap, size_t bytes)
     return TC_OK;
 }
 
+/* work list of the persistent rANS2 launch: slice indices by descending
+ * block count (longest first), ties in batch order */
+typedef struct work_key { uint32_t nblk, idx; } work_key;
+
+static int work_key_cmp(const void* a, const void* b)
+{
+    const work_key* x = (const work_key*)a;
+    const work_key* y = (const work_key*)b;
+    if (x->nblk != y->nblk) { return x->nblk > y->nblk ? -1 : 1; }
+    return x->idx < y->idx ? -1 : (x->idx > y->idx ? 1 : 0);
+}
+
+static int32_t work_list(work_key* keys, uint32_t n, tcg_work* w)
+{
+    qsort(keys, n, sizeof(work_key), work_key_cmp);
+    uint32_t* order = (uint32_t*)(w + 1);
+    w->count = n;
+    for (uint32_t i = 0; i < n; ++i) { order[i] = keys[i].idx; }
+    return TC_OK;
+}
+
+/* launch shape of the persistent rANS2 kernels for a batch */
+static void entropy_shape(tcg_batch_info* info, int encode, uint32_t nframes)
+{
+    tc_tune_entropy(encode, info->max_cols, info->ent_count, nframes, &info->ent_wpg, &info->ent_groups);
+}
+
 /* colour slice entropy coding of a frame generation */
 static uint32_t slice_mode(const tc_fhdr* fh)
 {
@@ -271,7 +298,8 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
     const size_t o_pl = al256(o_q + (size_t)nqctx * sizeof(tcg_qctx));
     const size_t o_sl = al256(o_pl + (size_t)nplanes * sizeof(tcg_eplane));
     const size_t o_fr = al256(o_sl + (size_t)nslices * sizeof(tcg_eslice));
-    const size_t desc_bytes = al256(o_fr + (size_t)n * sizeof(tcg_eframe));
+    const size_t o_wk = al256(o_fr + (size_t)n * sizeof(tcg_eframe));
+    const size_t desc_bytes = al256(o_wk + sizeof(tcg_work) + (size_t)nslices * sizeof(uint32_t));
     const size_t o_res = 0;
     const size_t o_fo = al256((size_t)nslices * sizeof(tcg_eresult));
     const size_t res_bytes = al256(o_fo + (size_t)n * sizeof(tcg_eframe_out));
@@ -298,7 +326,11 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
     tcg_eframe* hf = (tcg_eframe*)(c->hdesc + o_fr);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0;
     size_t in_off = 0, scr_off = 0, pkt_off = 0;
-    tcg_batch_info info = { 1u, 0u };
+    tcg_batch_info info;
+    memset(&info, 0, sizeof(info));
+    info.max_cols = 1u;
+    work_key* wkeys = (work_key*)malloc(sizeof(work_key) * (nslices ? nslices : 1u));
+    if (wkeys == NULL) { free(plan); return tc_fail(TC_ERR_OUT_OF_MEMORY, "work list"); }
     for (uint32_t f = 0; f < n && rc == TC_OK; ++f) {
         tc_enc_req* r = &reqs[f];
         enc_frame_plan* P = &plan[f];
@@ -352,6 +384,11 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
                 S->mode = slice_mode(fh);
                 S->qpd = tc_is_alpha(fh, p) ? 64u : (uint32_t)(64 + P->qp_eff[p] - (int32_t)fh->qp_base);
                 S->hdr_plane = p;
+                if (!tc_is_alpha(fh, p) && S->mode == TCG_MODE_RANS2) {
+                    wkeys[info.ent_count].nblk = S->nblk;
+                    wkeys[info.ent_count].idx = is - 1u;
+                    info.ent_count++;
+                }
                 scr_off += al256(S->scratch_cap);
                 frame_cap += TC_SLICE_HDR_SIZE + S->scratch_cap;
             }
@@ -373,7 +410,11 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
         tmp.packet_size = 0u;
         tc_fhdr_pack(&tmp, F->hdr);
     }
-    if (rc != TC_OK) { free(plan); return rc; }
+    if (rc != TC_OK) { free(wkeys); free(plan); return rc; }
+    (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
+    free(wkeys);
+    info.work = (tcg_work*)(uintptr_t)(c->desc.p + o_wk);
+    entropy_shape(&info, 1, n);
     if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
@@ -488,7 +529,8 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     const size_t o_pl = al256(o_q + (size_t)nslices * sizeof(tcg_qctx));
     const size_t o_sl = al256(o_pl + (size_t)nplanes * sizeof(tcg_dplane));
     const size_t o_rs = al256(o_sl + (size_t)nslices * sizeof(tcg_dslice));
-    const size_t desc_bytes = al256(o_rs + (size_t)nplanes * maxrows * sizeof(tcg_rowrec));
+    const size_t o_wk = al256(o_rs + (size_t)nplanes * maxrows * sizeof(tcg_rowrec));
+    const size_t desc_bytes = al256(o_wk + sizeof(tcg_work) + (size_t)nslices * sizeof(uint32_t));
     const size_t res_bytes = al256((size_t)nslices * sizeof(int32_t));
     int32_t rc;
     if ((rc = dbuf_need(c, &c->q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
@@ -509,9 +551,13 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     tcg_rowrec* hrs = (tcg_rowrec*)(c->hdesc + o_rs);
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0;
     size_t pkt_off = 0, out_off = 0, tmp_off = 0;
-    tcg_batch_info info = { 1u, 0u };
+    tcg_batch_info info;
+    memset(&info, 0, sizeof(info));
+    info.max_cols = 1u;
+    work_key* wkeys = (work_key*)malloc(sizeof(work_key) * (nslices ? nslices : 1u));
+    if (wkeys == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "work list"); }
     uint64_t* out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
-    if (out_dev == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode plan"); }
+    if (out_dev == NULL) { free(wkeys); return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode plan"); }
     for (uint32_t f = 0; f < n; ++f) {
         tc_dec_req* r = &reqs[f];
         const tc_view* v = r->view;
@@ -520,6 +566,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         if (pk == 0u) {
             pk = c->pkt.p + pkt_off;
             if (tcg_h2d(c->g, pk, r->data, r->size) != 0) {
+                free(wkeys);
                 free(out_dev);
                 return tc_fail(TC_ERR_OUT_OF_MEMORY, "packet upload: %s", tcg_last_error());
             }
@@ -590,6 +637,11 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             S->blk0 = D->blk0 + (uint32_t)h->y0 * fh->bcols[p];
             S->k1 = h->k1; S->k2 = h->k2; S->k3 = h->k3;
             S->mode = slice_mode(fh);
+            if (!tc_is_alpha(fh, p) && S->mode == TCG_MODE_RANS2) {
+                wkeys[info.ent_count].nblk = (uint32_t)h->h * fh->bcols[p];
+                wkeys[info.ent_count].idx = is;
+                info.ent_count++;
+            }
             S->dc_max = fh->bit_depth > 12u ? 0x0FFFFFFFu : 0x08000000u;
             S->lv_max = fh->bit_depth > 12u ? 0x07FFFFFFu : 0x04000000u;
             if (tc_is_alpha(fh, p)) {
@@ -628,6 +680,10 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         }
     }
     (void)ndb;
+    (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
+    free(wkeys);
+    info.work = (tcg_work*)(uintptr_t)(c->desc.p + o_wk);
+    entropy_shape(&info, 0, n);
     if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());

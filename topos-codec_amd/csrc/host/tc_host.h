@@ -96,6 +96,10 @@ void tc_engine_release(tc_ectx* c);
  * config): frame header plus every slice's header and entropy bound */
 size_t tc_engine_encode_cap(const topos_frame_config* cfg);
 tcg_ctx* tc_engine_stream(tc_ectx* c);
+/* launch shape of a persistent rANS2 launch (tc_tune.c): waves per
+ * workgroup and workgroups (0 = one wave per slice) for a batch whose widest
+ * colour plane is cols blocks and which holds count rANS2 slices */
+void tc_tune_entropy(int encode, uint32_t cols, uint32_t count, uint32_t nframes, uint32_t* wpg, uint32_t* groups);
 /* graph replays / captures summed over the contexts */
 void tc_engine_graph_stats(uint64_t* hits, uint64_t* builds);
 /* per-kernel times of the last finished encode and decode (timing on) */

@@ -212,29 +212,28 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, con
     return 0;
 }
 
-__global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict__ K,
-                                                      const tcg_dplane* __restrict__ planes,
-                                                      const tcg_dslice* __restrict__ slices,
-                                                      int32_t* __restrict__ qbuf,
-                                                      int32_t* __restrict__ status)
-{
-    __shared__ uint32_t sTab[256];
-    __shared__ uint32_t sX2n[32];
-    __shared__ uint16_t sFreq[11][64];
-    __shared__ uint16_t sCum[11][65];
-    __shared__ int32_t sBad[1];
-    // DC values of the block row above, one per block column (dynamic: the
-    // batch's widest plane)
-    extern __shared__ int32_t sDcRow[];
+// LDS words of one decoding wave: model frequencies (11 x 64 u16) and
+// cumulative frequencies (11 x 65 u16), a failure flag, then the DC values
+// of the block row above (one per block column of the batch's widest plane)
+#define DEC_FIXED_WORDS 712u
 
-    const uint32_t lane = threadIdx.x;
-    const tcg_dslice S = slices[blockIdx.x];
+__host__ __device__ static inline uint32_t dec_wave_words(uint32_t cols) { return (DEC_FIXED_WORDS + cols + 1u) & ~1u; }
+
+__device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const tcg_dplane* __restrict__ planes,
+                                        const tcg_dslice* __restrict__ slices, int32_t* __restrict__ qbuf,
+                                        int32_t* __restrict__ status, uint32_t si, uint32_t lane,
+                                        const uint32_t* sTab, const uint32_t* sX2n, uint32_t* sW)
+{
+    uint16_t (*const sFreq)[64] = (uint16_t (*)[64])sW;
+    uint16_t (*const sCum)[65] = (uint16_t (*)[65])(sW + 352);
+    int32_t* const sBad = (int32_t*)(sW + 710);
+    int32_t* const sDcRow = (int32_t*)(sW + DEC_FIXED_WORDS);
+
+    const tcg_dslice S = slices[si];
     if (S.kind != 0u || S.mode != TCG_MODE_RANS2) { return; }
     const tcg_dplane P = planes[S.plane];
-    for (uint32_t i = lane; i < 256u; i += 64u) { sTab[i] = K->crc[i]; }
-    if (lane < 32u) { sX2n[lane] = K->x2n[lane]; }
     if (lane == 0u) { sBad[0] = 0; }
-    __syncthreads();
+    wave_sync();
 
     const uint8_t* pl = (const uint8_t*)S.payload;
     const uint32_t n = S.size;
@@ -264,7 +263,7 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
             for (uint32_t s = 0; s < nsym; ++s) { rb[s] = pl[off + s]; }
             if (tc_model_from_bytes(rb, nsym, sFreq[lane], sCum[lane]) != 0) { atomicOr(&sBad[0], 1); }
         }
-        __syncthreads();
+        wave_sync();
         if (wuni((uint32_t)sBad[0]) != 0u) { rc = -9; }
     }
     if (rc == 0) {
@@ -300,7 +299,33 @@ __global__ __launch_bounds__(64) void k_slice_decode(const TcgConsts* __restrict
             if (rc == 0 && D.loaded - D.rc != 8u * (n - prefix - 4u)) { rc = -9; }
         }
     }
-    if (lane == 0u) { status[blockIdx.x] = rc == 0 ? 0 : 1; }
+    if (lane == 0u) { status[si] = rc == 0 ? 0 : 1; }
+}
+
+// Persistent launch over the batch's rANS2 colour slices (longest first):
+// wave k of the grid decodes entries k, k + W, ... (W waves in the grid).
+// The CRC tables are shared by the workgroup's waves.
+__global__ __launch_bounds__(256) void k_slice_decode(const TcgConsts* __restrict__ K,
+                                                       const tcg_dplane* __restrict__ planes,
+                                                       const tcg_dslice* __restrict__ slices,
+                                                       int32_t* __restrict__ qbuf,
+                                                       int32_t* __restrict__ status,
+                                                       const tcg_work* __restrict__ work, uint32_t wave_words)
+{
+    __shared__ uint32_t sTab[256];
+    __shared__ uint32_t sX2n[32];
+    extern __shared__ uint32_t sWaves[];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x) { sTab[i] = K->crc[i]; }
+    if (threadIdx.x < 32u) { sX2n[threadIdx.x] = K->x2n[threadIdx.x]; }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const WaveSlot ws = wave_slot();
+    uint32_t* const mine = sWaves + ws.wave * wave_words;
+    const uint32_t* const order = tcg_work_order(work);
+    const uint32_t count = work->count;
+    for (uint32_t i = ws.first; i < count; i += ws.stride) {
+        slice_decode_one(K, planes, slices, qbuf, status, order[i], lane, sTab, sX2n, mine);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -529,12 +554,14 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 2;
-    const size_t dc_lds = (size_t)info->max_cols * sizeof(int32_t);
+    const uint32_t wpg = info->ent_wpg ? info->ent_wpg : 1u;
+    const uint32_t wave_words = dec_wave_words(info->max_cols);
+    const uint32_t groups = info->ent_groups ? info->ent_groups : (info->ent_count + wpg - 1u) / wpg;
     auto launch = [&]() {
         tcg_mark(c, 0);
-        if (info->kinds & TCG_HAS_RANS2) {
-            hipLaunchKernelGGL(k_slice_decode, dim3(nslices), dim3(64), dc_lds, st, K, planes_dev, slices_dev,
-                               (int32_t*)qbuf, slice_status_dev);
+        if ((info->kinds & TCG_HAS_RANS2) && groups > 0u) {
+            hipLaunchKernelGGL(k_slice_decode, dim3(groups), dim3(64u * wpg), (size_t)wave_words * 4u * wpg, st, K,
+                               planes_dev, slices_dev, (int32_t*)qbuf, slice_status_dev, info->work, wave_words);
         }
         if (info->kinds & TCG_HAS_BITS) {
             tcg_bits_decode_launch(st, K, planes_dev, slices_dev, nslices, (int32_t*)qbuf, slice_status_dev,
@@ -558,6 +585,18 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         2u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,
         (uint64_t)(uintptr_t)slices_dev, (uint64_t)(uintptr_t)rowrec_dev, max_rows, qbuf, max_plane_tiles,
         (uint64_t)(uintptr_t)slice_status_dev, max_db_tiles, info->max_cols, info->kinds, (uint64_t)(uintptr_t)K,
-        0u, 0u, 0u, 0u, 0u };
+        (uint64_t)wpg << 32 | groups, (uint64_t)(uintptr_t)info->work, 0u, 0u, 0u };
     return tcg_run(c, key, launch);
+}
+
+uint32_t tcg_entropy_capacity_dec(uint32_t wpg, uint32_t max_cols)
+{
+    const uint32_t wave_words = dec_wave_words(max_cols);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_slice_decode, (int)(64u * wpg),
+                                                     (size_t)wave_words * 4u * wpg) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0u;
+    }
+    return (uint32_t)per_cu * tcg_cu_count();
 }

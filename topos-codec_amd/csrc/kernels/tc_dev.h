@@ -98,6 +98,10 @@ int32_t tcg_check(const char* what);
 void tcg_mark(tcg_ctx* c, int i);
 // 1 when launch sequences go through captured graphs (graphs on, timing off)
 int tcg_use_graphs();
+// compute units of the device
+uint32_t tcg_cu_count();
+uint32_t tcg_entropy_capacity_enc(uint32_t wpg, uint32_t max_cols);
+uint32_t tcg_entropy_capacity_dec(uint32_t wpg, uint32_t max_cols);
 // the cached executable graph for key, or null
 hipGraphExec_t tcg_graph_find(tcg_ctx* c, const uint64_t* key);
 // stores a graph captured for key (evicting the least recently used one)
@@ -132,6 +136,37 @@ int32_t tcg_run(tcg_ctx* c, const uint64_t* key, F&& launch)
     }
     if (hipGraphLaunch(exec, c->stream) != hipSuccess) { return tcg_check("hipGraphLaunch"); }
     return tcg_check("graph launch");
+}
+
+// Phase boundary inside one wavefront's slice work: a workgroup-scope fence
+// (waits for the wave's outstanding LDS and memory operations and keeps the
+// compiler from moving memory accesses across). The waves of a persistent
+// workgroup work on different slices, so they never meet at s_barrier.
+__device__ static inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// This wave's first entry and stride in a persistent launch's work list:
+// wave k of the grid takes entries k, k + W, k + 2W, ... (W = waves in the
+// grid). The list is sorted longest first, so every wave gets a similar share
+// of long and short slices. Both values are wave-uniform (SGPRs): the loop
+// over entries is a scalar loop with a single exit.
+struct WaveSlot { uint32_t first, stride, wave; };
+__device__ static inline WaveSlot wave_slot()
+{
+    WaveSlot w;
+    w.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t wpg = blockDim.x >> 6;
+    w.first = blockIdx.x * wpg + w.wave;
+    w.stride = gridDim.x * wpg;
+    return w;
+}
+
+__device__ static inline const uint32_t* tcg_work_order(const tcg_work* w)
+{
+    return (const uint32_t*)(w + 1);
 }
 
 // f64 cost accumulation in the reference's order with no contraction: the

@@ -267,18 +267,26 @@ __device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn
 #define ENC_GROUP 8
 // words of the model tables: row bytes, frequencies and cumulative frequencies
 #define ENC_MODEL_WORDS ((NROWS * 64 + NROWS * 64 * 2 + NROWS * 65 * 2 + 3) / 4)
+// five f64 costs (one spare) and the slice prefix bytes
+#define ENC_FIXED_WORDS (12 + (TCG_PREFIX_MAX + 3) / 4)
 
-__global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict__ K,
-                                                      const tcg_eplane* __restrict__ planes,
-                                                      const tcg_eslice* __restrict__ slices,
-                                                      const int32_t* __restrict__ qbuf,
-                                                      uint32_t* __restrict__ dcmbuf,
-                                                      uint8_t* __restrict__ scratch,
-                                                      tcg_eresult* __restrict__ results)
+// LDS words of one encoding wave whose plane is `cols` blocks wide: the phase
+// space (histograms + DC row | model tables | encoder constants), then the
+// fixed part
+__host__ __device__ static inline uint32_t enc_phase_words(uint32_t cols)
 {
-    __shared__ double sCost[5];
-    __shared__ uint8_t sPrefix[TCG_PREFIX_MAX];
-    // dynamic LDS (words), reused across the kernel's phases:
+    const uint32_t a = H_TOTAL + (cols > ENC_MODEL_WORDS ? cols : ENC_MODEL_WORDS);
+    const uint32_t b = 3u * 11u * 64u;
+    return ((a > b ? a : b) + 1u) & ~1u;  // even: the costs that follow are f64
+}
+
+__device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const tcg_eplane* __restrict__ planes,
+                                        const tcg_eslice* __restrict__ slices, const int32_t* __restrict__ qbuf,
+                                        uint32_t* __restrict__ dcmbuf, uint8_t* __restrict__ scratch,
+                                        tcg_eresult* __restrict__ results, uint32_t si, uint32_t lane,
+                                        uint32_t* __restrict__ sDyn)
+{
+    // this wave's LDS (words), reused across the slice's phases:
     //  forward pass      [0, H_TOTAL) histograms, then the DC values of the
     //                    block row above (one per block column of the batch's
     //                    widest plane)
@@ -286,7 +294,8 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
     //                    cumulative frequencies over the dead DC row
     //  backward pass     [0, 3 x 11 x 64) per-symbol encoder constants, stored
     //                    from registers after the model tables' last read
-    extern __shared__ uint32_t sDyn[];
+    //  always            the five model costs and the slice prefix bytes, in
+    //                    the ENC_FIXED_WORDS after the phase space
     uint32_t* const sH = sDyn;
     int32_t* const sDcRow = (int32_t*)(sDyn + H_TOTAL);
     uint8_t (*const sRowB)[64] = (uint8_t (*)[64])(sDyn + H_TOTAL);
@@ -295,16 +304,16 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
     uint32_t* const sTabA = sDyn;
     uint32_t* const sTabB = sDyn + 11 * 64;
     uint32_t* const sTabX = sDyn + 2 * 11 * 64;
-
-    const uint32_t lane = threadIdx.x;
-    const tcg_eslice S = slices[blockIdx.x];
+    const tcg_eslice S = slices[si];
+    const uint32_t cols = planes[S.plane].bcols;
+    double* const sCost = (double*)(sDyn + enc_phase_words(cols));
+    uint8_t* const sPrefix = (uint8_t*)(sCost + 6);
     if (S.mode != TCG_MODE_RANS2) { return; }
     const tcg_eplane P = planes[S.plane];
     if (P.kind != 0u) { return; }
-    const uint32_t cols = P.bcols;
 
     for (uint32_t i = lane; i < H_TOTAL; i += 64u) { sH[i] = 0u; }
-    __syncthreads();
+    wave_sync();
 
     // ---------------- forward pass ----------------
     // One wavefront: LDS accesses of this wave complete in program order, so
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
 #pragma unroll
         for (uint32_t j = 0; j < ENC_GROUP; ++j) { qa[j] = qb[j]; }
     }
-    __syncthreads();
+    wave_sync();
     const uint32_t any_viol = __ballot(viol != 0u) != 0ull;
 
     // ---------------- model choice ----------------
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         tc_row_bytes(sRowB[lane], nsym, c);
         tc_model_from_bytes(sRowB[lane], nsym, sFreq[lane], sCum[lane]);
     }
-    __syncthreads();
+    wave_sync();
     // exact cost sums, one candidate per lane, in the reference's order:
     // lane 0 lvl-none, 1 lvl-pos, 2 lvl-prev, 3 dc-none, 4 dc-prev
     if (lane < 5u) {
@@ -405,7 +414,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         }
         sCost[lane] = cost_add_const(bits, 8.0 * (double)(nr * nsym));
     }
-    __syncthreads();
+    wave_sync();
     uint32_t lvl_model = 0u, dc_model = 0u;
     {
         double best = sCost[0];
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
                 tc_rans_enc_consts(sFreq[row][lane], sCum[row][lane], &tA[slot], &tB[slot], &tX[slot]);
             }
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (uint32_t slot = 0; slot < 11u; ++slot) {
             if (slot < nslot) {
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
             }
         }
     }
-    __syncthreads();
+    wave_sync();
 
     // ---------------- backward pass ----------------
     // Per block, lanes gather their pair's table entries (level and run) and
@@ -561,7 +570,32 @@ __global__ __launch_bounds__(64) void k_slice_encode(const TcgConsts* __restrict
         R.size = ok_len ? n + 4u + prefix_len : 0u;
         R.status = any_viol ? -1 : (ok_len ? 0 : -15);
         R.k1 = 0; R.k2 = 0; R.k3 = 0; R.crc = 0; R.offset = 0; R.aux = 0; R.pad_ = 0;
-        results[blockIdx.x] = R;
+        results[si] = R;
+    }
+}
+
+// Persistent launch: the batch's rANS2 colour slices sit in a work list,
+// longest first; wave k of the grid encodes entries k, k + W, ... (W waves in
+// the grid), so a grid smaller than the list makes every wave work through
+// several slices. A workgroup holds `wpg` waves, each with its own LDS of
+// `wave_words` words.
+__global__ __launch_bounds__(256) void k_slice_encode(const TcgConsts* __restrict__ K,
+                                                       const tcg_eplane* __restrict__ planes,
+                                                       const tcg_eslice* __restrict__ slices,
+                                                       const int32_t* __restrict__ qbuf,
+                                                       uint32_t* __restrict__ dcmbuf,
+                                                       uint8_t* __restrict__ scratch,
+                                                       tcg_eresult* __restrict__ results,
+                                                       const tcg_work* __restrict__ work, uint32_t wave_words)
+{
+    extern __shared__ uint32_t sWaves[];
+    const uint32_t lane = threadIdx.x & 63u;
+    const WaveSlot ws = wave_slot();
+    uint32_t* const mine = sWaves + ws.wave * wave_words;
+    const uint32_t* const order = tcg_work_order(work);
+    const uint32_t count = work->count;
+    for (uint32_t i = ws.first; i < count; i += ws.stride) {
+        slice_encode_one(K, planes, slices, qbuf, dcmbuf, scratch, results, order[i], lane, mine);
     }
 }
 
@@ -753,8 +787,9 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     hipStream_t st = c->stream;
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 1;
-    const size_t enc_words = max((size_t)H_TOTAL + max((size_t)info->max_cols, (size_t)ENC_MODEL_WORDS),
-                                 (size_t)3 * 11 * 64);
+    const uint32_t wpg = info->ent_wpg ? info->ent_wpg : 1u;
+    const uint32_t wave_words = enc_phase_words(info->max_cols) + ENC_FIXED_WORDS;
+    const uint32_t groups = info->ent_groups ? info->ent_groups : (info->ent_count + wpg - 1u) / wpg;
     auto launch = [&]() {
         tcg_mark(c, 0);
         if (max_plane_tiles > 0u) {
@@ -762,9 +797,10 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
                                planes_dev, (int32_t*)qbuf);
         }
         tcg_mark(c, 1);
-        if (info->kinds & TCG_HAS_RANS2) {
-            hipLaunchKernelGGL(k_slice_encode, dim3(nslices), dim3(64), enc_words * 4u, st, K, planes_dev,
-                               slices_dev, (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch, results_dev);
+        if ((info->kinds & TCG_HAS_RANS2) && groups > 0u) {
+            hipLaunchKernelGGL(k_slice_encode, dim3(groups), dim3(64u * wpg), (size_t)wave_words * 4u * wpg, st, K,
+                               planes_dev, slices_dev, (const int32_t*)qbuf, (uint32_t*)dcmbuf, (uint8_t*)scratch,
+                               results_dev, info->work, wave_words);
         }
         if (info->kinds & TCG_HAS_BITS) {
             tcg_bits_encode_launch(st, K, planes_dev, slices_dev, nslices, (const int32_t*)qbuf,
@@ -785,6 +821,18 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         1u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,
         (uint64_t)(uintptr_t)slices_dev, nframes, (uint64_t)(uintptr_t)frames_dev, qbuf, dcmbuf, max_plane_tiles,
         scratch, (uint64_t)(uintptr_t)results_dev, (uint64_t)(uintptr_t)frame_out_dev, info->max_cols, info->kinds,
-        (uint64_t)(uintptr_t)K, 0u, 0u, 0u };
+        (uint64_t)(uintptr_t)K, (uint64_t)wpg << 32 | groups, (uint64_t)(uintptr_t)info->work, 0u };
     return tcg_run(c, key, launch);
+}
+
+uint32_t tcg_entropy_capacity_enc(uint32_t wpg, uint32_t max_cols)
+{
+    const uint32_t wave_words = enc_phase_words(max_cols) + ENC_FIXED_WORDS;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_slice_encode, (int)(64u * wpg),
+                                                     (size_t)wave_words * 4u * wpg) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0u;
+    }
+    return (uint32_t)per_cu * tcg_cu_count();
 }

@@ -17,6 +17,7 @@ namespace {
 int g_ready = 0;
 TcgConsts* g_consts = nullptr;
 int g_timing = 0;
+uint32_t g_cus = 0;
 int g_graphs = 1;
 thread_local char g_err[256];
 
@@ -37,6 +38,8 @@ int32_t tcg_check(const char* what)
 }
 
 int tcg_use_graphs() { return g_graphs && !g_timing; }
+
+uint32_t tcg_cu_count() { return g_cus; }
 
 hipGraphExec_t tcg_graph_find(tcg_ctx* c, const uint64_t* key)
 {
@@ -104,6 +107,12 @@ int32_t tcg_init(void)
     if (gv != nullptr && gv[0] == '0') { g_graphs = 0; }
     e = hipSetDevice(0);
     if (e != hipSuccess) { return fail("hipSetDevice", e); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) {
+        g_cus = (uint32_t)prop.multiProcessorCount;
+    } else {
+        g_cus = 1u;
+    }
     e = hipMalloc((void**)&g_consts, sizeof(TcgConsts));
     if (e != hipSuccess) { return fail("hipMalloc consts", e); }
     g_ready = 1;
@@ -284,3 +293,8 @@ void tcg_ctx_timing(tcg_ctx* c, tcg_timing* t)
 }
 
 }  // extern "C"
+
+extern "C" uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols)
+{
+    return encode ? tcg_entropy_capacity_enc(wpg, max_cols) : tcg_entropy_capacity_dec(wpg, max_cols);
+}

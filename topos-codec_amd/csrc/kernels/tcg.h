@@ -191,6 +191,13 @@ int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uin
 /* upload the V2 VLC books */
 int32_t tcg_set_vlc(const tcg_vlc_books* books);
 
+/* work list of a persistent entropy launch: the entry count, then `count`
+ * slice indices, longest slice first (uploaded with the descriptors) */
+typedef struct tcg_work {
+    uint32_t count;
+    uint32_t pad_[3];
+} tcg_work;
+
 /* what a batch holds, for launch shapes: the widest plane (blocks) sizes the
  * entropy kernels' DC-row staging in LDS; kernels for slice kinds the batch
  * lacks are not launched */
@@ -200,7 +207,18 @@ int32_t tcg_set_vlc(const tcg_vlc_books* books);
 typedef struct tcg_batch_info {
     uint32_t max_cols;
     uint32_t kinds;      /* TCG_HAS_* */
+    /* persistent rANS2 entropy launch: waves per workgroup (1, 2 or 4) and
+     * workgroups (0: enough for one slice per wave) */
+    uint32_t ent_wpg;
+    uint32_t ent_groups;
+    uint32_t ent_count;  /* rANS2 colour slices in the work list */
+    tcg_work* work;      /* device work list */
 } tcg_batch_info;
+
+/* workgroups of a persistent rANS2 launch that fit on the device at once
+ * (all compute units), for a batch whose widest plane is max_cols blocks */
+uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols);
+
 
 /* ---- encode ---- */
 int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
