@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes moved by one k_fdct_quant launch over the 8-frame batch, from two
 # rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same
 # counters read for a plain 256 MiB device copy (fetch x2.0, write x1.0).
-FDCT_TRAFFIC_BYTES = 796_840_000
+FDCT_TRAFFIC_BYTES = 796_632_857
 
 
 def load_pkg():
@@ -189,6 +189,24 @@ def server_poisson(pkg, srv, enc_req, dec_req, nf, sizes, rate_fps, seconds, see
     return lat, n / 2 / elapsed
 
 
+def tune_batch(codec, cfg, in_addr, nf, nt, pkt_cap):
+    """Autotunes the launch shapes on an nt-frame batch of the resident frames (untimed)."""
+    tune_out = [codec.alloc(pkt_cap) for _ in range(nt)]
+    st, rep = codec.autotune_batch([cfg] * nt, [in_addr[i % nf] for i in range(nt)], tune_out)
+    assert st == 0, codec.last_error()
+    for b in tune_out:
+        b.close()
+    return {"batch_frames": nt,
+            "encode": {"waves_per_workgroup": rep.enc_wpg, "persistent_grid": bool(rep.enc_persistent),
+                       "ms": round(rep.enc_ms_best, 3), "default_ms": round(rep.enc_ms_default, 3)},
+            "decode": {"waves_per_workgroup": rep.dec_wpg, "persistent_grid": bool(rep.dec_persistent),
+                       "ms": round(rep.dec_ms_best, 3), "default_ms": round(rep.dec_ms_default, 3)},
+            "fdct": {"tiles_per_workgroup": rep.fdct_tpw, "ms": round(rep.fdct_ms_best, 4),
+                     "default_ms": round(rep.fdct_ms_default, 4)},
+            "idct": {"tiles_per_workgroup": rep.idct_tpw, "ms": round(rep.idct_ms_best, 4),
+                     "default_ms": round(rep.idct_ms_default, 4)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,17 +302,9 @@ def main():
     # autotune the persistent entropy launch shape on one full serving batch (untimed)
     tune = None
     if not a.no_tune:
-        nt = a.max_batch
-        tune_out = [codec.alloc(pkt_cap) for _ in range(nt)]
-        st, rep = codec.autotune_batch([cfg] * nt, [in_addr[i % nf] for i in range(nt)], tune_out)
-        assert st == 0, codec.last_error()
-        tune = {"batch_frames": nt, "encode": {"waves_per_workgroup": rep.enc_wpg, "persistent_grid": bool(rep.enc_persistent),
-                                               "ms": round(rep.enc_ms_best, 3), "default_ms": round(rep.enc_ms_default, 3)},
-                "decode": {"waves_per_workgroup": rep.dec_wpg, "persistent_grid": bool(rep.dec_persistent),
-                           "ms": round(rep.dec_ms_best, 3), "default_ms": round(rep.dec_ms_default, 3)}}
-        for b in tune_out:
-            b.close()
-        print(f"# autotune ({nt}-frame batch): {tune}", file=sys.stderr, flush=True)
+        tune = [tune_batch(codec, cfg, in_addr, nf, nt, pkt_cap) for nt in sorted({a.max_batch, nf})]
+        for t in tune:
+            print(f"# autotune ({t['batch_frames']}-frame batch): {t}", file=sys.stderr, flush=True)
 
     # served-batched: the serving loop over requests on resident frames and packets
     srv = pkg.Server(codec, a.hbm_budget << 30, max_batch=a.max_batch, max_inflight=a.lanes)

@@ -158,26 +158,32 @@ int32_t tcx_server_get_stats(tcx_server* s, tcx_server_stats* st);
 void tcx_server_destroy(tcx_server* s);
 
 /* ---- autotuner ----------------------------------------------------------
- * The persistent rANS2 entropy launches have two knobs that never change the
- * output: waves (slices) per workgroup, 1, 2 or 4, and the grid, one wave per
- * slice or only the workgroups that fit on the device at once. Their best
- * values depend on the resolution and profile mix; tcx_autotune_batch times
- * all six candidates on a representative batch (arguments as for
- * tcx_encode_batch_dev; the packets it writes are the batch's packets) and
- * keeps the fastest per (direction, widest plane, slice count class) for the
- * launches that follow. TOPOS_TUNE_FILE names a file the table is loaded
+ * Launch knobs that never change the output: for the persistent rANS2
+ * entropy launches, waves (slices) per workgroup, 1, 2 or 4, and the grid,
+ * one wave per slice or only the workgroups that fit on the device at once;
+ * for the transform kernels, tiles of 32 blocks per workgroup, 1, 2 or 4.
+ * Their best values depend on the resolution and profile mix;
+ * tcx_autotune_batch times the candidates on a representative batch
+ * (arguments as for tcx_encode_batch_dev; the packets it writes are the
+ * batch's packets) and keeps the fastest per (kernel, widest plane, slice or
+ * tile count class) for the launches that follow. TOPOS_TUNE_FILE names a file the table is loaded
  * from at start-up and saved to after each run. Per-kernel timing is on
  * (graphs off) for the duration of the run. */
 typedef struct tcx_tune_report {
     uint32_t candidates;
     uint32_t enc_wpg, enc_persistent, dec_wpg, dec_persistent;  /* chosen shapes */
     float enc_ms_best, enc_ms_default, dec_ms_best, dec_ms_default;  /* entropy kernel times */
+    uint32_t fdct_tpw, idct_tpw;  /* chosen transform tiles per workgroup */
+    float fdct_ms_best, fdct_ms_default, idct_ms_best, idct_ms_default;
 } tcx_tune_report;
 int32_t tcx_autotune_batch(const topos_frame_config* cfgs, uint32_t n, const uint64_t* dev_planes,
                            const size_t* dev_strides, const uint64_t* dev_out, const size_t* out_caps, uint32_t reps,
                            tcx_tune_report* report);
 /* forgets every tuned shape (launches go back to one wave per slice) */
 void tcx_tune_reset(void);
+/* fixes the transform tiling of the calling thread's launches: tpw tiles of
+ * 32 blocks per workgroup (1, 2 or 4; 0 returns to the tuned tiling) */
+void tcx_tune_force_tiles(uint32_t tpw);
 /* fixes the rANS2 launch shape of the calling thread's launches: wpg waves
  * per workgroup (1, 2 or 4) and groups workgroups (0: one wave per slice);
  * wpg 0 returns to the tuned shapes */

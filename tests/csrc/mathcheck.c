@@ -68,16 +68,21 @@ uint32_t mc_enc_update_check(uint32_t per_f)
 
 uint32_t mc_zigzag_pos(uint32_t n) { return tc_zigzag_pos(n); }
 
-/* the fp64 quantiser divide against integer division: every Q in qs[0..nq),
- * numerators around every multiple boundary near the top of the range
- * (n < 2^31), small numerators and pseudo-random ones. Returns mismatches. */
-uint32_t mc_qdiv_f64_check(const uint32_t* qs, uint32_t nq, uint32_t per_q)
+/* the quantiser's 32-bit multiply-high divide against integer division:
+ * every Q in qs[0..nq), numerators around every multiple boundary near the
+ * top of the range (n < 2^31), small numerators and pseudo-random ones, and
+ * the constants' ranges (m < 2^32 by construction of the 64-bit division
+ * below, l <= 31). Returns mismatches. */
+uint32_t mc_qdiv32_check(const uint32_t* qs, uint32_t nq, uint32_t per_q)
 {
     uint32_t bad = 0u;
     uint64_t s = 0x2545F4914F6CDD1Dull;
     for (uint32_t i = 0; i < nq; ++i) {
         const uint32_t Q = qs[i];
-        const double r = 1.0 / (double)Q;
+        uint32_t m, l;
+        tc_qdiv32_consts(Q, &m, &l);
+        const uint64_t m64 = ((((uint64_t)1 << (31u + l)) + Q - 1u) / Q);
+        if (m64 > 0xFFFFFFFFull || (uint64_t)m != m64 || l > 31u) { ++bad; continue; }
         const uint32_t top = 0x7FFFFFFFu;
         const uint32_t kmax = top / Q;
         for (uint32_t j = 0; j < per_q + 16u; ++j) {
@@ -89,10 +94,19 @@ uint32_t mc_qdiv_f64_check(const uint32_t* qs, uint32_t nq, uint32_t per_q)
                 const int64_t nn = (int64_t)(k * Q) + d;
                 if (nn < 0 || nn > (int64_t)top) { continue; }
                 const uint32_t n = (uint32_t)nn;
-                if (tc_qdiv_f64(n, r) != n / Q) { ++bad; }
+                if (tc_qdiv32(n, m, l) != n / Q) { ++bad; }
             }
         }
     }
+    return bad;
+}
+
+/* exhaustive: every n < 2^31 for one divisor Q (returns mismatches) */
+uint32_t mc_qdiv32_all(uint32_t Q)
+{
+    uint32_t m, l, bad = 0u;
+    tc_qdiv32_consts(Q, &m, &l);
+    for (uint32_t n = 0; n < 0x80000000u; ++n) { bad += tc_qdiv32(n, m, l) != n / Q; }
     return bad;
 }
 

@@ -85,9 +85,40 @@ def test_autotune_keeps_output(abi, codec, mix):
     assert rep.candidates == 6
     assert rep.enc_wpg in (1, 2, 4) and rep.dec_wpg in (1, 2, 4)
     assert 0 < rep.enc_ms_best <= rep.enc_ms_default and 0 < rep.dec_ms_best <= rep.dec_ms_default
+    assert rep.fdct_tpw in (1, 2, 4) and rep.idct_tpw in (1, 2, 4)
+    assert 0 < rep.fdct_ms_best <= rep.fdct_ms_default and 0 < rep.idct_ms_best <= rep.idct_ms_default
     for i, m in enumerate(mix):
         assert outs[i].download(len(m["pkt"])).tobytes() == m["pkt"]
     # the tuned shape is used from now on
     pkts, decs = _roundtrip(abi, codec, mix)
     assert pkts == [m["pkt"] for m in mix]
     codec.tune_reset()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tpw", [1, 2, 4])
+def test_transform_tilings_match_oracle(abi, codec, oracle, mix, tpw):
+    codec.tune_force_tiles(tpw)
+    try:
+        pkts, decs = _roundtrip(abi, codec, mix)
+        # a damaged slice: concealed tiles inside multi-tile workgroups
+        m = mix[0]
+        pkt = bytearray(m["pkt"])
+        count = int.from_bytes(pkt[30:32], "big")
+        off = 53
+        for k in range(count):
+            size = int.from_bytes(pkt[off:off + 4], "big")
+            if k == 3:
+                pkt[off + 17 + size // 2] ^= 0x3C
+            off += 17 + size
+        pkt = bytes(pkt)
+        rc, planes, info = codec.decode(pkt, strict=False)
+    finally:
+        codec.tune_force_tiles(0)
+    for i, mm in enumerate(mix):
+        assert pkts[i] == mm["pkt"], f"frame {i} packet, {tpw} tiles per workgroup"
+        assert all((a == b).all() for a, b in zip(decs[i], mm["dec"])), f"frame {i} decode, {tpw} tiles"
+    o_rc, o_planes, o_status, o_nconc = orc_decode(oracle, pkt, _dims(abi, m["cfg"]))
+    assert rc == o_rc == abi.TC_WARN_CONCEALED
+    assert list(info.slice_status[:count]) == o_status[:count]
+    assert all((a == b).all() for a, b in zip(planes, o_planes))

@@ -28,7 +28,9 @@ def mc(tmp_path_factory):
     lib.mc_gf2_mulmod.restype = ctypes.c_uint32
     lib.mc_enc_update_check.restype = ctypes.c_uint32
     lib.mc_zigzag_pos.restype = ctypes.c_uint32
-    lib.mc_qdiv_f64_check.restype = ctypes.c_uint32
+    lib.mc_qdiv32_check.restype = ctypes.c_uint32
+    lib.mc_qdiv32_all.restype = ctypes.c_uint32
+    lib.mc_qdiv32_all.argtypes = [ctypes.c_uint32]
     lib.mc_ctx_tables_check.restype = ctypes.c_uint32
     lib.mc_dct8_check.restype = ctypes.c_uint32
     lib.mc_dct8_check.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int64]
@@ -88,17 +90,28 @@ def test_zigzag_closed_form(mc):
     assert [mc.mc_zigzag_pos(n) for n in range(64)] == inv
 
 
-def test_quant_division_f64_exact(mc, oracle):
+def test_quant_division_mulhi_exact(mc, oracle):
     # every step Q = max(1, (qm * qp_scale + 128) >> 8) the quantiser tables can
     # produce (5 matrices x 2 classes x 64 positions x 2 qp tables x 96 qp)
     qm = np.ctypeslib.as_array((ctypes.c_uint16 * (5 * 2 * 64)).in_dll(oracle, "orc_qm")).astype(np.int64)
     sc = np.ctypeslib.as_array((ctypes.c_uint32 * (2 * 96)).in_dll(oracle, "orc_qp_scale")).astype(np.int64)
     qs = np.unique(np.maximum(1, (qm[:, None] * sc[None, :] + 128) >> 8)).astype(np.uint32)
-    extra = np.array([1, 2, 3, 7, (1 << 21) - 1, 1 << 21, (1 << 24) + 1, (1 << 25) - 3], np.uint32)
+    extra = np.array([1, 2, 3, 7, (1 << 21) - 1, 1 << 21, (1 << 24) + 1, (1 << 25) - 3, (1 << 31) - 1, 1 << 31],
+                     np.uint32)
     qs = np.unique(np.concatenate([qs, extra]))
     assert qs.max() > (1 << 24)
     arr = (ctypes.c_uint32 * len(qs))(*qs.tolist())
-    assert mc.mc_qdiv_f64_check(arr, len(qs), 64) == 0
+    assert mc.mc_qdiv32_check(arr, len(qs), 64) == 0
+    # the transform kernel adds the bias Q/2 - dz (dz = Q/4 on AC) without
+    # saturating: it is never negative for any step the tables produce
+    q64 = qs.astype(np.int64)
+    assert ((q64 >> 1) - q64 // 4 >= 0).all()
+
+
+@pytest.mark.parametrize("q", [3, 1448, 27555])
+def test_quant_division_mulhi_exhaustive(mc, q):
+    # all 2^31 numerators for a few divisors (a few seconds each in C)
+    assert mc.mc_qdiv32_all(q) == 0
 
 
 def test_quant_division_exact(mc):

@@ -59,6 +59,8 @@ def load_library(path=LIB_PATH):
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
     lib.tcx_autotune_batch.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_uint32, vp]
+    lib.tcx_tune_force_tiles.argtypes = [ctypes.c_uint32]
+    lib.tcx_tune_force_tiles.restype = None
     lib.tcx_tune_force.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
     lib.tcx_tune_force.restype = None
     lib.tcx_tune_reset.argtypes = []
@@ -420,6 +422,10 @@ class DeviceCodec(ToposCodec):
 
     def tune_reset(self):
         self._lib.tcx_tune_reset()
+
+    def tune_force_tiles(self, tpw):
+        """Fixes the calling thread's transform tiles per workgroup (0: back to the tuned tiling)."""
+        self._lib.tcx_tune_force_tiles(tpw)
 
     def tune_force(self, wpg, groups=0):
         """Fixes the calling thread's rANS2 launch shape (wpg 0: back to the tuned shapes)."""

@@ -312,4 +312,7 @@ class TcxTuneReport(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_uint32), ("enc_wpg", ctypes.c_uint32), ("enc_persistent", ctypes.c_uint32),
                 ("dec_wpg", ctypes.c_uint32), ("dec_persistent", ctypes.c_uint32),
                 ("enc_ms_best", ctypes.c_float), ("enc_ms_default", ctypes.c_float),
-                ("dec_ms_best", ctypes.c_float), ("dec_ms_default", ctypes.c_float)]
+                ("dec_ms_best", ctypes.c_float), ("dec_ms_default", ctypes.c_float),
+                ("fdct_tpw", ctypes.c_uint32), ("idct_tpw", ctypes.c_uint32),
+                ("fdct_ms_best", ctypes.c_float), ("fdct_ms_default", ctypes.c_float),
+                ("idct_ms_best", ctypes.c_float), ("idct_ms_default", ctypes.c_float)]

@@ -23,6 +23,7 @@ HOST_C = ["tc_status.c", "tc_format.c", "tc_vlc.c", "tc_engine.c", "tc_api.c", "
 HIP_SRC = ["tc_runtime.hip", "tc_encode.hip", "tc_decode.hip", "tc_alpha.hip", "tc_bits.hip"]
 
 CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
+          "-Werror=implicit-function-declaration",
           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "host")]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -100,10 +100,12 @@ static int32_t work_list(work_key* keys, uint32_t n, tcg_work* w)
     return TC_OK;
 }
 
-/* launch shape of the persistent rANS2 kernels for a batch */
-static void entropy_shape(tcg_batch_info* info, int encode, uint32_t nframes)
+/* launch shapes of the persistent rANS2 kernels and the transform kernels
+ * for a batch (tiles: the batch's 32-block transform tiles) */
+static void entropy_shape(tcg_batch_info* info, int encode, uint32_t nframes, uint32_t tiles)
 {
     tc_tune_entropy(encode, info->max_cols, info->ent_count, nframes, &info->ent_wpg, &info->ent_groups);
+    info->tile_tpw = tc_tune_tiles(encode, info->max_cols, tiles);
 }
 
 /* colour slice entropy coding of a frame generation */
@@ -414,7 +416,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
     (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
     free(wkeys);
     info.work = (tcg_work*)(uintptr_t)(c->desc.p + o_wk);
-    entropy_shape(&info, 1, n);
+    entropy_shape(&info, 1, n, tile);
     if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(plan);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
@@ -683,7 +685,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
     free(wkeys);
     info.work = (tcg_work*)(uintptr_t)(c->desc.p + o_wk);
-    entropy_shape(&info, 0, n);
+    entropy_shape(&info, 0, n, tile);
     if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());

@@ -14,6 +14,7 @@
  *   quantiser context   native/src/transform/quant.c:217-308
  */
 #include "tc_host.h"
+#include "../kernels/tc_math.h"
 
 #include <string.h>
 
@@ -686,7 +687,7 @@ void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bd, ui
         q->Q[i] = Q;
         q->half[i] = Q >> 1;
         q->dz[i] = i == 0 ? 0u : Q / 4u;
-        q->rq[i] = 1.0 / (double)Q;
+        tc_qdiv32_consts(Q, &q->qmul[i], &q->qsh[i]);
     }
     q->bit_depth = bd;
     q->mid = 1 << (bd - 1u);

@@ -100,6 +100,9 @@ tcg_ctx* tc_engine_stream(tc_ectx* c);
  * workgroup and workgroups (0 = one wave per slice) for a batch whose widest
  * colour plane is cols blocks and which holds count rANS2 slices */
 void tc_tune_entropy(int encode, uint32_t cols, uint32_t count, uint32_t nframes, uint32_t* wpg, uint32_t* groups);
+/* transform tiles per workgroup (1, 2, 4) for a batch of `tiles` 32-block
+ * transform tiles */
+uint32_t tc_tune_tiles(int encode, uint32_t cols, uint32_t tiles);
 /* graph replays / captures summed over the contexts */
 void tc_engine_graph_stats(uint64_t* hits, uint64_t* builds);
 /* per-kernel times of the last finished encode and decode (timing on) */

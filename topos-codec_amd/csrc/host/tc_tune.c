@@ -40,6 +40,13 @@ static int g_loaded = 0;
 static __thread int t_force = 0;
 static __thread uint32_t t_wpg = 1u, t_mode = 0u, t_groups = 0u;
 static __thread uint32_t t_last_cols = 0u, t_last_count = 0u;
+static __thread uint32_t t_tpw = 0u;  /* forced transform tiles per workgroup (0: tuned) */
+static __thread uint32_t t_last_tiles = 0u;
+
+/* table kinds (the `encode` field): 0 decode entropy, 1 encode entropy,
+ * 2 FDCT tiles per workgroup, 3 IDCT tiles per workgroup */
+#define KIND_FDCT 2u
+#define KIND_IDCT 3u
 
 static uint32_t bucket_of(uint32_t n)
 {
@@ -59,7 +66,7 @@ static void load_locked(void)
     tune_ent e;
     memset(&e, 0, sizeof(e));
     while (g_n < TUNE_MAX && fscanf(f, "%u %u %u %u %u %f", &e.encode, &e.cols, &e.bucket, &e.wpg, &e.mode, &e.ms) == 6) {
-        if ((e.wpg == 1u || e.wpg == 2u || e.wpg == 4u) && e.mode <= 1u) { g_tab[g_n++] = e; }
+        if ((e.wpg == 1u || e.wpg == 2u || e.wpg == 4u) && e.mode <= 1u && e.encode <= KIND_IDCT) { g_tab[g_n++] = e; }
     }
     fclose(f);
 }
@@ -137,6 +144,26 @@ void tc_tune_entropy(int encode, uint32_t cols, uint32_t count, uint32_t nframes
         const uint32_t cap = tcg_entropy_capacity(encode, w, cols);
         if (cap > 0u && cap < full) { *groups = cap; }
     }
+}
+
+uint32_t tc_tune_tiles(int encode, uint32_t cols, uint32_t tiles)
+{
+    t_last_tiles = tiles;
+    if (t_tpw != 0u) { return t_tpw; }
+    uint32_t tpw = 1u;
+    const uint32_t kind = encode ? KIND_FDCT : KIND_IDCT;
+    pthread_mutex_lock(&g_mu);
+    load_locked();
+    const uint32_t b = bucket_of(tiles);
+    for (uint32_t i = 0; i < g_n; ++i) {
+        if (g_tab[i].encode == kind && g_tab[i].cols == cols && g_tab[i].bucket == b) {
+            tpw = g_tab[i].wpg;
+            g_tab[i].used = ++g_clock;
+            break;
+        }
+    }
+    pthread_mutex_unlock(&g_mu);
+    return tpw;
 }
 
 /* ------------------------------------------------------------ tuning run */
@@ -232,8 +259,51 @@ int32_t tcx_autotune_batch(const topos_frame_config* cfgs, uint32_t n, const uin
         }
         if (k == 0u || ms < best_d) { best_d = ms; bd = k; }
     }
+    /* transform tiles per workgroup, with the entropy shapes just chosen */
+    static const uint32_t k_tpw[3] = { 1u, 2u, 4u };
+    float best_f = 0.0f, best_i = 0.0f;
+    uint32_t bf = 0u, bi = 0u, fq_tiles = 0u, iq_tiles = 0u;
+    for (uint32_t k = 0; k < 3u && rc == TC_OK; ++k) {
+        float fms = 0.0f, ims = 0.0f;
+        for (uint32_t r = 0; r < reps && rc == TC_OK; ++r) {
+            t_tpw = k_tpw[k];
+            t_force = 1;
+            t_wpg = k_cand[be][0];
+            t_mode = k_cand[be][1];
+            rc = tcx_encode_batch_dev(cfgs, n, dev_planes, dev_strides, dev_out, out_caps, sizes, rcs, NULL);
+            fq_tiles = t_last_tiles;
+            tcg_timing te, td;
+            tc_engine_last_timing(&te, &td);
+            if (rc == TC_OK && memcmp(sizes0, sizes, sizeof(uint32_t) * n) != 0) {
+                rc = tc_fail(TC_ERR_STATE, "transform tiling %u changed the packets", k_tpw[k]);
+            }
+            fms = r == 0u || te.fdct_ms < fms ? te.fdct_ms : fms;
+            t_wpg = k_cand[bd][0];
+            t_mode = k_cand[bd][1];
+            if (rc == TC_OK) {
+                rc = tcx_decode_batch_dev(n, (const uint8_t* const*)host, hsz, dev_out, planes, NULL, infos, rcs);
+            }
+            iq_tiles = t_last_tiles;
+            tc_engine_last_timing(&te, &td);
+            ims = r == 0u || td.idct_ms < ims ? td.idct_ms : ims;
+            t_force = 0;
+            t_tpw = 0u;
+        }
+        if (rc != TC_OK) { break; }
+        if (k == 0u) { rep->fdct_ms_default = fms; rep->idct_ms_default = ims; }
+        if (k == 0u || fms < best_f) { best_f = fms; bf = k; }
+        if (k == 0u || ims < best_i) { best_i = ims; bi = k; }
+    }
+    t_force = 0;
+    t_tpw = 0u;
     tcg_set_timing(0);
     if (rc == TC_OK) {
+        record(KIND_FDCT, enc_cols, fq_tiles, k_tpw[bf], 0u, best_f);
+        record(KIND_IDCT, dec_cols, iq_tiles, k_tpw[bi], 0u, best_i);
+        rep->fdct_tpw = k_tpw[bf];
+        rep->idct_tpw = k_tpw[bi];
+        rep->fdct_ms_best = best_f;
+        rep->idct_ms_best = best_i;
         record(1u, enc_cols, enc_count, k_cand[be][0], k_cand[be][1], best_e);
         record(0u, dec_cols, dec_count, k_cand[bd][0], k_cand[bd][1], best_d);
         rep->enc_ms_best = best_e;
@@ -255,6 +325,11 @@ void tcx_tune_reset(void)
     g_n = 0;
     g_loaded = 1;
     pthread_mutex_unlock(&g_mu);
+}
+
+void tcx_tune_force_tiles(uint32_t tpw)
+{
+    t_tpw = tpw == 1u || tpw == 2u || tpw == 4u ? tpw : 0u;
 }
 
 void tcx_tune_force(uint32_t wpg, uint32_t groups)

@@ -342,10 +342,25 @@ __global__ __launch_bounds__(256) void k_slice_decode(const TcgConsts* __restric
 __device__ static inline void idct8(const double* in, double* out) { tc_idct8_f64(in, out); }
 __device__ static inline void idct8(const int64_t* in, int64_t* out) { tc_idct8_i64(in, out); }
 
+// coefficients of row u of block j of a tile (thread t = 8 j + u), loaded
+// ahead of the transform
+struct IqRow { int4 r0, r1; };
+
+__device__ static inline void iq_load(const tcg_dplane& P, uint32_t brow, uint32_t bx0, uint32_t nb,
+                                      const int32_t* __restrict__ qbuf, IqRow& q)
+{
+    const uint32_t j = threadIdx.x >> 3, u = threadIdx.x & 7u;
+    if (j < nb) {
+        const int32_t* src = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0 + j) * 64u + u * 8u;
+        q.r0 = *(const int4*)src;
+        q.r1 = *(const int4*)(src + 4);
+    }
+}
+
 template <bool WIDE>
 __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __restrict__ qc,
                                          uint32_t brow, uint32_t bx0, uint32_t nb, bool conceal,
-                                         const int32_t* __restrict__ qbuf, void* sBraw, uint32_t* sQ)
+                                         const IqRow& pre, void* sBraw, uint32_t* sQ, const double* sW)
 {
     typedef typename std::conditional<WIDE, int64_t, double>::type acc_t;
     acc_t* sB = (acc_t*)sBraw;
@@ -368,19 +383,25 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
     {
         const uint32_t j = t >> 3, u = t & 7u;
         if (j < nb) {
-            const int32_t* src = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0 + j) * 64u + u * 8u;
-            const int4 r0 = *(const int4*)src;
-            const int4 r1 = *(const int4*)(src + 4);
+            const int4 r0 = pre.r0, r1 = pre.r1;
             const int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
             acc_t G[8];
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
-                int64_t d = (int64_t)qv[v] * (int64_t)sQ[u * 8 + v];
-                d = d > fclamp ? fclamp : (d < -fclamp ? -fclamp : d);
                 if (WIDE) {
+                    int64_t d = (int64_t)qv[v] * (int64_t)sQ[u * 8 + v];
+                    d = d > fclamp ? fclamp : (d < -fclamp ? -fclamp : d);
                     d = d > (1 << 30) ? (1 << 30) : (d < -(1 << 30) ? -(1 << 30) : d);
+                    G[v] = (acc_t)d * (acc_t)kTW[u * 8 + v];
+                } else {
+                    // |q| < 2^27 and Q < 2^26, so q Q is an integer below 2^53 and
+                    // exact in f64; the clamp bounds it by 2^25 and the weight by
+                    // 2^14, so the product with the weight is exact as well
+                    const double fc = (double)fclamp;
+                    double d = (double)qv[v] * (double)sQ[u * 8 + v];
+                    d = fmin(fmax(d, -fc), fc);
+                    G[v] = (acc_t)(d * sW[u * 8 + v]);
                 }
-                G[v] = (acc_t)d * (acc_t)kTW[u * 8 + v];
             }
             acc_t Bv[8];
             idct8(G, Bv);
@@ -420,9 +441,11 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
     }
 }
 
-// grid: x = tile (32 blocks of a block row) within the plane, y = plane of the
-// batch. The row record (quantiser context, slice) loads beside the plane
-// descriptor; the context and the slice status then load together.
+// grid: x = group of TPW tiles (32 blocks of a block row each) within the
+// plane, y = plane of the batch. The coefficients of all TPW tiles are
+// loaded first (TPW x 8 KiB in flight per workgroup); each tile's row record
+// (quantiser context, slice) then selects its context and concealment.
+template <uint32_t TPW>
 __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict__ qctxs,
                                                        const tcg_dplane* __restrict__ planes,
                                                        const tcg_rowrec* __restrict__ rowrec,
@@ -432,22 +455,39 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
 {
     __shared__ uint64_t sB[IQ_BLOCKS * 65];
     __shared__ uint32_t sQ[64];
+    __shared__ double sW[64];
     const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (P.kind != 0u) { return; }
+    if (threadIdx.x < 64u) { sW[threadIdx.x] = (double)kTW[threadIdx.x]; }  // read after each tile's barrier
     const uint32_t tiles_per_row = (P.bcols + IQ_BLOCKS - 1u) / IQ_BLOCKS;
-    const uint32_t local = blockIdx.x;
-    if (local >= P.brows * tiles_per_row) { return; }
-    const uint32_t brow = local / tiles_per_row;
-    const uint32_t bx0 = (local % tiles_per_row) * IQ_BLOCKS;
-    const uint32_t nb = min((uint32_t)IQ_BLOCKS, P.bcols - bx0);
-    const tcg_rowrec rr = rowrec[(size_t)pi * max_rows + brow];
-    const tcg_qctx* qc = &qctxs[rr.qctx];
-    const bool conceal = status[rr.slice] != 0;
-    if (qc->bit_depth > 12u) {
-        dequant_idct_tile<true>(P, qc, brow, bx0, nb, conceal, qbuf, sB, sQ);
-    } else {
-        dequant_idct_tile<false>(P, qc, brow, bx0, nb, conceal, qbuf, sB, sQ);
+    const uint32_t ntiles = P.brows * tiles_per_row;
+    IqRow pre[TPW];
+#pragma unroll
+    for (uint32_t k = 0; k < TPW; ++k) {
+        const uint32_t local = blockIdx.x * TPW + k;
+        if (local < ntiles) {
+            const uint32_t brow = local / tiles_per_row;
+            const uint32_t bx0 = (local % tiles_per_row) * IQ_BLOCKS;
+            iq_load(P, brow, bx0, min((uint32_t)IQ_BLOCKS, P.bcols - bx0), qbuf, pre[k]);
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < TPW; ++k) {
+        const uint32_t local = blockIdx.x * TPW + k;
+        if (local >= ntiles) { break; }
+        const uint32_t brow = local / tiles_per_row;
+        const uint32_t bx0 = (local % tiles_per_row) * IQ_BLOCKS;
+        const uint32_t nb = min((uint32_t)IQ_BLOCKS, P.bcols - bx0);
+        const tcg_rowrec rr = rowrec[(size_t)pi * max_rows + brow];
+        const tcg_qctx* qc = &qctxs[rr.qctx];
+        const bool conceal = status[rr.slice] != 0;
+        if (k > 0u) { __syncthreads(); }  // the previous tile is done with sQ and sB
+        if (qc->bit_depth > 12u) {
+            dequant_idct_tile<true>(P, qc, brow, bx0, nb, conceal, pre[k], sB, sQ, sW);
+        } else {
+            dequant_idct_tile<false>(P, qc, brow, bx0, nb, conceal, pre[k], sB, sQ, sW);
+        }
     }
 }
 
@@ -555,6 +595,7 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 2;
     const uint32_t wpg = info->ent_wpg ? info->ent_wpg : 1u;
+    const uint32_t iq_tpw = info->tile_tpw ? info->tile_tpw : 1u;
     const uint32_t wave_words = dec_wave_words(info->max_cols);
     const uint32_t groups = info->ent_groups ? info->ent_groups : (info->ent_count + wpg - 1u) / wpg;
     auto launch = [&]() {
@@ -572,8 +613,16 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         }
         tcg_mark(c, 1);
         if (max_plane_tiles > 0u) {
-            hipLaunchKernelGGL(k_dequant_idct, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev,
-                               planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
+            if (iq_tpw == 4u) {
+                hipLaunchKernelGGL(k_dequant_idct<4>, dim3((max_plane_tiles + 3u) / 4u, nplanes), dim3(256), 0, st,
+                                   qctx_dev, planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
+            } else if (iq_tpw == 2u) {
+                hipLaunchKernelGGL(k_dequant_idct<2>, dim3((max_plane_tiles + 1u) / 2u, nplanes), dim3(256), 0, st,
+                                   qctx_dev, planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
+            } else {
+                hipLaunchKernelGGL(k_dequant_idct<1>, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev,
+                                   planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
+            }
         }
         tcg_mark(c, 2);
         if (max_db_tiles > 0u) {
@@ -585,7 +634,7 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         2u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,
         (uint64_t)(uintptr_t)slices_dev, (uint64_t)(uintptr_t)rowrec_dev, max_rows, qbuf, max_plane_tiles,
         (uint64_t)(uintptr_t)slice_status_dev, max_db_tiles, info->max_cols, info->kinds, (uint64_t)(uintptr_t)K,
-        (uint64_t)wpg << 32 | groups, (uint64_t)(uintptr_t)info->work, 0u, 0u, 0u };
+        (uint64_t)wpg << 32 | groups, (uint64_t)(uintptr_t)info->work, iq_tpw, 0u, 0u };
     return tcg_run(c, key, launch);
 }
 

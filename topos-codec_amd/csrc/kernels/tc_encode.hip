@@ -80,8 +80,8 @@ __device__ static inline void fq_load(const tcg_eplane& P, const FqTile& T, int3
 
 template <bool WIDE>
 __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int32_t fclamp, int32_t* x,
-                                       int32_t* __restrict__ qbuf, int32_t* sA, const double* sRq,
-                                       const uint32_t* sHalf, const uint32_t* sDz, const uint8_t* sZz)
+                                       int32_t* __restrict__ qbuf, int32_t* sA, const uint32_t* sQm,
+                                       const uint32_t* sQl, const uint32_t* sBias, const uint8_t* sZzc)
 {
     const uint32_t t = threadIdx.x;
     const uint32_t nb = T.nb;
@@ -116,17 +116,21 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int
             if (WIDE) { F = predistort(F, nat); }
             int32_t c = F > fclamp ? fclamp : (F < -fclamp ? -fclamp : F);
             const uint32_t mag = (uint32_t)(c < 0 ? -c : c);
-            const uint32_t num = mag + sHalf[nat];
-            const uint32_t ne = num > sDz[nat] ? num - sDz[nat] : 0u;
-            const uint32_t qm = tc_qdiv_f64(ne, sRq[nat]);
+            // Q/2 - dz >= 0 (dz = Q/4 for AC, 0 for DC), so |F| + Q/2 - dz never
+            // goes below zero and needs no saturation
+            const uint32_t ne = mag + sBias[nat];
+            const uint32_t qm = tc_qdiv32(ne, sQm[nat], sQl[nat]);
             qv[u] = c < 0 ? -(int32_t)qm : (int32_t)qm;
         }
     }
     __syncthreads();
     if (j < nb) {
+        // scan positions of this column's eight coefficients (one 8-byte read)
+        const uint2 zz2 = *(const uint2*)(sZzc + v * 8u);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            sA[j * FQ_PITCH + sZz[u * 8 + v]] = qv[u];
+            const uint32_t pos = ((u < 4 ? zz2.x : zz2.y) >> (8 * (u & 3))) & 0xFFu;
+            sA[j * FQ_PITCH + pos] = qv[u];
         }
     }
     __syncthreads();
@@ -145,39 +149,50 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int
     }
 }
 
-// grid: x = tile (FQ_BLOCKS blocks of a block row) within the plane, y = plane
-// of the batch; quantiser contexts are indexed by plane, so the context and
-// the plane descriptor load in parallel and no tile -> plane search is needed.
-// The zigzag table is staged in LDS with the quantiser context (a per-lane
-// LDS read is cheaper than a constant-memory gather).
+// grid: x = group of TPW tiles (FQ_BLOCKS blocks of a block row each) within
+// the plane, y = plane of the batch; quantiser contexts are indexed by plane,
+// so the context and the plane descriptor load in parallel and no tile ->
+// plane search is needed. The zigzag table is staged in LDS with the
+// quantiser context (a per-lane LDS read is cheaper than a constant-memory
+// gather). The samples of all TPW tiles are loaded before the first tile is
+// transformed, so a workgroup keeps TPW x 4 KiB of loads in flight.
+template <uint32_t TPW>
 __global__ __launch_bounds__(FQ_THREADS) void k_fdct_quant(const tcg_qctx* __restrict__ qctxs,
                                                      const tcg_eplane* __restrict__ planes,
                                                      int32_t* __restrict__ qbuf)
 {
     __shared__ int32_t sA[FQ_BLOCKS * FQ_PITCH];
-    __shared__ double sRq[64];
-    __shared__ uint32_t sHalf[64], sDz[64];
-    __shared__ uint8_t sZz[64];
+    __shared__ uint32_t sQm[64], sQl[64];
+    __shared__ uint32_t sBias[64];
+    __shared__ __align__(8) uint8_t sZzc[64];  // scan position of natural u * 8 + v at v * 8 + u
     const uint32_t pi = blockIdx.y, t = threadIdx.x;
     const tcg_qctx* qc = &qctxs[pi];
     if (t < 64u) {
-        sRq[t] = qc->rq[t];
-        sHalf[t] = qc->half[t];
-        sDz[t] = qc->dz[t];
-        sZz[t] = kZigzagInv[t];
+        sQm[t] = qc->qmul[t];
+        sQl[t] = qc->qsh[t];
+        sBias[t] = qc->half[t] - qc->dz[t];
+        sZzc[(t & 7u) * 8u + (t >> 3)] = kZigzagInv[t];
     }
     const uint32_t bd = qc->bit_depth;
     const int32_t mid = qc->mid, fclamp = qc->fclamp;
     const tcg_eplane P = planes[pi];
     if (P.kind != 0u) { return; }
-    const FqTile T = fq_tile(P, blockIdx.x);
-    if (!T.live) { return; }
-    int32_t x[8];
-    fq_load(P, T, mid, t, x);
-    if (bd > 12u) {
-        fdct_quant_tile<true>(P, T, fclamp, x, qbuf, sA, sRq, sHalf, sDz, sZz);
-    } else {
-        fdct_quant_tile<false>(P, T, fclamp, x, qbuf, sA, sRq, sHalf, sDz, sZz);
+    FqTile T[TPW];
+    int32_t x[TPW][8];
+#pragma unroll
+    for (uint32_t k = 0; k < TPW; ++k) {
+        T[k] = fq_tile(P, blockIdx.x * TPW + k);
+        fq_load(P, T[k], mid, t, x[k]);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < TPW; ++k) {
+        if (!T[k].live) { break; }
+        if (k > 0u) { __syncthreads(); }  // the previous tile's store has read sA
+        if (bd > 12u) {
+            fdct_quant_tile<true>(P, T[k], fclamp, x[k], qbuf, sA, sQm, sQl, sBias, sZzc);
+        } else {
+            fdct_quant_tile<false>(P, T[k], fclamp, x[k], qbuf, sA, sQm, sQl, sBias, sZzc);
+        }
     }
 }
 
@@ -788,13 +803,22 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
     const TcgConsts* K = tcg_consts_dev();
     c->kind = 1;
     const uint32_t wpg = info->ent_wpg ? info->ent_wpg : 1u;
+    const uint32_t fq_tpw = info->tile_tpw ? info->tile_tpw : 1u;
     const uint32_t wave_words = enc_phase_words(info->max_cols) + ENC_FIXED_WORDS;
     const uint32_t groups = info->ent_groups ? info->ent_groups : (info->ent_count + wpg - 1u) / wpg;
     auto launch = [&]() {
         tcg_mark(c, 0);
         if (max_plane_tiles > 0u) {
-            hipLaunchKernelGGL(k_fdct_quant, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0, st, qctx_dev,
-                               planes_dev, (int32_t*)qbuf);
+            if (fq_tpw == 4u) {
+                hipLaunchKernelGGL(k_fdct_quant<4>, dim3((max_plane_tiles + 3u) / 4u, nplanes), dim3(FQ_THREADS), 0,
+                                   st, qctx_dev, planes_dev, (int32_t*)qbuf);
+            } else if (fq_tpw == 2u) {
+                hipLaunchKernelGGL(k_fdct_quant<2>, dim3((max_plane_tiles + 1u) / 2u, nplanes), dim3(FQ_THREADS), 0,
+                                   st, qctx_dev, planes_dev, (int32_t*)qbuf);
+            } else {
+                hipLaunchKernelGGL(k_fdct_quant<1>, dim3(max_plane_tiles, nplanes), dim3(FQ_THREADS), 0, st,
+                                   qctx_dev, planes_dev, (int32_t*)qbuf);
+            }
         }
         tcg_mark(c, 1);
         if ((info->kinds & TCG_HAS_RANS2) && groups > 0u) {
@@ -821,7 +845,7 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         1u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,
         (uint64_t)(uintptr_t)slices_dev, nframes, (uint64_t)(uintptr_t)frames_dev, qbuf, dcmbuf, max_plane_tiles,
         scratch, (uint64_t)(uintptr_t)results_dev, (uint64_t)(uintptr_t)frame_out_dev, info->max_cols, info->kinds,
-        (uint64_t)(uintptr_t)K, (uint64_t)wpg << 32 | groups, (uint64_t)(uintptr_t)info->work, 0u };
+        (uint64_t)(uintptr_t)K, (uint64_t)wpg << 32 | groups, (uint64_t)(uintptr_t)info->work, fq_tpw };
     return tcg_run(c, key, launch);
 }
 

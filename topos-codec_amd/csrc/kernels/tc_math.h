@@ -87,18 +87,24 @@ TC_HD uint32_t tc_rans_enc_update(uint32_t x, uint32_t A, uint32_t B)
     return x + q * ((B >> 12) & 0xFFFu) + (B & 0xFFFu);
 }
 
-/* Exact floor(n / Q) for n < 2^31, Q >= 1, from r = fl(1/Q) (IEEE division)
- * with one fused multiply-add: q = trunc(fma(n, r, r * 2^-20)).
- * r = (1/Q)(1 + e1) and the FMA rounds once, |e1|, |e2| <= 2^-53, so the
- * result lies within t * 2^-52 + 2^-20/Q of t + 2^-20/Q where t = n/Q < 2^31/Q:
- *  - t = k integer: the bias 2^-20/Q exceeds the error t * 2^-52 < 2^-21/Q,
- *    so the result is >= k;
- *  - t <= k + 1 - 1/Q: bias + error < 1.25 * 2^-20/Q + 2^-53 < 1/Q, so the
- *    result stays below k + 1.
- * (the quantiser's divide on the GPU; fp64 FMA runs at full rate there) */
-TC_HD uint32_t tc_qdiv_f64(uint32_t n, double r)
+/* Exact floor(n / Q) for n < 2^31, Q >= 1, with one 32-bit high multiply
+ * (division by an invariant integer, Granlund and Montgomery 1994, thm 4.2):
+ * with l = ceil(log2 Q) and m = ceil(2^(31+l) / Q), 0 <= m Q - 2^(31+l) < Q
+ * <= 2^l, hence floor(n / Q) = floor(m n / 2^(31+l)) for every n < 2^31.
+ * m < 2^32 (m = 2^31 for Q a power of two, m < 2^32 otherwise since
+ * Q > 2^(l-1)), and m n / 2^(31+l) = umulhi(2n, m) / 2^l with 2n < 2^32.
+ * (the quantiser's divide on the GPU) */
+TC_HD void tc_qdiv32_consts(uint32_t Q, uint32_t* m, uint32_t* l)
 {
-    return (uint32_t)fma((double)n, r, r * 0x1p-20);
+    uint32_t s = 0u;
+    while (((uint64_t)1 << s) < Q) { ++s; }
+    *m = (uint32_t)((((uint64_t)1 << (31u + s)) + Q - 1u) / Q);
+    *l = s;
+}
+
+TC_HD uint32_t tc_qdiv32(uint32_t n, uint32_t m, uint32_t l)
+{
+    return (uint32_t)(((uint64_t)(n << 1) * m) >> 32) >> l;
 }
 
 /* Exact floor(n / d) for n * d < 2^51 via m = floor(2^51 / d) + 1:

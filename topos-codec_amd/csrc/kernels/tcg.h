@@ -44,9 +44,8 @@ typedef struct tcg_vlc_books {
     uint32_t len[12][64];  /* symbol -> code length (0: not in the alphabet) */
 } tcg_vlc_books;
 
-/* quantiser context of one (plane, qp): natural-order step, half step, dead
- * zone and the 51-bit exact reciprocal m = floor(2^51/Q)+1 split in two words
- * (q = (n * m) >> 51, exact for bit depth <= 12). */
+/* quantiser context of one (plane, qp): natural-order step Q, half step,
+ * dead zone and the constants of the exact divide by Q (tc_qdiv32). */
 /* encode transform tile: blocks of one block row per workgroup (the engine
  * sizes the grid with it) */
 #ifndef TCG_FQ_BLOCKS
@@ -54,7 +53,8 @@ typedef struct tcg_vlc_books {
 #endif
 
 typedef struct tcg_qctx {
-    double rq[64];       /* fl(1 / Q): the quantiser divide (tc_qdiv_f64) */
+    uint32_t qmul[64];   /* the quantiser divide floor(n / Q) = tc_qdiv32(n, qmul, qsh) */
+    uint32_t qsh[64];
     uint32_t Q[64];
     uint32_t half[64];
     uint32_t dz[64];
@@ -212,6 +212,9 @@ typedef struct tcg_batch_info {
     uint32_t ent_wpg;
     uint32_t ent_groups;
     uint32_t ent_count;  /* rANS2 colour slices in the work list */
+    /* transform kernels (FDCT + quantisation / dequantisation + IDCT): tiles
+     * of 32 blocks per workgroup, 1, 2 or 4 */
+    uint32_t tile_tpw;
     tcg_work* work;      /* device work list */
 } tcg_batch_info;
 
