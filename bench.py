@@ -207,6 +207,179 @@ def tune_batch(codec, cfg, in_addr, nf, nt, pkt_cap):
                      "default_ms": round(rep.idct_ms_default, 4)}}
 
 
+# other batch configurations: (kind, w, h, count, cfg kwargs) with product settings per tier
+XQ_BATCH = [("4444", 4096, 2160, 16, dict(qp=69, profile=6, pixel_format=2, bit_depth=12, slice_rows=4,
+                                          alpha_mode=1, alpha_bit_depth=16))]
+UNEVEN_BATCH = [
+    ("422", 1280, 720, 4, dict(qp=75, profile=3, qmatrix_id=4, features=1, slice_rows=8)),   # Proxy
+    ("422", 1920, 1080, 4, dict(qp=68, profile=3, qmatrix_id=4, features=1, slice_rows=8)),  # LT
+    ("422", 3840, 2160, 2, dict(qp=62, profile=3, qmatrix_id=0, features=3, slice_rows=8)),  # HQ
+    ("422", 8192, 4320, 1, dict(qp=62, profile=3, qmatrix_id=0, features=3, slice_rows=8)),  # HQ 8K
+    ("raw", 4096, 3072, 2, dict(qp=64, profile=7, pixel_format=3, bit_depth=12, slice_rows=8)),  # RAW 6:1
+]
+# serving mix: (share, kind, w, h, cfg kwargs); 60 % decodes, 40 % encodes
+SERVE_MIX = [
+    (0.3, "422", 1920, 1080, dict(qp=68, profile=3, qmatrix_id=4, features=1, slice_rows=8)),
+    (0.4, "422", 3840, 2160, dict(qp=62, profile=3, qmatrix_id=0, features=3, slice_rows=8)),
+    (0.2, "4444", 4096, 2160, dict(qp=69, profile=6, pixel_format=2, bit_depth=12, slice_rows=4,
+                                   alpha_mode=1, alpha_bit_depth=16)),
+    (0.1, "raw", 4096, 3072, dict(qp=64, profile=7, pixel_format=3, bit_depth=12, slice_rows=8)),
+]
+
+
+def make_frame(pkg, syn, kind, w, h, seed):
+    if kind == "422":
+        return pkg.synth_frame422(syn, w, h, seed)
+    if kind == "4444":
+        return pkg.synth_frame4444(syn, w, h, seed, alpha=True)
+    return pkg.synth_raw12(syn, w, h, seed)
+
+
+def resident_frame(pkg, abi, syn, codec, kind, w, h, seed, kw):
+    """A generated frame in HBM with its config, packet (host + HBM) and sample count."""
+    kw = dict(kw)
+    qp = kw.pop("qp")
+    cfg = abi.make_config(w, h, qp, entropy_mode=8, **kw)
+    planes = make_frame(pkg, syn, kind, w, h, seed)
+    bufs = []
+    for p in planes:
+        b = codec.alloc(p.nbytes)
+        b.upload(p)
+        bufs.append(b)
+    out = codec.alloc(sum(p.nbytes for p in planes) * 2 + (1 << 20))
+    st, sizes, rcs, _ = codec.encode_batch_dev([cfg], [[b.addr for b in bufs]], [out])
+    assert st == 0, codec.last_error()
+    pkt = out.download(sizes[0]).tobytes()
+    return {"cfg": cfg, "planes": planes, "bufs": bufs, "addr": [b.addr for b in bufs], "pkt": pkt, "dev_pkt": out,
+            "samples": sum(p.size for p in planes), "dims": [(p.shape[1], p.shape[0]) for p in planes],
+            "name": f"{kind} {w}x{h}"}
+
+
+def batch_config_bench(pkg, abi, syn, codec, orc, spec, steps):
+    """One launch sequence per direction over the whole batch; frames/s enc+dec
+    and samples/s, the packets checked for stability and against the oracle
+    (first frame of each kind)."""
+    from helpers import orc_decode, orc_encode
+    distinct, order = [], []
+    for kind, w, h, count, kw in spec:
+        reps = min(count, 4)
+        base = len(distinct)
+        for r in range(reps):
+            distinct.append(resident_frame(pkg, abi, syn, codec, kind, w, h, 100 + len(distinct), kw))
+        order += [base + (i % reps) for i in range(count)]
+    n = len(order)
+    outs = [codec.alloc(len(distinct[k]["pkt"]) + (1 << 20)) for k in order]
+    planes = [[codec.alloc(w * h * 2) for (w, h) in distinct[k]["dims"]] for k in order]
+    cfgs = [distinct[k]["cfg"] for k in order]
+    addrs = [distinct[k]["addr"] for k in order]
+
+    def enc():
+        st, sizes, rcs, _ = codec.encode_batch_dev(cfgs, addrs, outs)
+        assert st == 0, codec.last_error()
+        return sizes
+
+    def dec():
+        st, rcs, _ = codec.decode_batch_dev([distinct[k]["pkt"] for k in order], [distinct[k]["dev_pkt"] for k in order],
+                                            [[b.addr for b in p] for p in planes])
+        assert st == 0, codec.last_error()
+
+    sizes0 = enc()
+    dec()
+    for i, k in enumerate(order):
+        assert outs[i].download(sizes0[i]).tobytes() == distinct[k]["pkt"], distinct[k]["name"]
+    te, td = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        assert list(enc()) == list(sizes0)
+        t1 = time.perf_counter()
+        dec()
+        td.append(time.perf_counter() - t1)
+        te.append(t1 - t0)
+    checked = {}
+    for kind, w, h, count, kw in spec:
+        m = next(d for d in distinct if d["name"] == f"{kind} {w}x{h}")
+        rc, opk = orc_encode(orc, m["cfg"], m["planes"])
+        rc2, oplanes, _, _ = orc_decode(orc, m["pkt"], m["dims"])
+        i = next(j for j, k in enumerate(order) if distinct[k] is m)
+        got = [b.download(w2 * h2 * 2, np.uint16).reshape(h2, w2) for b, (w2, h2) in zip(planes[i], m["dims"])]
+        checked[m["name"]] = bool(rc == 0 and opk == m["pkt"] and rc2 == 0
+                                  and all((x == y).all() for x, y in zip(got, oplanes)))
+    samples = sum(distinct[k]["samples"] for k in order)
+    t = float(np.median(np.asarray(te) + np.asarray(td)))
+    return {"frames": n, "frames_per_s_enc_dec": round(n / t, 2), "samples_per_s_enc_dec": round(samples / t, 1),
+            "encode_ms": round(float(np.median(te)) * 1e3, 3), "decode_ms": round(float(np.median(td)) * 1e3, 3),
+            "equal_to_oracle": checked}
+
+
+def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
+    """The mixed serving load: 60 % decodes / 40 % encodes over 1080p LT, 4K HQ,
+    4K 4444 XQ + alpha and RAW 4096x3072 requests (shares 30/40/20/10 %), max
+    rate from a saturated run, then p50 / p99 under Poisson arrivals."""
+    import threading
+    frames = [resident_frame(pkg, abi, syn, codec, kind, w, h, 200 + i, kw)
+              for i, (_, kind, w, h, kw) in enumerate(SERVE_MIX)]
+    shares = np.array([m[0] for m in SERVE_MIX])
+    srv = pkg.Server(codec, hbm_gib << 30, max_batch=max_batch, max_inflight=lanes)
+
+    def submit(k, d):
+        f = frames[k]
+        if d:
+            return srv.submit_decode(f["pkt"], dev_packet=f["dev_pkt"].addr)
+        return srv.submit_encode(f["cfg"], dev_planes=f["addr"])
+
+    def drain(n):
+        lat = []
+        while len(lat) < n:
+            for c in srv.poll(256, timeout_s=30.0):
+                assert c.status == 0, c.status
+                lat.append((c.t_done_ns - c.t_submit_ns) / 1e6)
+                srv.release(c.id)
+        return lat
+
+    rng = np.random.default_rng(5)
+    nsat = 160
+    picks = [(int(rng.choice(4, p=shares)), bool(rng.random() < 0.6)) for _ in range(nsat)]
+    for k, d in picks[:16]:
+        assert submit(k, d)[0] == 0
+    drain(16)
+    t0 = time.perf_counter()
+    for k, d in picks:
+        assert submit(k, d)[0] == 0
+    drain(nsat)
+    max_rate = nsat / (time.perf_counter() - t0)
+    loads = []
+    for load in (0.5, 0.7, 0.9):
+        rate = load * max_rate
+        n = max(16, int(rate * seconds))
+        gaps = rng.exponential(1.0 / rate, size=n)
+        pk = [(int(rng.choice(4, p=shares)), bool(rng.random() < 0.6)) for _ in range(n)]
+        err = []
+
+        def arrivals():
+            t = time.perf_counter()
+            for g, (k, d) in zip(gaps, pk):
+                t += float(g)
+                dt = t - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                if submit(k, d)[0] != 0:
+                    err.append(k)
+                    return
+
+        th = threading.Thread(target=arrivals)
+        t1 = time.perf_counter()
+        th.start()
+        lat = drain(n)
+        th.join()
+        assert not err
+        loads.append({"load": load, "offered_req_per_s": round(rate, 2),
+                      "achieved_req_per_s": round(n / (time.perf_counter() - t1), 2), "requests": n,
+                      "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3)})
+    srv.close()
+    return {"mix": "60% decode / 40% encode; 30% 1080p LT, 40% 4K 422 HQ, 20% 4K 4444 XQ + alpha, 10% RAW 4096x3072",
+            "max_req_per_s": round(max_rate, 2), "loads": loads}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +394,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=2, help="serving loop: launch sequences in flight")
     ap.add_argument("--load-seconds", type=float, default=4.0, help="length of each Poisson load run")
     ap.add_argument("--no-tune", action="store_true", help="skip the entropy launch-shape autotuning")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other batch configurations (4444 XQ + alpha, uneven batch, serving mix)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -426,6 +601,16 @@ def main():
     dec0 = [b.download(w * h * 2, np.uint16).reshape(h, w) for b, (w, h) in zip(out_planes[0], dims)]
     bitexact = {"packets_stable": True}
 
+    configs = None
+    if not a.no_configs:
+        orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
+        t0 = time.perf_counter()
+        configs = {"xq4444_alpha_16x4096x2160": batch_config_bench(pkg, abi, syn, codec, orc, XQ_BATCH, 3),
+                   "uneven_batch": batch_config_bench(pkg, abi, syn, codec, orc, UNEVEN_BATCH, 3),
+                   "serving_mix": serve_mix_bench(pkg, abi, syn, codec, a.hbm_budget, a.max_batch, a.lanes,
+                                                  max(1.0, a.load_seconds / 2))}
+        print(f"# other configurations ({time.perf_counter() - t0:.1f}s): {configs}", file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu:
         cpu, cpu_pkt0 = cpu_baseline(abi, frames[:2], cfg, a.cpu_seconds)
@@ -494,6 +679,7 @@ def main():
                      "idct_achieved": round(samples * 6 / (idct_ms * 1e-3) / 1e9, 1) if idct_ms > 0 else None,
                      "deblock_achieved": round(samples * 4 / (db_ms * 1e-3) / 1e9, 1) if db_ms > 0 else None},
         "cpu_baseline": cpu,
+        "other_configs": configs,
         "autotune": tune,
         "bitexact": bitexact,
     }
