@@ -9,7 +9,7 @@ slice) at fixed qp 62. Inputs are resident in HBM before the timed region.
 A step is one pass of the batch through the hot path: 8 frame encodes and 8
 frame decodes, all data in HBM.
  - served-batched path (the reported value): the serving loop (tcx_server_*,
-   continuous batching over two engine lanes, batches of up to 32 frames, HBM-budget admission) is given
+   continuous batching over two engine lanes, batches of up to 64 frames, HBM-budget admission) is given
    K steps' worth of requests (8K encodes of the resident frames and 8K decodes
    of their resident packets, interleaved) and timed until the last completes;
  - latency under load: Poisson arrivals (half encodes, half decodes) at 50 %,
@@ -311,6 +311,50 @@ def batch_config_bench(pkg, abi, syn, codec, orc, spec, steps):
             "equal_to_oracle": checked}
 
 
+def convert_bench(pkg, abi, codec, orc, fmt, reps=20):
+    """Packed RGB(A) -> 4:2:2 10-bit planes + alpha, 3840x2160, every buffer
+    resident in HBM (tcx_convert_packed_rgb_dev): kernel time from HIP events,
+    algorithmic bytes = packed source + four 16-bit output planes."""
+    from helpers import cvt_call, cvt_source
+    w, h = W, H
+    rng = np.random.default_rng(300 + fmt)
+    src, _ = cvt_source(rng, fmt, abi, w, h)
+    rc, _, ref = cvt_call(orc.orc_convert_packed_rgb, abi, src, w, h, fmt, 0, with_alpha=True)
+    assert rc == 0
+    dsrc = codec.alloc(src.nbytes)
+    dsrc.upload(src)
+    shapes = abi.cvt_plane_shapes(w, h, 0, True)
+    outs = [codec.alloc(r * c * 2) for (r, c) in shapes]
+    p = abi.CvtParams()
+    p.format, p.width, p.height, p.out_mode, p.bit_depth, p.kr, p.kb = fmt, w, h, 0, 10, 0.2126, 0.0722
+    p.src = dsrc.addr
+    p.y_out, p.u_out, p.v_out, p.a_out = (b.addr for b in outs)
+    codec.set_timing(True)
+    ms = []
+    try:
+        for i in range(reps + 2):
+            assert codec.convert_packed_rgb_dev(p) == 0, codec.last_error()
+            if i >= 2:
+                ms.append(float(codec._lib.tcx_last_convert_ms()))
+    finally:
+        codec.set_timing(False)
+    equal = all(np.array_equal(b.download(r * c * 2, np.uint16).reshape(r, c), e)
+                for b, (r, c), e in zip(outs, shapes, ref))
+    nbytes = src.nbytes + sum(r * c * 2 for (r, c) in shapes)
+    t0 = time.perf_counter()
+    arr = src if abi.cvt_sample_bytes(fmt) == 1 else src.view("<u2")
+    for _ in range(3):
+        codec.convert_packed_rgb(arr, w, h, fmt=fmt, out_mode=0)
+    host_ms = (time.perf_counter() - t0) / 3 * 1e3
+    kms = float(np.mean(ms))
+    return {"workload": f"3840x2160 {['BGR24', 'RGB24', 'BGRA32', 'RGBA32', 'BGR48', 'RGB48', 'BGRA64', 'RGBA64'][fmt]}"
+                        " -> YUV 4:2:2 10-bit + 16-bit alpha, BT.709 limited",
+            "kernel_ms": round(kms, 4), "frames_per_s": round(1e3 / kms, 1),
+            "bytes_per_frame": nbytes, "achieved_gbs": round(nbytes / (kms * 1e-3) / 1e9, 1),
+            "frac_hbm_peak": round(nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "host_buffers_ms": round(host_ms, 2), "equal_to_oracle": bool(equal)}
+
+
 def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
     """The mixed serving load: 60 % decodes / 40 % encodes over 1080p LT, 4K HQ,
     4K 4444 XQ + alpha and RAW 4096x3072 requests (shares 30/40/20/10 %), max
@@ -390,7 +434,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
     ap.add_argument("--hbm-budget", type=int, default=24, help="serving loop HBM budget (GiB)")
-    ap.add_argument("--max-batch", type=int, default=32, help="serving loop: most frames per launch sequence")
+    ap.add_argument("--max-batch", type=int, default=64, help="serving loop: most frames per launch sequence")
     ap.add_argument("--lanes", type=int, default=2, help="serving loop: launch sequences in flight")
     ap.add_argument("--load-seconds", type=float, default=4.0, help="length of each Poisson load run")
     ap.add_argument("--no-tune", action="store_true", help="skip the entropy launch-shape autotuning")
@@ -607,6 +651,8 @@ def main():
         t0 = time.perf_counter()
         configs = {"xq4444_alpha_16x4096x2160": batch_config_bench(pkg, abi, syn, codec, orc, XQ_BATCH, 3),
                    "uneven_batch": batch_config_bench(pkg, abi, syn, codec, orc, UNEVEN_BATCH, 3),
+                   "convert_bgra32_4k": convert_bench(pkg, abi, codec, orc, 2),
+                   "convert_rgba64_4k": convert_bench(pkg, abi, codec, orc, 7),
                    "serving_mix": serve_mix_bench(pkg, abi, syn, codec, a.hbm_budget, a.max_batch, a.lanes,
                                                   max(1.0, a.load_seconds / 2))}
         print(f"# other configurations ({time.perf_counter() - t0:.1f}s): {configs}", file=sys.stderr, flush=True)

@@ -212,6 +212,77 @@ int32_t tc_decoder_decode(tc_decoder* dec, const uint8_t* packet, size_t size,
 int32_t tc_decoder_decode_batch(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
                                 const topos_plane_view* out, topos_frame_output* infos);
 
+/* ---- packed RGB(A) -> planar conversion (reference topos_codec.h:1016-1102) ----
+ * The step before encode in every export: packed 8-bit or 16-bit RGB(A)
+ * pixels to YUV 4:2:2 / 4:4:4 or GBR planes of 10 or 12 bits, float32
+ * arithmetic in the reference's operation order (no contraction, IEEE
+ * division, rint half to even), 4:2:2 chroma a box of the two quantised
+ * samples. Runs on the GPU; host buffers in and out. */
+typedef enum topos_cvt_format {
+    TOPOS_CVT_BGR24 = 0,
+    TOPOS_CVT_RGB24 = 1,
+    TOPOS_CVT_BGRA32 = 2,
+    TOPOS_CVT_RGBA32 = 3,
+    TOPOS_CVT_BGR48 = 4,
+    TOPOS_CVT_RGB48 = 5,
+    TOPOS_CVT_BGRA64 = 6,
+    TOPOS_CVT_RGBA64 = 7
+} topos_cvt_format;
+
+#define TOPOS_CVT_OUT_YUV422 0u
+#define TOPOS_CVT_OUT_YUV444 1u
+#define TOPOS_CVT_OUT_GBR 2u
+
+/* reference topos_codec.h:1043-1073 */
+typedef struct topos_cvt_params {
+    uint32_t struct_size;
+    uint32_t format;        /* topos_cvt_format */
+    uint32_t width;
+    uint32_t height;
+    uint32_t out_mode;      /* TOPOS_CVT_OUT_* */
+    uint32_t bit_depth;     /* 10 or 12 */
+    double kr;              /* forward matrix coefficients (ignored for GBR) */
+    double kb;
+    uint32_t full_range;    /* 1 full / 0 limited */
+    uint32_t alpha_shift;   /* alpha left shift = 16 - declared alpha depth */
+    uint32_t alpha_opaque;  /* 1: ignore input alpha, write the opaque value */
+    const void* src;        /* packed pixels, height x width x channels */
+    uint32_t src_stride;    /* bytes per row; 0 = tight */
+    uint16_t* y_out;
+    uint32_t y_stride;      /* elements per row; 0 = tight */
+    uint16_t* u_out;        /* YUV: the U plane (ceil(w/2) wide for 4:2:2); GBR: the B plane */
+    uint32_t u_stride;
+    uint16_t* v_out;        /* YUV: the V plane; GBR: the R plane (y_out holds G) */
+    uint32_t v_stride;
+    uint16_t* a_out;        /* alpha plane or NULL */
+    uint32_t a_stride;
+    uint32_t max_workers;   /* accepted for compatibility; the GPU splits the work itself */
+} topos_cvt_params;
+
+/* reference topos_codec.h:1075-1078 */
+int32_t tc_convert_packed_rgb(const topos_cvt_params* params);
+
+/* reference topos_codec.h:1080-1097 */
+typedef struct topos_planar_444_to_422_params {
+    uint32_t struct_size;
+    uint32_t width;         /* 4:4:4 input width */
+    uint32_t height;
+    uint32_t bit_depth;     /* 10, 12 or 16 (validation only) */
+    const uint16_t* u_in;
+    size_t u_in_stride;     /* elements; 0 = width */
+    const uint16_t* v_in;
+    size_t v_in_stride;
+    uint16_t* u_out;        /* height x ceil(width / 2) */
+    size_t u_out_stride;
+    uint16_t* v_out;
+    size_t v_out_stride;
+    uint32_t max_workers;   /* accepted for compatibility */
+} topos_planar_444_to_422_params;
+
+/* reference topos_codec.h:1100-1101: (a + b + 1) >> 1 of horizontal pairs,
+ * the edge sample repeated for an odd width */
+int32_t tc_convert_yuv444_to_422(const topos_planar_444_to_422_params* params);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,15 @@ typedef struct tcx_dec_batch {
 
 int32_t tcx_serve_step(const tcx_enc_batch* enc, const tcx_dec_batch* dec);
 
+/* the conversions of include/topos_codec.h with every buffer in HBM: the
+ * pointers of the parameter struct are device addresses (tcx_dev_alloc);
+ * same rules, constants and results, no staging copies */
+int32_t tcx_convert_packed_rgb_dev(const topos_cvt_params* params);
+int32_t tcx_convert_yuv444_to_422_dev(const topos_planar_444_to_422_params* params);
+/* kernel time (ms, HIP events) of the last device-resident conversion while
+ * tcx_set_timing is on */
+float tcx_last_convert_ms(void);
+
 /* ---- serving loop ------------------------------------------------------
  * A request loop over the batch engine. Encode and decode requests of any
  * size, profile and format are accepted as they arrive (tcx_server_submit

@@ -142,3 +142,50 @@ def toos_repair(data, fix_packet=True):
     b[48:52] = b"\0\0\0\0"
     b[48:52] = _zlib.crc32(bytes(b[:64])).to_bytes(4, "big")
     return bytes(b)
+
+
+def cvt_call(fn, abi, src, width, height, fmt, out_mode, *, bit_depth=10, kr=0.2126, kb=0.0722, full_range=0,
+             alpha_shift=0, alpha_opaque=0, with_alpha=False, src_stride=0, out_pad=0):
+    """Calls a tc_convert_packed_rgb-shaped function (oracle, compiled reference
+    or the product) with host buffers; out_pad extra elements per output row
+    exercise the output strides. Returns (rc, planes cropped to their widths)."""
+    shapes = abi.cvt_plane_shapes(width, height, out_mode, with_alpha)
+    full = [np.full((h, w + out_pad), 0xBEEF, np.uint16) for (h, w) in shapes]
+    p = abi.CvtParams()
+    p.struct_size = ctypes.sizeof(p)
+    p.format, p.width, p.height, p.out_mode, p.bit_depth = fmt, width, height, out_mode, bit_depth
+    p.kr, p.kb, p.full_range, p.alpha_shift, p.alpha_opaque = kr, kb, full_range, alpha_shift, alpha_opaque
+    p.src, p.src_stride = src.ctypes.data, src_stride
+    p.y_out, p.y_stride = full[0].ctypes.data, (width + out_pad) if out_pad else 0
+    p.u_out, p.u_stride = full[1].ctypes.data, (shapes[1][1] + out_pad) if out_pad else 0
+    p.v_out, p.v_stride = full[2].ctypes.data, (shapes[2][1] + out_pad) if out_pad else 0
+    if with_alpha:
+        p.a_out, p.a_stride = full[3].ctypes.data, (width + out_pad) if out_pad else 0
+    rc = fn(ctypes.byref(p))
+    return rc, [f[:, : s[1]] for f, s in zip(full, shapes)], full
+
+
+def p422_call(fn, abi, u, v, width, height, *, bit_depth=10, in_stride=0, out_pad=0):
+    cw = (width + 1) // 2
+    uo = np.full((height, cw + out_pad), 0xBEEF, np.uint16)
+    vo = np.full((height, cw + out_pad), 0xBEEF, np.uint16)
+    p = abi.Planar422Params()
+    p.struct_size = ctypes.sizeof(p)
+    p.width, p.height, p.bit_depth = width, height, bit_depth
+    p.u_in, p.u_in_stride, p.v_in, p.v_in_stride = u.ctypes.data, in_stride, v.ctypes.data, in_stride
+    p.u_out, p.v_out = uo.ctypes.data, vo.ctypes.data
+    p.u_out_stride = p.v_out_stride = (cw + out_pad) if out_pad else 0
+    rc = fn(ctypes.byref(p))
+    return rc, uo[:, :cw], vo[:, :cw]
+
+
+def cvt_source(rng, fmt, abi, width, height, row_pad=0, kind="random"):
+    """Packed source rows (uint8 view of the bytes) with optional row padding."""
+    nch, smp = abi.cvt_channels(fmt), abi.cvt_sample_bytes(fmt)
+    row = width * nch * smp
+    buf = np.zeros((height, row + row_pad), np.uint8)
+    if kind == "random":
+        buf[:, :row] = rng.integers(0, 256, size=(height, row), dtype=np.uint8)
+    elif kind == "max":
+        buf[:, :row] = 255
+    return buf, row + row_pad if row_pad else 0

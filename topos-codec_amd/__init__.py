@@ -84,6 +84,11 @@ def load_library(path=LIB_PATH):
     lib.tc_image_decode.argtypes = [vp, vp, vp]
     lib.tc_image_decode_preview.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp]
     lib.tc_image_query_decode_buffer.argtypes = [vp, ctypes.c_uint32, vp, vp]
+    for name in ("tc_convert_packed_rgb", "tc_convert_yuv444_to_422", "tcx_convert_packed_rgb_dev",
+                 "tcx_convert_yuv444_to_422_dev"):
+        getattr(lib, name).argtypes = [vp]
+        getattr(lib, name).restype = ctypes.c_int32
+    lib.tcx_last_convert_ms.restype = ctypes.c_float
     del u8p
     return lib
 
@@ -188,6 +193,89 @@ class ToposCodec:
                 ptrs[4 * i + p] = a.ctypes.data
         status = int(self._lib.tc_frame_decode_batch(pk, n, ptrs, None, infos))
         return status, outs, infos
+
+
+    def convert_packed_rgb(self, src, width, height, *, fmt=abi.CVT_BGR24, out_mode=abi.CVT_OUT_YUV422,
+                           bit_depth=10, kr=0.2126, kb=0.0722, full_range=False, alpha_shift=0,
+                           alpha_opaque=False, src_stride=0, max_workers=1, planes_out=None):
+        """Packed RGB(A) rows -> list of '<u2' planes [Y, U, V(, A)] or [G, B, R(, A)]
+        (reference topos_binding.py:3548 ToposCodec.convert_packed_rgb: same
+        arguments, plane shapes and checks). src is an (h, w * channels) C-contiguous
+        uint8 or uint16 array; 4:2:2 chroma planes are ceil(w / 2) wide."""
+        bad = abi.TC_ERR_INVALID_ARGUMENT
+        if src is None or not src.flags.c_contiguous:
+            raise ToposCodecError(bad, "convert_packed_rgb: src must be a C-contiguous array")
+        if src.dtype == np.uint8:
+            expect = (abi.CVT_BGR24, abi.CVT_RGB24, abi.CVT_BGRA32, abi.CVT_RGBA32)
+        elif src.dtype == np.uint16:
+            expect = (abi.CVT_BGR48, abi.CVT_RGB48, abi.CVT_BGRA64, abi.CVT_RGBA64)
+        else:
+            raise ToposCodecError(bad, f"convert_packed_rgb: dtype {src.dtype} not supported (u8/u16)")
+        if fmt not in expect:
+            raise ToposCodecError(bad, f"convert_packed_rgb: fmt {fmt} does not match dtype {src.dtype}")
+        nch = abi.cvt_channels(fmt)
+        if src.ndim != 2 or src.shape[1] != width * nch:
+            raise ToposCodecError(bad, f"convert_packed_rgb: src shape {src.shape} != (h, w*{nch})")
+        if src.shape[0] != height:
+            raise ToposCodecError(bad, f"convert_packed_rgb: src height {src.shape[0]} != {height}")
+        has_alpha = bool(alpha_opaque) or nch == 4
+        shapes = abi.cvt_plane_shapes(width, height, out_mode, has_alpha)
+        if planes_out is not None:
+            if len(planes_out) != len(shapes) or any(
+                    p.dtype != np.dtype("<u2") or p.shape != s or not p.flags.c_contiguous
+                    for p, s in zip(planes_out, shapes)):
+                raise ToposCodecError(bad, "convert_packed_rgb: planes_out shape/dtype mismatch")
+            planes = list(planes_out)
+        else:
+            planes = [np.empty(s, dtype="<u2") for s in shapes]
+        cp = abi.CvtParams()
+        cp.struct_size = ctypes.sizeof(abi.CvtParams)
+        cp.format, cp.width, cp.height, cp.out_mode, cp.bit_depth = fmt, width, height, out_mode, bit_depth
+        cp.kr, cp.kb = kr, kb
+        cp.full_range, cp.alpha_shift, cp.alpha_opaque = int(bool(full_range)), alpha_shift, int(bool(alpha_opaque))
+        cp.src = src.ctypes.data
+        cp.src_stride = src_stride if src_stride else width * nch * src.itemsize
+        cp.y_out, cp.y_stride = planes[0].ctypes.data, width
+        cp.u_out, cp.v_out = planes[1].ctypes.data, planes[2].ctypes.data
+        if has_alpha:
+            cp.a_out, cp.a_stride = planes[3].ctypes.data, width
+        cp.max_workers = max_workers
+        self._check(int(self._lib.tc_convert_packed_rgb(ctypes.byref(cp))), "tc_convert_packed_rgb")
+        return planes
+
+    def convert_yuv444_to_422(self, u_in, v_in, *, bit_depth=16, max_workers=1, planes_out=None):
+        """4:4:4 uint16 U/V planes -> 4:2:2 ((a + b + 1) >> 1 of horizontal pairs)
+        (reference topos_binding.py:3652 ToposCodec.convert_yuv444_to_422)."""
+        bad = abi.TC_ERR_INVALID_ARGUMENT
+        u, v = np.asarray(u_in), np.asarray(v_in)
+        if u.ndim != 2 or v.ndim != 2 or u.shape != v.shape or u.dtype != np.dtype("<u2") or \
+                v.dtype != np.dtype("<u2"):
+            raise ToposCodecError(bad, "convert_yuv444_to_422: inputs must be 2-D uint16 planes of one shape")
+        height, width = int(u.shape[0]), int(u.shape[1])
+        if width <= 0 or height <= 0:
+            raise ToposCodecError(bad, "convert_yuv444_to_422: empty planes")
+        if int(bit_depth) not in (10, 12, 16):
+            raise ToposCodecError(bad, f"convert_yuv444_to_422: bit_depth {bit_depth!r} not supported")
+        shape = (height, (width + 1) // 2)
+        if planes_out is None:
+            u_out, v_out = np.empty(shape, dtype="<u2"), np.empty(shape, dtype="<u2")
+        else:
+            if len(planes_out) != 2 or any(p.dtype != np.dtype("<u2") or p.shape != shape or
+                                           not p.flags.c_contiguous for p in planes_out):
+                raise ToposCodecError(bad, "convert_yuv444_to_422: planes_out shape/dtype mismatch")
+            u_out, v_out = planes_out
+        if not (u.flags.c_contiguous and v.flags.c_contiguous):
+            raise ToposCodecError(bad, "convert_yuv444_to_422: planes must be C-contiguous")
+        pp = abi.Planar422Params()
+        pp.struct_size = ctypes.sizeof(abi.Planar422Params)
+        pp.width, pp.height, pp.bit_depth = width, height, int(bit_depth)
+        pp.u_in, pp.u_in_stride = u.ctypes.data, u.strides[0] // 2
+        pp.v_in, pp.v_in_stride = v.ctypes.data, v.strides[0] // 2
+        pp.u_out, pp.u_out_stride = u_out.ctypes.data, u_out.strides[0] // 2
+        pp.v_out, pp.v_out_stride = v_out.ctypes.data, v_out.strides[0] // 2
+        pp.max_workers = max(1, min(64, int(max_workers)))
+        self._check(int(self._lib.tc_convert_yuv444_to_422(ctypes.byref(pp))), "tc_convert_yuv444_to_422")
+        return [u_out, v_out]
 
 
 class ToposImageError(RuntimeError):
@@ -350,6 +438,15 @@ class DeviceCodec(ToposCodec):
         st = self._lib.tcx_copy(ctypes.c_uint64(dst.addr), ctypes.c_uint64(src.addr), ctypes.c_size_t(nbytes))
         if st != 0:
             raise ToposCodecError(st, self.last_error())
+
+    def convert_packed_rgb_dev(self, params):
+        """tcx_convert_packed_rgb_dev: params is an abi.CvtParams whose pointers are device addresses."""
+        params.struct_size = ctypes.sizeof(abi.CvtParams)
+        return int(self._lib.tcx_convert_packed_rgb_dev(ctypes.byref(params)))
+
+    def convert_yuv444_to_422_dev(self, params):
+        params.struct_size = ctypes.sizeof(abi.Planar422Params)
+        return int(self._lib.tcx_convert_yuv444_to_422_dev(ctypes.byref(params)))
 
     def set_timing(self, on):
         self._lib.tcx_set_timing(1 if on else 0)

@@ -316,3 +316,52 @@ class TcxTuneReport(ctypes.Structure):
                 ("fdct_tpw", ctypes.c_uint32), ("idct_tpw", ctypes.c_uint32),
                 ("fdct_ms_best", ctypes.c_float), ("fdct_ms_default", ctypes.c_float),
                 ("idct_ms_best", ctypes.c_float), ("idct_ms_default", ctypes.c_float)]
+
+
+# ---- packed RGB(A) -> planar conversion (include/topos_codec.h) ----
+CVT_BGR24, CVT_RGB24, CVT_BGRA32, CVT_RGBA32, CVT_BGR48, CVT_RGB48, CVT_BGRA64, CVT_RGBA64 = range(8)
+CVT_OUT_YUV422, CVT_OUT_YUV444, CVT_OUT_GBR = 0, 1, 2
+
+
+def cvt_channels(fmt):
+    return 4 if fmt in (CVT_BGRA32, CVT_RGBA32, CVT_BGRA64, CVT_RGBA64) else 3
+
+
+def cvt_sample_bytes(fmt):
+    return 2 if fmt >= CVT_BGR48 else 1
+
+
+class CvtParams(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("format", ctypes.c_uint32), ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32), ("out_mode", ctypes.c_uint32), ("bit_depth", ctypes.c_uint32),
+        ("kr", ctypes.c_double), ("kb", ctypes.c_double),
+        ("full_range", ctypes.c_uint32), ("alpha_shift", ctypes.c_uint32), ("alpha_opaque", ctypes.c_uint32),
+        ("src", ctypes.c_void_p), ("src_stride", ctypes.c_uint32),
+        ("y_out", ctypes.c_void_p), ("y_stride", ctypes.c_uint32),
+        ("u_out", ctypes.c_void_p), ("u_stride", ctypes.c_uint32),
+        ("v_out", ctypes.c_void_p), ("v_stride", ctypes.c_uint32),
+        ("a_out", ctypes.c_void_p), ("a_stride", ctypes.c_uint32),
+        ("max_workers", ctypes.c_uint32),
+    ]
+
+
+class Planar422Params(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+        ("bit_depth", ctypes.c_uint32),
+        ("u_in", ctypes.c_void_p), ("u_in_stride", ctypes.c_size_t),
+        ("v_in", ctypes.c_void_p), ("v_in_stride", ctypes.c_size_t),
+        ("u_out", ctypes.c_void_p), ("u_out_stride", ctypes.c_size_t),
+        ("v_out", ctypes.c_void_p), ("v_out_stride", ctypes.c_size_t),
+        ("max_workers", ctypes.c_uint32),
+    ]
+
+
+def cvt_plane_shapes(width, height, out_mode, has_alpha):
+    """(rows, cols) of the output planes: Y (or G), U (or B), V (or R)[, A]."""
+    cw = (width + 1) // 2 if out_mode == CVT_OUT_YUV422 else width
+    shapes = [(height, width), (height, cw), (height, cw)]
+    if has_alpha:
+        shapes.append((height, width))
+    return shapes

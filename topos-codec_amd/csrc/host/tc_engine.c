@@ -195,6 +195,18 @@ int32_t tc_engine_acquire(tc_ectx** out)
 
 void tc_engine_release(tc_ectx* c) { pthread_mutex_unlock(&c->mu); }
 
+int32_t tc_engine_staging(tc_ectx* c, size_t in_bytes, size_t out_bytes, uint64_t* in, uint64_t* out)
+{
+    int32_t rc;
+    if ((rc = dbuf_need(c, &c->in, in_bytes ? in_bytes : 256u)) != TC_OK ||
+        (rc = dbuf_need(c, &c->out, out_bytes ? out_bytes : 256u)) != TC_OK) {
+        return rc;
+    }
+    *in = c->in.p;
+    *out = c->out.p;
+    return TC_OK;
+}
+
 size_t tc_engine_encode_cap(const topos_frame_config* cfg)
 {
     tc_fhdr fh;
@@ -237,6 +249,26 @@ void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec)
     *enc = g_last_enc;
     *dec = g_last_dec;
     pthread_mutex_unlock(&g_tmu);
+}
+
+static float g_last_cvt_ms;
+
+void tc_engine_record_convert(tc_ectx* c)
+{
+    if (!tcg_timing_on()) { return; }
+    tcg_timing t;
+    tcg_ctx_timing(c->g, &t);
+    pthread_mutex_lock(&g_tmu);
+    g_last_cvt_ms = t.convert_ms;
+    pthread_mutex_unlock(&g_tmu);
+}
+
+float tc_engine_last_convert_ms(void)
+{
+    pthread_mutex_lock(&g_tmu);
+    const float v = g_last_cvt_ms;
+    pthread_mutex_unlock(&g_tmu);
+    return v;
 }
 
 static void record_timing(tc_ectx* c, int encode)

@@ -92,6 +92,9 @@ int32_t tc_engine_acquire(tc_ectx** out);
 /* never blocks: 1 (and *out = NULL) when every context is in use */
 int32_t tc_engine_try_acquire(tc_ectx** out);
 void tc_engine_release(tc_ectx* c);
+/* the context's input / output staging buffers, grown to at least the sizes
+ * given (conversions of host buffers) */
+int32_t tc_engine_staging(tc_ectx* c, size_t in_bytes, size_t out_bytes, uint64_t* in, uint64_t* out);
 /* device packet capacity the encoder needs for one frame of cfg (0 on a bad
  * config): frame header plus every slice's header and entropy bound */
 size_t tc_engine_encode_cap(const topos_frame_config* cfg);
@@ -107,6 +110,9 @@ uint32_t tc_tune_tiles(int encode, uint32_t cols, uint32_t tiles);
 void tc_engine_graph_stats(uint64_t* hits, uint64_t* builds);
 /* per-kernel times of the last finished encode and decode (timing on) */
 void tc_engine_last_timing(tcg_timing* enc, tcg_timing* dec);
+/* keeps the conversion kernel time of the context's last launch (timing on) */
+void tc_engine_record_convert(tc_ectx* c);
+float tc_engine_last_convert_ms(void);
 
 /* encode n frames (all with the same cfg-derived header but separate inputs)
  * into device packets; host_out[i] receives frame i's packet when non-NULL */

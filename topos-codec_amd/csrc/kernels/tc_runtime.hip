@@ -289,6 +289,8 @@ void tcg_ctx_timing(tcg_ctx* c, tcg_timing* t)
         t->fdct_ms = ms[0]; t->slice_enc_ms = ms[1]; t->scan_ms = ms[2]; t->emit_ms = ms[3];
     } else if (c->kind == 2) {
         t->slice_dec_ms = ms[0]; t->idct_ms = ms[1]; t->deblock_ms = ms[2];
+    } else if (c->kind == 3) {
+        t->convert_ms = ms[0];
     }
 }
 

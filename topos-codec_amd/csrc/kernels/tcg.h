@@ -159,6 +159,26 @@ typedef struct tcg_rowrec {
     uint32_t slice;
 } tcg_rowrec;
 
+/* packed RGB(A) -> planes: everything one launch needs (device addresses) */
+typedef struct tcg_cvt {
+    uint64_t src, y, u, v, a;   /* a = 0: no alpha plane */
+    uint32_t src_stride;        /* bytes per source row */
+    uint32_t y_st, u_st, v_st, a_st;  /* elements per output row */
+    uint32_t width, height, cw; /* cw = ceil(width / 2) */
+    uint32_t mode;              /* TOPOS_CVT_OUT_* (0 4:2:2, 1 4:4:4, 2 GBR) */
+    uint32_t nch, smp;          /* channels, bytes per sample */
+    uint32_t ro, go, bo, ao;    /* channel offsets in samples */
+    uint32_t alpha_shift, opaque;
+    float scale, kr, kb, kf, du, dv, y_mult, y_off, c_mult, c_off, a_nmax, norm_div;
+} tcg_cvt;
+
+/* 4:4:4 -> 4:2:2 chroma (device addresses, strides in elements) */
+typedef struct tcg_p422 {
+    uint64_t u_in, v_in, u_out, v_out;
+    uint64_t uis, vis, uos, vos;
+    uint32_t width, height, cw, pad_;
+} tcg_p422;
+
 /* ---- runtime ---- */
 /* A context is one default-priority stream plus its timing events; work on
  * different contexts runs concurrently on the device. */
@@ -223,6 +243,10 @@ typedef struct tcg_batch_info {
 uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols);
 
 
+/* ---- conversions (queued on the context's stream) ---- */
+int32_t tcg_convert_rgb(tcg_ctx* c, const tcg_cvt* k);
+int32_t tcg_convert_444_422(tcg_ctx* c, const tcg_p422* k);
+
 /* ---- encode ---- */
 int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_eplane* planes_dev,
                    uint32_t nslices, const tcg_eslice* slices_dev, uint32_t nframes,
@@ -243,6 +267,7 @@ int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const
 typedef struct tcg_timing {
     float fdct_ms, slice_enc_ms, scan_ms, emit_ms;
     float slice_dec_ms, idct_ms, deblock_ms;
+    float convert_ms;   /* conversion kernel */
 } tcg_timing;
 void tcg_set_timing(int on);
 /* launch sequences through captured HIP graphs (default on; the environment
