@@ -190,6 +190,18 @@ size_t tc_frame_packet_bound(const topos_frame_config* cfg);
 /* reference topos_codec.h:296-297: out untouched unless TC_OK */
 int32_t tc_frame_encode(const topos_frame_config* cfg, const topos_frame_input* input,
                         uint8_t* out, size_t out_cap, topos_frame_stats* stats);
+/* reference topos_codec.h:315-328: target-size encode. Deterministic qp search
+ * in [qp_min, qp_max] (strict by default: the smallest qp whose packet fits
+ * target_bytes; legacy linear when a probe sequence breaks monotonicity or
+ * the 24-probe budget runs out; TOPOS_SIZED_SEARCH=linear|r6|strict selects
+ * the search once per process), seeded by the process's last two working
+ * points; the packet equals tc_frame_encode at *qp_used. out untouched unless
+ * TC_OK; TC_ERR_BUFFER_TOO_SMALL reports the final size in stats. RAW
+ * (pixel format 3) and entropy modes 9-11 return TC_ERR_NOT_IMPLEMENTED. */
+int32_t tc_frame_encode_sized(const topos_frame_config* cfg, const topos_frame_input* input,
+                              uint32_t target_bytes, uint8_t qp_min, uint8_t qp_max,
+                              uint8_t* qp_used, uint8_t* out, size_t out_cap,
+                              topos_frame_stats* stats);
 /* reference topos_codec.h:336-339: planes_out == NULL is query mode */
 int32_t tc_frame_decode(const uint8_t* data, size_t size,
                         uint16_t* const planes_out[TC_FRAME_MAX_PLANES],

@@ -49,6 +49,41 @@ int32_t tcx_decode_batch_dev(uint32_t n, const uint8_t* const* host_packets, con
                              const uint64_t* dev_packets, const uint64_t* dev_planes,
                              const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs);
 
+/* ---- target-size encode of many frames -----------------------------------
+ * Each request runs the search of tc_frame_encode_sized on its own frame
+ * (device planes in, device packet out) with its own seed state; the
+ * searches of all requests share every launch sequence: each round encodes
+ * every unfinished frame at the qp its search needs next and at nearby qps.
+ * stream: the request's seed state, updated on success (NULL: no seed). */
+typedef struct tcx_sized_stream {
+    uint32_t qp, valid, bytes, prev, prev_bytes;
+} tcx_sized_stream;
+
+typedef struct tcx_sized_req {
+    topos_frame_config cfg;       /* qp_base: where an unseeded search starts */
+    uint64_t dev_planes[4];
+    size_t dev_strides[4];        /* elements; 0 = visible width */
+    uint32_t target_bytes;
+    uint8_t qp_min, qp_max;
+    tcx_sized_stream* stream;
+    uint64_t dev_out;
+    size_t dev_out_cap;
+    /* results */
+    int32_t rc;
+    uint8_t qp_used;
+    uint32_t packet_size;
+    uint32_t encodes;             /* probes + final, counted as the reference counts them */
+    topos_frame_stats stats;
+} tcx_sized_req;
+
+/* returns the first non-OK request status */
+int32_t tcx_encode_sized_batch_dev(tcx_sized_req* reqs, uint32_t n);
+/* process-wide seed state of tc_frame_encode_sized: forget it; search mode
+ * (0 strict, 1 legacy linear, 2 r6); seed / seed-model switches */
+void tcx_sized_reset(void);
+void tcx_sized_set_mode(int32_t mode);
+void tcx_sized_set_seed(int32_t disable, int32_t model_disable);
+
 /* One serving step: an encode batch and a decode batch run concurrently, each
  * on its own stream (either may be NULL or empty); returns when both are done.
  * Fields mean what the arguments of tcx_encode_batch_dev / tcx_decode_batch_dev

@@ -1502,3 +1502,15 @@ int orc_image_synth(uint64_t seed, uint32_t w, uint32_t h, uint32_t kind, uint32
     }
     return 0;
 }
+
+/* colour-plane 8x8 block count of a frame (sized encode's F-cache size rule) */
+uint64_t orc_color_blocks(const orc_cfg* cfg)
+{
+    orc_hdr h;
+    cfg_to_hdr(cfg, &h);
+    derive_geometry(&h);
+    const uint32_t colour = h.pixel_format == 3u ? 4u : 3u;
+    uint64_t n = 0;
+    for (uint32_t p = 0; p < colour; ++p) { n += (uint64_t)h.bcols[p] * h.brows[p]; }
+    return n;
+}

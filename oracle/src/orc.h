@@ -76,6 +76,19 @@ int32_t orc_frame_encode(const orc_cfg* cfg, const uint16_t* const planes[4],
                          const size_t strides[4], uint8_t* out, size_t cap, size_t* size_out);
 int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes[4],
                          const size_t strides[4], uint8_t* slice_status, uint32_t* concealed);
+uint64_t orc_color_blocks(const orc_cfg* cfg);
+
+/* target-size encode: qp search over whole-frame encodes (orc_sized.c) */
+int32_t orc_frame_encode_sized(const orc_cfg* cfg, const uint16_t* const planes[4], const size_t strides[4],
+                               uint32_t target, uint8_t qp_min, uint8_t qp_max, uint8_t* qp_used,
+                               uint8_t* out, size_t cap, size_t* size_out, uint32_t* probes);
+void orc_sized_reset(void);
+void orc_sized_set_mode(int32_t mode);
+void orc_sized_set_hint(int32_t disable, int32_t model_disable);
+/* the process-wide seed state (qp, valid, bytes, prev, prev_bytes) */
+void orc_sized_get_state(uint32_t s[5]);
+void orc_sized_put_state(const uint32_t s[5]);
+
 void orc_deblock_plane(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uint32_t qp,
                        uint32_t tbl, uint32_t bit_depth);
 

@@ -189,3 +189,36 @@ def cvt_source(rng, fmt, abi, width, height, row_pad=0, kind="random"):
     elif kind == "max":
         buf[:, :row] = 255
     return buf, row + row_pad if row_pad else 0
+
+
+# ---------------------------------------------------------------- sized encode
+def orc_sized(oracle, cfg, planes, target, qp_min, qp_max, cap=None):
+    """Oracle target-size encode: (rc, qp_used, packet, size, encodes counted)."""
+    ptrs = (ctypes.c_void_p * 4)(*([p.ctypes.data for p in planes] + [None] * (4 - len(planes))))
+    if cap is None:
+        cap = sum(p.nbytes for p in planes) * 4 + 65536
+    out = np.empty(max(cap, 1), np.uint8)
+    q, n, cnt = ctypes.c_uint8(0), ctypes.c_size_t(0), ctypes.c_uint32(0)
+    rc = oracle.orc_frame_encode_sized(ctypes.byref(cfg), ptrs, None, ctypes.c_uint32(target), ctypes.c_uint8(qp_min),
+                                       ctypes.c_uint8(qp_max), ctypes.byref(q), ctypes.c_void_p(out.ctypes.data),
+                                       ctypes.c_size_t(cap), ctypes.byref(n), ctypes.byref(cnt))
+    return rc, q.value, out[: n.value].tobytes() if rc == 0 else b"", n.value, cnt.value
+
+
+def lib_sized(lib, abi, cfg, planes, target, qp_min, qp_max, cap=None):
+    """tc_frame_encode_sized of a library (the compiled reference or the
+    product): (rc, qp_used, packet, stats.packet_size)."""
+    inp = abi.FrameInput()
+    inp.struct_size = ctypes.sizeof(inp)
+    inp.abi_version = 2
+    for i, p in enumerate(planes):
+        inp.planes[i] = p.ctypes.data
+    if cap is None:
+        cap = lib.tc_frame_packet_bound(ctypes.byref(cfg))
+    out = ctypes.create_string_buffer(max(cap, 1))
+    st = abi.FrameStats()
+    q = ctypes.c_uint8(0)
+    rc = lib.tc_frame_encode_sized(ctypes.byref(cfg), ctypes.byref(inp), ctypes.c_uint32(target),
+                                   ctypes.c_uint8(qp_min), ctypes.c_uint8(qp_max), ctypes.byref(q),
+                                   out if cap else None, ctypes.c_size_t(cap), ctypes.byref(st))
+    return rc, q.value, out.raw[: st.packet_size] if rc == 0 else b"", st.packet_size

@@ -52,6 +52,27 @@ int main(void) {
     assert got == want
 
 
+def test_sized_struct_layouts(abi, tmp_path):
+    src = tmp_path / "sized_sz.c"
+    src.write_text(r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "topos_gpu.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tcx_sized_req), sizeof(tcx_sized_stream),
+           offsetof(tcx_sized_req, target_bytes), offsetof(tcx_sized_req, stream), offsetof(tcx_sized_req, rc),
+           offsetof(tcx_sized_req, packet_size), offsetof(tcx_sized_req, encodes), offsetof(tcx_sized_req, stats));
+    return 0;
+}
+""")
+    exe = _build(tmp_path, str(src), "sized_sz")
+    got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    R = abi.TcxSizedReq
+    want = [ctypes.sizeof(R), ctypes.sizeof(abi.TcxSizedStream), R.target_bytes.offset, R.stream.offset,
+            R.rc.offset, R.packet_size.offset, R.encodes.offset, R.stats.offset]
+    assert got == want
+
+
 def test_server_without_gpu_fails_loudly(pkg):
     from conftest import gpu_available
     if gpu_available():

@@ -89,6 +89,15 @@ def load_library(path=LIB_PATH):
         getattr(lib, name).argtypes = [vp]
         getattr(lib, name).restype = ctypes.c_int32
     lib.tcx_last_convert_ms.restype = ctypes.c_float
+    lib.tc_frame_encode_sized.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint8, ctypes.c_uint8, vp, vp,
+                                          ctypes.c_size_t, vp]
+    lib.tcx_encode_sized_batch_dev.argtypes = [vp, ctypes.c_uint32]
+    lib.tcx_sized_reset.argtypes = []
+    lib.tcx_sized_reset.restype = None
+    lib.tcx_sized_set_mode.argtypes = [ctypes.c_int32]
+    lib.tcx_sized_set_mode.restype = None
+    lib.tcx_sized_set_seed.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    lib.tcx_sized_set_seed.restype = None
     del u8p
     return lib
 
@@ -150,6 +159,36 @@ class ToposCodec:
                                                ctypes.c_size_t(cap), ctypes.byref(st)))
         pkt = out[: st.packet_size].tobytes() if status == 0 else b""
         return status, pkt, st
+
+    def encode_sized(self, config, planes, target_bytes, qp_min=0, qp_max=abi.TC_QP_MAX, out_cap=None):
+        """Target-size encode (reference topos_binding.py:3235 ToposCodec.encode_sized):
+        returns (packet, stats, qp_used); raises on a failed status."""
+        st, pkt, stats, q = self.encode_sized_status(config, planes, target_bytes, qp_min, qp_max, out_cap)
+        self._check(st, "tc_frame_encode_sized")
+        return pkt, stats, q
+
+    def encode_sized_status(self, config, planes, target_bytes, qp_min=0, qp_max=abi.TC_QP_MAX, out_cap=None):
+        inp = abi.FrameInput()
+        inp.struct_size = ctypes.sizeof(abi.FrameInput)
+        inp.abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        keep = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+        for i, p in enumerate(keep):
+            inp.planes[i] = p.ctypes.data
+        cap = self.packet_bound(config) if out_cap is None else out_cap
+        out = np.empty(max(cap, 1), np.uint8)
+        stats = abi.FrameStats()
+        q = ctypes.c_uint8(0)
+        st = int(self._lib.tc_frame_encode_sized(ctypes.byref(config), ctypes.byref(inp), target_bytes, qp_min,
+                                                 qp_max, ctypes.byref(q), _ptr(out) if cap else None, cap,
+                                                 ctypes.byref(stats)))
+        pkt = out[: stats.packet_size].tobytes() if st == 0 else b""
+        return st, pkt, stats, q.value
+
+    def sized_reset(self):
+        self._lib.tcx_sized_reset()
+
+    def sized_set_mode(self, mode):
+        self._lib.tcx_sized_set_mode(mode)
 
     def query(self, packet):
         info = abi.FrameOutput()
@@ -438,6 +477,14 @@ class DeviceCodec(ToposCodec):
         st = self._lib.tcx_copy(ctypes.c_uint64(dst.addr), ctypes.c_uint64(src.addr), ctypes.c_size_t(nbytes))
         if st != 0:
             raise ToposCodecError(st, self.last_error())
+
+    def encode_sized_batch_dev(self, reqs):
+        """tcx_encode_sized_batch_dev over an array of abi.TcxSizedReq (filled in place)."""
+        arr = (abi.TcxSizedReq * len(reqs))(*reqs)
+        st = int(self._lib.tcx_encode_sized_batch_dev(arr, len(reqs)))
+        for i in range(len(reqs)):
+            reqs[i] = arr[i]
+        return st
 
     def convert_packed_rgb_dev(self, params):
         """tcx_convert_packed_rgb_dev: params is an abi.CvtParams whose pointers are device addresses."""

@@ -365,3 +365,22 @@ def cvt_plane_shapes(width, height, out_mode, has_alpha):
     if has_alpha:
         shapes.append((height, width))
     return shapes
+
+
+# ---- target-size encode (include/topos_gpu.h) ----
+TC_QP_MAX = 95
+
+
+class TcxSizedStream(ctypes.Structure):
+    _fields_ = [("qp", ctypes.c_uint32), ("valid", ctypes.c_uint32), ("bytes", ctypes.c_uint32),
+                ("prev", ctypes.c_uint32), ("prev_bytes", ctypes.c_uint32)]
+
+
+class TcxSizedReq(ctypes.Structure):
+    _fields_ = [
+        ("cfg", FrameConfig), ("dev_planes", ctypes.c_uint64 * 4), ("dev_strides", ctypes.c_size_t * 4),
+        ("target_bytes", ctypes.c_uint32), ("qp_min", ctypes.c_uint8), ("qp_max", ctypes.c_uint8),
+        ("stream", ctypes.POINTER(TcxSizedStream)), ("dev_out", ctypes.c_uint64), ("dev_out_cap", ctypes.c_size_t),
+        ("rc", ctypes.c_int32), ("qp_used", ctypes.c_uint8), ("packet_size", ctypes.c_uint32),
+        ("encodes", ctypes.c_uint32), ("stats", FrameStats),
+    ]

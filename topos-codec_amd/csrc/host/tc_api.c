@@ -85,6 +85,11 @@ int32_t tc_enc_check_dev(const topos_frame_config* cfg, const uint64_t* dev_plan
     return enc_check(cfg, NULL, dev_planes, dev_strides);
 }
 
+int32_t tc_enc_check_host(const topos_frame_config* cfg, const topos_frame_input* input)
+{
+    return enc_check(cfg, input, NULL, NULL);
+}
+
 int32_t tc_frame_encode(const topos_frame_config* cfg, const topos_frame_input* input,
                         uint8_t* out, size_t out_cap, topos_frame_stats* stats)
 {

@@ -62,6 +62,7 @@ int32_t tcx_d2h(void* dst, uint64_t src, size_t bytes)
 
 int32_t tcx_copy(uint64_t dst, uint64_t src, size_t bytes)
 {
+    if (bytes % 16u != 0u) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "copy size %zu is not a multiple of 16", bytes); }
     tc_ectx* c = NULL;
     int32_t rc = tc_engine_acquire(&c);
     if (rc != TC_OK) { return rc; }

@@ -85,6 +85,10 @@ void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bit_de
 uint32_t tc_deblock_alpha(uint32_t tbl, uint32_t qp);
 void tc_fill_output_info(const tc_fhdr* fh, topos_frame_output* info);
 
+/* encode argument checks of tc_frame_encode (tc_api.c): host planes or device planes */
+int32_t tc_enc_check_host(const topos_frame_config* cfg, const topos_frame_input* input);
+int32_t tc_enc_check_dev(const topos_frame_config* cfg, const uint64_t* dev_planes, const size_t* dev_strides);
+
 /* ---- device engine (tc_engine.c) ---- */
 typedef struct tc_ectx tc_ectx;
 /* lazily initialises the GPU; returns a free context (locked until release) */

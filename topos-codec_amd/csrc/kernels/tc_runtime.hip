@@ -163,6 +163,13 @@ int32_t tcg_copy(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes)
     return e == hipSuccess ? 0 : fail("copy", e);
 }
 
+int32_t tcg_d2d(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes)
+{
+    const hipError_t e = hipMemcpyAsync((void*)(uintptr_t)dst, (const void*)(uintptr_t)src, bytes,
+                                        hipMemcpyDeviceToDevice, c->stream);
+    return e == hipSuccess ? 0 : fail("device copy", e);
+}
+
 int32_t tcg_malloc(uint64_t* dptr, size_t bytes)
 {
     void* p = nullptr;
