@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same
 # counters read for a plain 256 MiB device copy (fetch x2.0, write x1.0).
 FDCT_TRAFFIC_BYTES = 796_632_857
+SIZED_TARGET = 6_000_000          # bytes per 4K 422 HQ frame for the target-size encode leg
 
 
 def load_pkg():
@@ -355,6 +356,68 @@ def convert_bench(pkg, abi, codec, orc, fmt, reps=20):
             "host_buffers_ms": round(host_ms, 2), "equal_to_oracle": bool(equal)}
 
 
+def sized_bench(pkg, abi, codec, frames, in_addr, cfg, target=SIZED_TARGET):
+    """Target-size encode of 4K 422 HQ frames to target bytes (qp 0..95):
+    tc_frame_encode_sized on one frame (host buffers, cold seed and seeded
+    repeat) and tcx_encode_sized_batch_dev on the resident batch (one seed
+    state per frame, cold and seeded); the compiled reference's
+    tc_frame_encode_sized on the same frame when it is built (its own slice
+    threads). The chosen qp and packet agree with the reference's."""
+    codec.sized_set_mode(0)
+    res = {"workload": f"3840x2160 422 HQ frames to a {target}-byte target, qp 0..95, strict search"}
+    times, pk0 = [], None
+    codec.sized_reset()
+    for rep in range(3):
+        t0 = time.perf_counter()
+        st, pkt, stats, q = codec.encode_sized_status(cfg, frames[0], target, 0, 95)
+        times.append(time.perf_counter() - t0)
+        assert st == 0 and len(pkt) <= target, (st, codec.last_error())
+        if rep == 0:
+            pk0, q0 = pkt, q
+        assert pkt == pk0
+    res["single_frame_ms"] = {"cold": round(times[0] * 1e3, 2), "seeded": round(float(np.median(times[1:])) * 1e3, 2)}
+    res["qp_used"] = q0
+    res["packet_bytes"] = len(pk0)
+    streams = [abi.TcxSizedStream() for _ in in_addr]
+    outs = [codec.alloc(codec.packet_bound(cfg)) for _ in in_addr]
+    bt = []
+    for rep in range(3):
+        reqs = []
+        for k, addr in enumerate(in_addr):
+            r = abi.TcxSizedReq()
+            r.cfg = cfg
+            for p in range(3):
+                r.dev_planes[p] = addr[p]
+            r.dev_out, r.dev_out_cap = outs[k].addr, codec.packet_bound(cfg)
+            r.target_bytes, r.qp_min, r.qp_max = target, 0, 95
+            r.stream = ctypes.pointer(streams[k])
+            reqs.append(r)
+        t0 = time.perf_counter()
+        assert codec.encode_sized_batch_dev(reqs) == 0, codec.last_error()
+        bt.append(time.perf_counter() - t0)
+    assert reqs[0].qp_used == q0 and outs[0].download(reqs[0].packet_size).tobytes() == pk0
+    n = len(in_addr)
+    res["batch"] = {"frames": n, "cold_ms": round(bt[0] * 1e3, 2), "seeded_ms": round(float(np.median(bt[1:])) * 1e3, 2),
+                    "frames_per_s_seeded": round(n / float(np.median(bt[1:])), 2),
+                    "encodes_per_frame": [int(r.encodes) for r in reqs]}
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libtopos_ref.so")
+    if os.path.exists(ref_path):
+        from helpers import lib_sized
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["TOPOS_SLICE_THREADS"] = str(threads)
+        ref = ctypes.CDLL(ref_path)
+        ref.tc_frame_packet_bound.restype = ctypes.c_size_t
+        rt = []
+        for rep in range(2):
+            t0 = time.perf_counter()
+            r = lib_sized(ref, abi, cfg, frames[0], target, 0, 95)
+            rt.append(time.perf_counter() - t0)
+        res["cpu_reference_ms"] = {"first": round(rt[0] * 1e3, 1), "repeat": round(rt[1] * 1e3, 1),
+                                   "threads": threads}
+        res["equal_to_reference"] = bool(r[0] == 0 and r[1] == q0 and r[2] == pk0)
+    return res
+
+
 def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
     """The mixed serving load: 60 % decodes / 40 % encodes over 1080p LT, 4K HQ,
     4K 4444 XQ + alpha and RAW 4096x3072 requests (shares 30/40/20/10 %), max
@@ -651,6 +714,7 @@ def main():
         t0 = time.perf_counter()
         configs = {"xq4444_alpha_16x4096x2160": batch_config_bench(pkg, abi, syn, codec, orc, XQ_BATCH, 3),
                    "uneven_batch": batch_config_bench(pkg, abi, syn, codec, orc, UNEVEN_BATCH, 3),
+                   "sized_4k_422hq": sized_bench(pkg, abi, codec, frames, in_addr, cfg),
                    "convert_bgra32_4k": convert_bench(pkg, abi, codec, orc, 2),
                    "convert_rgba64_4k": convert_bench(pkg, abi, codec, orc, 7),
                    "serving_mix": serve_mix_bench(pkg, abi, syn, codec, a.hbm_budget, a.max_batch, a.lanes,

@@ -91,6 +91,8 @@ struct tcg_ctx {
     int ngraphs;
     uint64_t clock;
     uint64_t graph_hits, graph_builds;
+    void* pin;      // pinned host staging for strided host copies (grow-only)
+    size_t pin_cap;
 };
 const TcgConsts* tcg_consts_dev();
 int32_t tcg_check(const char* what);

@@ -149,6 +149,7 @@ void tcg_ctx_destroy(tcg_ctx* c)
     if (c == nullptr) { return; }
     (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < c->ngraphs; ++i) { (void)hipGraphExecDestroy(c->graphs[i].exec); }
+    if (c->pin != nullptr) { (void)hipHostFree(c->pin); }
     for (int i = 0; i < 8; ++i) { (void)hipEventDestroy(c->ev[i]); }
     (void)hipStreamDestroy(c->stream);
     free(c);
@@ -221,21 +222,80 @@ int32_t tcg_d2h_sync(tcg_ctx* c, void* dst, uint64_t src, size_t bytes)
     return e == hipSuccess ? 0 : fail("hipMemcpy D2H", e);
 }
 
+// Host copies: rows that are tight on both sides go as one plain copy (the
+// runtime streams pageable memory at full PCIe rate); strided ones go through
+// the context's pinned staging buffer: the rows are packed on the CPU and
+// cross PCIe as one DMA (a strided copy of pageable memory is row by row).
+// The stream is drained before the buffer is refilled, so the previous copy
+// out of it has finished.
+static int pin_need(tcg_ctx* c, size_t bytes)
+{
+    if (c->pin_cap >= bytes) { return 1; }
+    if (c->pin != nullptr) { (void)hipHostFree(c->pin); }
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    const size_t cap = (bytes + ((size_t)1 << 20) - 1u) & ~(((size_t)1 << 20) - 1u);
+    if (hipHostMalloc(&c->pin, cap, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->pin = nullptr;
+        return 0;
+    }
+    c->pin_cap = cap;
+    return 1;
+}
+
 int32_t tcg_h2d_2d(tcg_ctx* c, uint64_t dst, size_t dpitch, const void* src, size_t spitch, size_t width,
                    size_t rows)
 {
     if (width == 0u || rows == 0u) { return 0; }
-    const hipError_t e = hipMemcpy2DAsync((void*)(uintptr_t)dst, dpitch, src, spitch, width, rows,
-                                          hipMemcpyHostToDevice, c->stream);
+    if (spitch == width && dpitch == width) {
+        const hipError_t e1 = hipMemcpyAsync((void*)(uintptr_t)dst, src, width * rows, hipMemcpyHostToDevice,
+                                             c->stream);
+        return e1 == hipSuccess ? 0 : fail("hipMemcpy H2D", e1);
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && pin_need(c, width * rows)) {
+        uint8_t* p = (uint8_t*)c->pin;
+        for (size_t r = 0; r < rows; ++r) { memcpy(p + r * width, (const uint8_t*)src + r * spitch, width); }
+        e = dpitch == width ? hipMemcpyAsync((void*)(uintptr_t)dst, p, width * rows, hipMemcpyHostToDevice, c->stream)
+                            : hipMemcpy2DAsync((void*)(uintptr_t)dst, dpitch, p, width, width, rows,
+                                               hipMemcpyHostToDevice, c->stream);
+    } else if (e == hipSuccess) {
+        e = hipMemcpy2DAsync((void*)(uintptr_t)dst, dpitch, src, spitch, width, rows, hipMemcpyHostToDevice,
+                             c->stream);
+    }
     return e == hipSuccess ? 0 : fail("hipMemcpy2D H2D", e);
 }
 
 int32_t tcg_d2h_2d(tcg_ctx* c, void* dst, size_t dpitch, uint64_t src, size_t spitch, size_t width, size_t rows)
 {
     if (width == 0u || rows == 0u) { return 0; }
-    hipError_t e = hipMemcpy2DAsync(dst, dpitch, (const void*)(uintptr_t)src, spitch, width, rows,
-                                    hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
+    if (spitch == width && dpitch == width) {
+        hipError_t e1 = hipMemcpyAsync(dst, (const void*)(uintptr_t)src, width * rows, hipMemcpyDeviceToHost,
+                                       c->stream);
+        if (e1 == hipSuccess) { e1 = hipStreamSynchronize(c->stream); }
+        return e1 == hipSuccess ? 0 : fail("hipMemcpy D2H", e1);
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess && pin_need(c, width * rows)) {
+        uint8_t* p = (uint8_t*)c->pin;
+        e = spitch == width ? hipMemcpyAsync(p, (const void*)(uintptr_t)src, width * rows, hipMemcpyDeviceToHost,
+                                             c->stream)
+                            : hipMemcpy2DAsync(p, width, (const void*)(uintptr_t)src, spitch, width, rows,
+                                               hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
+        if (e == hipSuccess) {
+            if (dpitch == width) {
+                memcpy(dst, p, width * rows);
+            } else {
+                for (size_t r = 0; r < rows; ++r) { memcpy((uint8_t*)dst + r * dpitch, p + r * width, width); }
+            }
+        }
+    } else if (e == hipSuccess) {
+        e = hipMemcpy2DAsync(dst, dpitch, (const void*)(uintptr_t)src, spitch, width, rows, hipMemcpyDeviceToHost,
+                             c->stream);
+        if (e == hipSuccess) { e = hipStreamSynchronize(c->stream); }
+    }
     return e == hipSuccess ? 0 : fail("hipMemcpy2D D2H", e);
 }
 
