@@ -224,6 +224,85 @@ int32_t tc_decoder_decode(tc_decoder* dec, const uint8_t* packet, size_t size,
 int32_t tc_decoder_decode_batch(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
                                 const topos_plane_view* out, topos_frame_output* infos);
 
+/* ---- reduced / scaled preview decode (reference topos_codec.h:341-380,
+ * 403-407, 441-497, 529-559) ----
+ * A target-size decode reads and checks every slice, reconstructs the frame
+ * keeping the coefficients a fixed ratio needs (reduced: scan positions up to
+ * 24 / 16 / 10 / 4 for 1/2, 1/3, 1/4, 1/8; scaled: all) and samples each
+ * target position from the nearest source sample (floor(t * source /
+ * target)); a deblocking stream is filtered on a 4-sample grid when the
+ * target is exactly half the source. Output planes: target size, 4:2:2
+ * chroma (w+1)/2 wide, RAW phase planes half size. */
+typedef enum tc_decode_scale {
+    TC_DECODE_SCALE_FULL = 0,
+    TC_DECODE_SCALE_HALF = 1,
+    TC_DECODE_SCALE_THIRD = 2,
+    TC_DECODE_SCALE_QUARTER = 3,
+    TC_DECODE_SCALE_EIGHTH = 4
+} tc_decode_scale;
+
+/* reference topos_codec.h:362-366: ceil(source / divisor) */
+int32_t tc_decode_scale_dimensions(uint32_t source_width, uint32_t source_height, tc_decode_scale scale,
+                                   uint32_t* target_width, uint32_t* target_height);
+/* reference topos_codec.h:345-349 */
+int32_t tc_frame_decode_scaled(const uint8_t* data, size_t size, uint32_t target_width, uint32_t target_height,
+                               uint16_t* const planes_out[TC_FRAME_MAX_PLANES],
+                               const size_t strides[TC_FRAME_MAX_PLANES], topos_frame_output* out_info);
+/* reference topos_codec.h:368-372 */
+int32_t tc_frame_decode_reduced(const uint8_t* data, size_t size, tc_decode_scale scale,
+                                uint16_t* const planes_out[TC_FRAME_MAX_PLANES],
+                                const size_t strides[TC_FRAME_MAX_PLANES], topos_frame_output* out_info);
+/* reference topos_codec.h:403-407 */
+int32_t tc_frame_decode_batch_reduced(const topos_batch_packet* packets, uint32_t count, tc_decode_scale scale,
+                                      uint16_t* const* planes_out, const size_t* strides,
+                                      topos_frame_output* infos);
+
+enum { TC_DECODE_MEMORY_CPU = 0u, TC_DECODE_MEMORY_HOST_VISIBLE_GPU = 1u };
+
+typedef enum topos_decode_mode {
+    TC_DECODE_MODE_FULL = 0u,
+    TC_DECODE_MODE_SCALED = 1u,
+    TC_DECODE_MODE_REDUCED = 2u,
+    TC_DECODE_MODE_AUTO_2K = 3u    /* sources above 2048 on a side: the first fixed ratio that fits */
+} topos_decode_mode;
+
+typedef enum tc_decode_quality {
+    TC_DECODE_QUALITY_DEFAULT = 0u,
+    TC_DECODE_QUALITY_FAST = 1u,
+    TC_DECODE_QUALITY_BALANCED = 2u,
+    TC_DECODE_QUALITY_HIGH = 3u
+} tc_decode_quality;
+
+/* reference topos_codec.h:465-476 */
+typedef struct topos_decode_request {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t mode;          /* topos_decode_mode */
+    uint32_t scale;         /* tc_decode_scale; REDUCED only */
+    uint32_t target_width;  /* SCALED / AUTO_2K; 0 together with height */
+    uint32_t target_height;
+    uint32_t quality;       /* tc_decode_quality; checked and kept */
+    uint32_t memory_type;   /* TC_DECODE_MEMORY_* */
+    uint32_t flags;         /* TC_DECODE_FLAG_* */
+    uint32_t reserved[4];   /* must be 0 */
+} topos_decode_request;
+
+#define TC_DECODE_FLAG_DROP_ALPHA (1u << 0)
+
+/* reference topos_codec.h:488-493: planes_out == NULL is query mode */
+int32_t tc_frame_decode_request(const uint8_t* data, size_t size, const topos_decode_request* request,
+                                uint16_t* const planes_out[TC_FRAME_MAX_PLANES],
+                                const size_t strides[TC_FRAME_MAX_PLANES], topos_frame_output* out_info);
+/* reference topos_codec.h:529-532, 541-545, 555-559 */
+int32_t tc_decoder_prepare_request(tc_decoder* dec, const uint8_t* packet, size_t size,
+                                   const topos_decode_request* request, topos_frame_output* info);
+int32_t tc_decoder_decode_request(tc_decoder* dec, const uint8_t* packet, size_t size,
+                                  const topos_decode_request* request,
+                                  const topos_plane_view out[TC_FRAME_MAX_PLANES], topos_frame_output* info);
+int32_t tc_decoder_decode_batch_request(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
+                                        const topos_decode_request* request, const topos_plane_view* out,
+                                        topos_frame_output* infos);
+
 /* ---- packed RGB(A) -> planar conversion (reference topos_codec.h:1016-1102) ----
  * The step before encode in every export: packed 8-bit or 16-bit RGB(A)
  * pixels to YUV 4:2:2 / 4:4:4 or GBR planes of 10 or 12 bits, float32

@@ -49,6 +49,13 @@ int32_t tcx_decode_batch_dev(uint32_t n, const uint8_t* const* host_packets, con
                              const uint64_t* dev_packets, const uint64_t* dev_planes,
                              const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs);
 
+/* device-resident batch decode through one decode request (reduced, scaled,
+ * AUTO_2K, DROP_ALPHA): dev_planes receive the request's target geometry */
+int32_t tcx_decode_batch_dev_request(uint32_t n, const uint8_t* const* host_packets, const size_t* sizes,
+                                     const uint64_t* dev_packets, const uint64_t* dev_planes,
+                                     const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs,
+                                     const topos_decode_request* request);
+
 /* ---- target-size encode of many frames -----------------------------------
  * Each request runs the search of tc_frame_encode_sized on its own frame
  * (device planes in, device packet out) with its own seed state; the

@@ -180,8 +180,9 @@ int32_t tc_image_validate(const topos_io* io, uint32_t flags, topos_image_info* 
 /* reference topos_image.h:219-221: the PIXL packet through tc_frame_decode */
 int32_t tc_image_decode(const topos_io* io, const topos_plane_view planes[TC_FRAME_MAX_PLANES],
                         topos_frame_output* out_info);
-/* reference topos_image.h:231-235: reduced-size decode; this backend returns
- * TC_ERR_NOT_IMPLEMENTED after the argument and envelope checks */
+/* reference topos_image.h:231-235: target-size decode of the PIXL packet
+ * (the reduced request when the target is a fixed ratio of the source, else
+ * the scaled one); out_info holds the target geometry */
 int32_t tc_image_decode_preview(const topos_io* io, uint32_t target_width, uint32_t target_height,
                                 const topos_plane_view planes[TC_FRAME_MAX_PLANES], topos_frame_output* out_info);
 /* reference topos_image.h:239-240 */

@@ -1087,6 +1087,7 @@ int32_t orc_frame_encode(const orc_cfg* cfg, const uint16_t* const planes[4],
 
 typedef struct {
     uint16_t* dst; size_t stride; uint32_t vis_w, vis_h;
+    uint32_t limit;   /* highest kept scan position (63: all); reduced decode */
 } plane_out;
 
 /* dequantise + inverse transform one block and store its visible samples
@@ -1096,7 +1097,16 @@ static void store_block(const plane_out* po, const int32_t blk[64], const uint16
 {
     const int32_t mid = 1 << (bd - 1), maxv = (1 << bd) - 1;
     int32_t xh[64];
-    orc_dequant_idct(blk, qm, qp, tbl, bd, xh);
+    if (po->limit < 63u) {
+        /* reduced decode keeps scan positions 0..limit (native/src/codec/
+         * color_store.h:723-757, native/src/entropy/color_scan.h:395-420) */
+        int32_t low[64];
+        for (int i = 0; i < 64; ++i) { low[i] = 0; }
+        for (uint32_t k = 0; k <= po->limit; ++k) { low[orc_zigzag[k]] = blk[orc_zigzag[k]]; }
+        orc_dequant_idct(low, qm, qp, tbl, bd, xh);
+    } else {
+        orc_dequant_idct(blk, qm, qp, tbl, bd, xh);
+    }
     const uint32_t px0 = bx * 8u, py0 = brow * 8u;
     for (uint32_t yy = 0; yy < 8u && py0 + yy < po->vis_h; ++yy) {
         for (uint32_t xx = 0; xx < 8u && px0 + xx < po->vis_w; ++xx) {
@@ -1305,15 +1315,16 @@ static int decode_alpha(const orc_hdr* h, const orc_slice* s, const uint8_t* pl,
     return rc;
 }
 
-int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes[4],
-                         const size_t strides[4], uint8_t* slice_status, uint32_t* concealed)
+/* slice loop of a frame decode into planes of the source geometry: CRC,
+ * entropy decode, reconstruction (coefficients up to scan position limit),
+ * concealment of failed slices; alpha slices are skipped (status OK) when
+ * skip_alpha (native/src/codec/codec.c:8206-8515, 8581-8644) */
+static int32_t decode_core(const orc_hdr* hp, const uint8_t* pkt, size_t size, uint16_t* const planes[4],
+                           const size_t strides[4], uint8_t* slice_status, uint32_t* nconc_out, uint32_t limit,
+                           int skip_alpha)
 {
-    orc_hdr h;
-    int rc = hdr_read(pkt, size, &h);
-    if (rc) { return rc; }
+    const orc_hdr h = *hp;
     const int bits = h.major == 1u || (h.major == 2u && h.entropy_mode == 1u);
-    if (!(h.major == 7u && h.entropy_mode == 7u) && !bits) { return ORC_ENOTIMPL; }
-    if (h.packet_size != size) { return ORC_EMALFORMED; }
     uint32_t nconc = 0;
     size_t off = 53;
     for (uint32_t i = 0; i < h.slice_count; ++i) {
@@ -1327,7 +1338,11 @@ int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes
         const uint8_t* pl = pkt + off;
         off += s.size;
         const uint32_t p = s.plane;
-        plane_out po = { planes[p], strides && strides[p] ? strides[p] : h.pvw[p], h.pvw[p], h.pvh[p] };
+        if (skip_alpha && is_alpha(&h, p)) {
+            if (slice_status) { slice_status[i] = 0u; }
+            continue;
+        }
+        plane_out po = { planes[p], strides && strides[p] ? strides[p] : h.pvw[p], h.pvw[p], h.pvh[p], limit };
         int src = orc_crc32(pl, s.size) == s.crc ? 0 : ORC_ECRC;
         if (src == 0) {
             src = is_alpha(&h, p) ? decode_alpha(&h, &s, pl, &po)
@@ -1346,6 +1361,36 @@ int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes
         }
     }
     if (off != size) { return ORC_EMALFORMED; }
+    *nconc_out = nconc;
+    return ORC_OK;
+}
+
+static int32_t decode_header(const uint8_t* pkt, size_t size, orc_hdr* h)
+{
+    int rc = hdr_read(pkt, size, h);
+    if (rc) { return rc; }
+    const int bits = h->major == 1u || (h->major == 2u && h->entropy_mode == 1u);
+    if (!(h->major == 7u && h->entropy_mode == 7u) && !bits) { return ORC_ENOTIMPL; }
+    if (h->packet_size != size) { return ORC_EMALFORMED; }
+    return ORC_OK;
+}
+
+int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes[4],
+                         const size_t strides[4], uint8_t* slice_status, uint32_t* concealed)
+{
+    return orc_frame_decode_ex(pkt, size, planes, strides, slice_status, concealed, 0);
+}
+
+/* full-size decode; skip_alpha leaves plane 3 out (decode request flag) */
+int32_t orc_frame_decode_ex(const uint8_t* pkt, size_t size, uint16_t* const planes[4], const size_t strides[4],
+                            uint8_t* slice_status, uint32_t* concealed, int32_t skip_alpha)
+{
+    orc_hdr h;
+    int rc = decode_header(pkt, size, &h);
+    if (rc) { return rc; }
+    uint32_t nconc = 0;
+    rc = decode_core(&h, pkt, size, planes, strides, slice_status, &nconc, 63u, skip_alpha && h.alpha_mode != 0u);
+    if (rc) { return rc; }
     if (h.flags & 4u) {
         for (uint32_t p = 0; p < h.plane_count && p < 3u; ++p) {
             const size_t st = strides && strides[p] ? strides[p] : h.pvw[p];
@@ -1353,6 +1398,59 @@ int32_t orc_frame_decode(const uint8_t* pkt, size_t size, uint16_t* const planes
                               h.bit_depth);
         }
     }
+    if (concealed) { *concealed = nconc; }
+    return nconc ? ORC_WARN_CONCEALED : ORC_OK;
+}
+
+/* Target-size decode (tc_frame_decode_scaled / _reduced / _request; native/
+ * src/codec/codec.c:9100-9172, 9505-9640, 8581-8647 and color_store.h:
+ * 104-288): the frame is reconstructed at source size keeping coefficients up
+ * to scan position limit, then each target sample (tx, ty) of a plane takes
+ * the source sample (floor(tx*vw/dw), floor(ty*vh/dh)); target plane sizes
+ * are (tw, th), 4:2:2 chroma ((tw+1)/2, th), RAW phase planes ((tw+1)/2,
+ * (th+1)/2). With the deblock flag and an exact 2:1 target (tw*2 = W and
+ * th*2 = H) the colour target planes are filtered on a 4-sample grid
+ * (codec.c:8649-8660, deblock.c:304-340). skip_alpha drops plane 3. */
+int32_t orc_frame_decode_scaled(const uint8_t* pkt, size_t size, uint32_t tw, uint32_t th, uint32_t limit,
+                                int32_t skip_alpha, uint16_t* const planes[4], const size_t strides[4],
+                                uint8_t* slice_status, uint32_t* concealed)
+{
+    orc_hdr h;
+    int rc = decode_header(pkt, size, &h);
+    if (rc) { return rc; }
+    if (tw == 0u || th == 0u || tw > h.visible_w || th > h.visible_h) { return ORC_EINVAL; }
+    uint16_t* src[4] = { NULL, NULL, NULL, NULL };
+    for (uint32_t p = 0; p < h.plane_count; ++p) {
+        src[p] = (uint16_t*)calloc((size_t)h.pvw[p] * h.pvh[p], 2u);
+        if (src[p] == NULL) { rc = ORC_ENOMEM; }
+    }
+    uint32_t nconc = 0;
+    const int drop = skip_alpha && h.alpha_mode != 0u;
+    if (rc == 0) { rc = decode_core(&h, pkt, size, src, NULL, slice_status, &nconc, limit, drop); }
+    const int exact_half = tw * 2u == h.visible_w && th * 2u == h.visible_h;
+    for (uint32_t p = 0; p < h.plane_count && rc == 0; ++p) {
+        if (drop && p == 3u) { continue; }
+        uint32_t dw = tw, dh = th;
+        if (h.pixel_format == 3u) {
+            dw = (tw + 1u) / 2u; dh = (th + 1u) / 2u;
+        } else if (h.pixel_format == 0u && (p == 1u || p == 2u)) {
+            dw = (tw + 1u) / 2u;
+        }
+        const size_t ds = strides && strides[p] ? strides[p] : dw;
+        const uint32_t vw = h.pvw[p], vh = h.pvh[p];
+        for (uint32_t ty = 0; ty < dh; ++ty) {
+            const uint32_t sy = (uint32_t)(((uint64_t)ty * vh) / dh);
+            for (uint32_t tx = 0; tx < dw; ++tx) {
+                const uint32_t sx = (uint32_t)(((uint64_t)tx * vw) / dw);
+                planes[p][(size_t)ty * ds + tx] = src[p][(size_t)sy * vw + sx];
+            }
+        }
+        if ((h.flags & 4u) && p < 3u && exact_half) {
+            orc_deblock_plane_unit(planes[p], ds, dw, dh, h.qp_base, (h.flags & 8u) ? 1u : 0u, h.bit_depth, 4u);
+        }
+    }
+    for (uint32_t p = 0; p < 4u; ++p) { free(src[p]); }
+    if (rc) { return rc; }
     if (concealed) { *concealed = nconc; }
     return nconc ? ORC_WARN_CONCEALED : ORC_OK;
 }
@@ -1382,12 +1480,14 @@ static void db_edge(uint16_t* p1, uint16_t* p0, uint16_t* q0, uint16_t* q1,
     *q1 = (uint16_t)clampv(e - ds, maxv);
 }
 
-/* Output 8x8-grid weak filter: all vertical edges, then all horizontal edges
- * (native/src/codec/deblock.c:71-157,163-212,304-334). */
-void orc_deblock_plane(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uint32_t qp,
-                       uint32_t tbl, uint32_t bd)
+/* Output weak filter on a grid of unit samples (8 for full-size output, 4 on
+ * an exact 2:1 target): all vertical edges, then all horizontal edges
+ * (native/src/codec/deblock.c:71-157,163-212,304-340). */
+void orc_deblock_plane_unit(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uint32_t qp, uint32_t tbl,
+                            uint32_t bd, uint32_t unit)
 {
-    if (pix == NULL || qp > 95u || w < 16u || h < 16u || (w % 8u) || (h % 8u) || bd < 8u || bd > 16u) {
+    if (pix == NULL || qp > 95u || unit < 4u || unit > 8u || w < 2u * unit || h < 2u * unit || (w % unit) ||
+        (h % unit) || bd < 8u || bd > 16u) {
         return;
     }
     const int32_t maxv = (int32_t)((1u << bd) - 1u);
@@ -1399,18 +1499,24 @@ void orc_deblock_plane(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uin
     const int32_t alpha = (int32_t)al, beta = alpha / 2 + 2, clip = alpha >> 1;
     for (uint32_t y = 0; y < h; ++y) {
         uint16_t* r = pix + (size_t)y * stride;
-        for (uint32_t x0 = 8; x0 + 1u < w; x0 += 8) {
+        for (uint32_t x0 = unit; x0 + 1u < w; x0 += unit) {
             db_edge(r + x0 - 2, r + x0 - 1, r + x0, r + x0 + 1, alpha, beta, clip, maxv);
         }
     }
-    for (uint32_t rr = 1; rr < h / 8u; ++rr) {
-        const uint32_t y0 = rr * 8u;
+    for (uint32_t rr = 1; rr < h / unit; ++rr) {
+        const uint32_t y0 = rr * unit;
         for (uint32_t x = 0; x < w; ++x) {
             db_edge(pix + (size_t)(y0 - 2) * stride + x, pix + (size_t)(y0 - 1) * stride + x,
                     pix + (size_t)y0 * stride + x, pix + (size_t)(y0 + 1) * stride + x,
                     alpha, beta, clip, maxv);
         }
     }
+}
+
+void orc_deblock_plane(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uint32_t qp, uint32_t tbl,
+                       uint32_t bd)
+{
+    orc_deblock_plane_unit(pix, stride, w, h, qp, tbl, bd, 8u);
 }
 
 /* -------------------------------------------------- test image synthesiser */

@@ -91,5 +91,13 @@ void orc_sized_put_state(const uint32_t s[5]);
 
 void orc_deblock_plane(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uint32_t qp,
                        uint32_t tbl, uint32_t bit_depth);
+void orc_deblock_plane_unit(uint16_t* pix, size_t stride, uint32_t w, uint32_t h, uint32_t qp, uint32_t tbl,
+                            uint32_t bd, uint32_t unit);
+int32_t orc_frame_decode_ex(const uint8_t* pkt, size_t size, uint16_t* const planes[4], const size_t strides[4],
+                            uint8_t* slice_status, uint32_t* concealed, int32_t skip_alpha);
+/* target-size decode: limit = highest kept scan position (63 all) */
+int32_t orc_frame_decode_scaled(const uint8_t* pkt, size_t size, uint32_t tw, uint32_t th, uint32_t limit,
+                                int32_t skip_alpha, uint16_t* const planes[4], const size_t strides[4],
+                                uint8_t* slice_status, uint32_t* concealed);
 
 #endif

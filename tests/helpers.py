@@ -222,3 +222,32 @@ def lib_sized(lib, abi, cfg, planes, target, qp_min, qp_max, cap=None):
                                    ctypes.c_uint8(qp_min), ctypes.c_uint8(qp_max), ctypes.byref(q),
                                    out if cap else None, ctypes.c_size_t(cap), ctypes.byref(st))
     return rc, q.value, out.raw[: st.packet_size] if rc == 0 else b"", st.packet_size
+
+
+# ---------------------------------------------------------------- preview decode
+def orc_scaled(oracle, packet, tw, th, limit, dims, skip_alpha=0):
+    """Oracle target-size decode into planes of dims [(w, h)]: (rc, planes, status, concealed)."""
+    outs = [np.zeros((h, w), np.uint16) for (w, h) in dims]
+    ptrs = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None] * (4 - len(outs))))
+    st = (ctypes.c_uint8 * 512)()
+    nc = ctypes.c_uint32(0)
+    rc = oracle.orc_frame_decode_scaled(packet, ctypes.c_size_t(len(packet)), ctypes.c_uint32(tw),
+                                        ctypes.c_uint32(th), ctypes.c_uint32(limit), ctypes.c_int32(skip_alpha),
+                                        ptrs, None, st, ctypes.byref(nc))
+    return rc, outs, list(st), nc.value
+
+
+def lib_decode_request(lib, abi, packet, request):
+    """tc_frame_decode_request of a library: query, then decode into tight
+    planes of the reported geometry. (rc, planes, info)."""
+    info = abi.FrameOutput()
+    rc = lib.tc_frame_decode_request(packet, ctypes.c_size_t(len(packet)), ctypes.byref(request), None, None,
+                                     ctypes.byref(info))
+    if rc < 0:
+        return rc, [], info
+    dims = abi.target_plane_dims(info.visible_width, info.visible_height, info.pixel_format, info.plane_count)
+    outs = [np.full((h, w), 0xABCD, np.uint16) for (w, h) in dims]
+    ptrs = (ctypes.c_void_p * 4)(*([o.ctypes.data for o in outs] + [None] * (4 - len(outs))))
+    rc = lib.tc_frame_decode_request(packet, ctypes.c_size_t(len(packet)), ctypes.byref(request), ptrs, None,
+                                     ctypes.byref(info))
+    return rc, outs, info

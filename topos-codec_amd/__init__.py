@@ -92,6 +92,12 @@ def load_library(path=LIB_PATH):
     lib.tc_frame_encode_sized.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint8, ctypes.c_uint8, vp, vp,
                                           ctypes.c_size_t, vp]
     lib.tcx_encode_sized_batch_dev.argtypes = [vp, ctypes.c_uint32]
+    lib.tc_frame_decode_request.argtypes = [vp, ctypes.c_size_t, vp, vp, vp, vp]
+    lib.tc_frame_decode_reduced.argtypes = [vp, ctypes.c_size_t, ctypes.c_uint32, vp, vp, vp]
+    lib.tc_frame_decode_scaled.argtypes = [vp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    lib.tc_frame_decode_batch_reduced.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    lib.tc_decode_scale_dimensions.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]
+    lib.tcx_decode_batch_dev_request.argtypes = [ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.tcx_sized_reset.argtypes = []
     lib.tcx_sized_reset.restype = None
     lib.tcx_sized_set_mode.argtypes = [ctypes.c_int32]
@@ -417,6 +423,22 @@ class ToposImageCodec:
         out.abi_version = abi.TOPOS_CODEC_ABI_VERSION
         rc = self._lib.tc_image_decode(io.ref(), ctypes.cast(views, ctypes.c_void_p), ctypes.byref(out))
         return rc, arrays, out
+
+    def decode_preview_status(self, src, target_width, target_height, arrays):
+        """tc_image_decode_preview into the given uint16 arrays (one per output
+        plane, target geometry): (status, frame output)."""
+        io = _image_source(src)
+        views = (abi.PlaneView * abi.TC_FRAME_MAX_PLANES)()
+        for p in range(abi.TC_FRAME_MAX_PLANES):
+            views[p].struct_size = ctypes.sizeof(abi.PlaneView)
+            views[p].abi_version = abi.TOPOS_CODEC_ABI_VERSION
+        for p, arr in enumerate(arrays):
+            views[p].pixels = arr.ctypes.data
+            views[p].stride = arr.strides[0] // 2
+        out = abi.FrameOutput()
+        rc = self._lib.tc_image_decode_preview(io.ref(), target_width, target_height, ctypes.cast(views, ctypes.c_void_p),
+                                               ctypes.byref(out))
+        return rc, out
 
     def decode(self, src):
         rc, arrays, out = self.decode_status(src)

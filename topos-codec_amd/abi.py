@@ -384,3 +384,35 @@ class TcxSizedReq(ctypes.Structure):
         ("rc", ctypes.c_int32), ("qp_used", ctypes.c_uint8), ("packet_size", ctypes.c_uint32),
         ("encodes", ctypes.c_uint32), ("stats", FrameStats),
     ]
+
+
+# ---- reduced / scaled preview decode (include/topos_codec.h) ----
+DECODE_SCALE_FULL, DECODE_SCALE_HALF, DECODE_SCALE_THIRD, DECODE_SCALE_QUARTER, DECODE_SCALE_EIGHTH = range(5)
+DECODE_MODE_FULL, DECODE_MODE_SCALED, DECODE_MODE_REDUCED, DECODE_MODE_AUTO_2K = range(4)
+DECODE_FLAG_DROP_ALPHA = 1
+SCALE_DIVISOR = {0: 1, 1: 2, 2: 3, 3: 4, 4: 8}
+SCALE_LIMIT = {0: 63, 1: 24, 2: 16, 3: 10, 4: 4}
+
+
+class DecodeRequest(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32), ("mode", ctypes.c_uint32),
+                ("scale", ctypes.c_uint32), ("target_width", ctypes.c_uint32), ("target_height", ctypes.c_uint32),
+                ("quality", ctypes.c_uint32), ("memory_type", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+def make_request(mode, scale=0, target=(0, 0), flags=0):
+    r = DecodeRequest()
+    r.struct_size = ctypes.sizeof(DecodeRequest)
+    r.abi_version = TOPOS_CODEC_ABI_VERSION
+    r.mode, r.scale, r.flags = mode, scale, flags
+    r.target_width, r.target_height = target
+    return r
+
+
+def target_plane_dims(tw, th, pixel_format, plane_count):
+    """Output plane sizes of a target-size decode."""
+    if pixel_format == 3:
+        return [((tw + 1) // 2, (th + 1) // 2)] * plane_count
+    cw = (tw + 1) // 2 if pixel_format == 0 else tw
+    return [(tw, th), (cw, th), (cw, th), (tw, th)][:plane_count]

@@ -28,7 +28,7 @@ typedef struct enc_frame_plan {
 struct tc_ectx {
     pthread_mutex_t mu;
     tcg_ctx* g;
-    dbuf in, q, dcm, scratch, pkt, desc, res, out, tmp;
+    dbuf in, q, dcm, scratch, pkt, desc, res, out, tmp, src, tgt;
     uint8_t* hdesc;    /* pinned staging for descriptor uploads */
     size_t hdesc_cap;
     uint8_t* hres;     /* pinned staging for result downloads */
@@ -536,15 +536,41 @@ int32_t tc_engine_encode(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
 
 /* ------------------------------------------------------------------ decode */
 
+void tc_target_plane_dims(const tc_fhdr* fh, uint32_t tw, uint32_t th, uint32_t p, uint32_t* w, uint32_t* h)
+{
+    if (fh->pixel_format == 3u) {
+        *w = (tw + 1u) / 2u;
+        *h = (th + 1u) / 2u;
+    } else {
+        *w = fh->pixel_format == 0u && (p == 1u || p == 2u) ? (tw + 1u) / 2u : tw;
+        *h = th;
+    }
+}
+
+/* output stride of plane p of a request (elements) */
+static size_t dec_out_stride(const tc_dec_req* r, uint32_t p, uint32_t w)
+{
+    if (r->dev_planes != NULL) { return r->dev_strides && r->dev_strides[p] ? r->dev_strides[p] : w; }
+    return w;
+}
+
+/* A target-size request decodes every plane at source size into `src`
+ * (coefficients up to the request's limit, no deblocking), then a target
+ * descriptor samples it into the output (through `tgt` and the deblock pass
+ * on a 4-sample grid when the target is exactly 2:1 and the stream deblocks).
+ * A dropped alpha plane decodes into `src` and is not output. */
 int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
 {
     c->out_dev = NULL;
-    uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, maxrows = 1, ndb = 0;
-    size_t pkt_bytes = 0, out_bytes = 0, tmp_bytes = 0;
+    uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, maxrows = 1, nscaled = 0;
+    size_t pkt_bytes = 0, out_bytes = 0, tmp_bytes = 0, src_bytes = 0, tgt_bytes = 0;
     for (uint32_t f = 0; f < n; ++f) {
-        const tc_fhdr* fh = &reqs[f].view->fh;
-        nslices += reqs[f].view->nslices;
-        if (reqs[f].dev_data == 0u) { pkt_bytes += al256(reqs[f].size + 1024u); }
+        const tc_dec_req* r = &reqs[f];
+        const tc_fhdr* fh = &r->view->fh;
+        const int scaled = r->tw != 0u;
+        nscaled += scaled ? 1u : 0u;
+        nslices += r->view->nslices;
+        if (r->dev_data == 0u) { pkt_bytes += al256(r->size + 1024u); }
         const int db = (fh->flags & 4u) != 0u;
         for (uint32_t p = 0; p < fh->plane_count; ++p) {
             nplanes++;
@@ -554,9 +580,16 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                 const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
                 ntiles = pt > ntiles ? pt : ntiles;
             }
-            const size_t pb = al256((size_t)fh->pvw[p] * fh->pvh[p] * 2u);
-            if (reqs[f].dev_planes == NULL) { out_bytes += pb; }
-            if (db && p < 3u) { tmp_bytes += pb; }
+            const int dropped = r->skip_alpha && tc_is_alpha(fh, p);
+            uint32_t ow = fh->pvw[p], oh = fh->pvh[p];
+            if (scaled) { tc_target_plane_dims(fh, r->tw, r->th, p, &ow, &oh); }
+            if (scaled || dropped) { src_bytes += al256((size_t)fh->pvw[p] * fh->pvh[p] * 2u); }
+            if (dropped) { continue; }
+            const size_t ost = dec_out_stride(r, p, ow);
+            if (r->dev_planes == NULL) { out_bytes += al256((size_t)ow * oh * 2u); }
+            if (db && p < 3u) {
+                if (scaled) { tgt_bytes += al256(ost * oh * 2u); } else { tmp_bytes += al256(ost * oh * 2u); }
+            }
         }
     }
     const size_t o_q = 0;
@@ -564,7 +597,8 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     const size_t o_sl = al256(o_pl + (size_t)nplanes * sizeof(tcg_dplane));
     const size_t o_rs = al256(o_sl + (size_t)nslices * sizeof(tcg_dslice));
     const size_t o_wk = al256(o_rs + (size_t)nplanes * maxrows * sizeof(tcg_rowrec));
-    const size_t desc_bytes = al256(o_wk + sizeof(tcg_work) + (size_t)nslices * sizeof(uint32_t));
+    const size_t o_tg = al256(o_wk + sizeof(tcg_work) + (size_t)nslices * sizeof(uint32_t));
+    const size_t desc_bytes = al256(o_tg + (nscaled ? (size_t)nplanes * sizeof(tcg_dplane) : 0u));
     const size_t res_bytes = al256((size_t)nslices * sizeof(int32_t));
     int32_t rc;
     if ((rc = dbuf_need(c, &c->q, (size_t)nblocks * 64u * 4u)) != TC_OK ||
@@ -577,14 +611,17 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     if (pkt_bytes && (rc = dbuf_need(c, &c->pkt, pkt_bytes)) != TC_OK) { return rc; }
     if (out_bytes && (rc = dbuf_need(c, &c->out, out_bytes)) != TC_OK) { return rc; }
     if (tmp_bytes && (rc = dbuf_need(c, &c->tmp, tmp_bytes)) != TC_OK) { return rc; }
+    if (src_bytes && (rc = dbuf_need(c, &c->src, src_bytes)) != TC_OK) { return rc; }
+    if (tgt_bytes && (rc = dbuf_need(c, &c->tgt, tgt_bytes)) != TC_OK) { return rc; }
 
     memset(c->hdesc, 0, desc_bytes);
     tcg_qctx* hq = (tcg_qctx*)(c->hdesc + o_q);
     tcg_dplane* hp = (tcg_dplane*)(c->hdesc + o_pl);
     tcg_dslice* hs = (tcg_dslice*)(c->hdesc + o_sl);
     tcg_rowrec* hrs = (tcg_rowrec*)(c->hdesc + o_rs);
-    uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0;
-    size_t pkt_off = 0, out_off = 0, tmp_off = 0;
+    tcg_dplane* ht = nscaled ? (tcg_dplane*)(c->hdesc + o_tg) : NULL;
+    uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0, max_px = 0, tdbt = 0;
+    size_t pkt_off = 0, out_off = 0, tmp_off = 0, src_off = 0, tgt_off = 0;
     tcg_batch_info info;
     memset(&info, 0, sizeof(info));
     info.max_cols = 1u;
@@ -596,6 +633,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         tc_dec_req* r = &reqs[f];
         const tc_view* v = r->view;
         const tc_fhdr* fh = &v->fh;
+        const int scaled = r->tw != 0u;
         uint64_t pk = r->dev_data;
         if (pk == 0u) {
             pk = c->pkt.p + pkt_off;
@@ -608,26 +646,42 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         }
         const uint32_t tbl = (fh->flags & 8u) ? 1u : 0u;
         const int db = (fh->flags & 4u) != 0u;
+        const int half = scaled && r->tw * 2u == fh->visible_w && r->th * 2u == fh->visible_h;
         const uint32_t ip0 = ip;
         for (uint32_t p = 0; p < fh->plane_count; ++p) {
             tcg_dplane* D = &hp[ip + p];
             const uint32_t vw = fh->pvw[p], vh = fh->pvh[p];
-            if (r->dev_planes != NULL) {
-                D->dst = r->dev_planes[p];
-                D->stride = (uint32_t)(r->dev_strides && r->dev_strides[p] ? r->dev_strides[p] : vw);
-            } else {
-                D->dst = c->out.p + out_off;
-                D->stride = vw;
-                out_off += al256((size_t)vw * vh * 2u);
+            const int dropped = r->skip_alpha && tc_is_alpha(fh, p);
+            uint32_t ow = vw, oh = vh;
+            if (scaled) { tc_target_plane_dims(fh, r->tw, r->th, p, &ow, &oh); }
+            const uint32_t ost = (uint32_t)dec_out_stride(r, p, ow);
+            uint64_t final_dst = 0u;
+            if (!dropped) {
+                if (r->dev_planes != NULL) {
+                    final_dst = r->dev_planes[p];
+                } else {
+                    final_dst = c->out.p + out_off;
+                    out_off += al256((size_t)ow * oh * 2u);
+                }
+                out_dev[(size_t)f * 4u + p] = final_dst;
             }
-            out_dev[(size_t)f * 4u + p] = D->dst;
+            if (scaled || dropped) {
+                D->dst = c->src.p + src_off;
+                D->stride = vw;
+                src_off += al256((size_t)vw * vh * 2u);
+            } else {
+                D->dst = final_dst;
+                D->stride = ost;
+            }
             D->vis_w = vw; D->vis_h = vh;
             D->bcols = fh->bcols[p]; D->brows = fh->brows[p];
             D->blk0 = blk;
             D->row_slice = row;
             D->coded_w = fh->pcw[p];
             D->tile0 = tile;
-            if (tc_is_alpha(fh, p)) {
+            D->limit = scaled ? r->limit : 63u;
+            const int colour = !tc_is_alpha(fh, p);
+            if (!colour) {
                 D->kind = 1u;
                 info.kinds |= TCG_HAS_ALPHA;
                 D->fill = 0xFFFFu;
@@ -640,15 +694,42 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                 D->fill = 1u << (fh->bit_depth - 1u);
                 D->maxv = (1u << fh->bit_depth) - 1u;
                 tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
-                if (db && p < 3u) {
+                if (db && p < 3u && !scaled) {
                     D->tmp = c->tmp.p + tmp_off;
-                    tmp_off += al256((size_t)vw * vh * 2u);
+                    tmp_off += al256((size_t)ost * vh * 2u);
                     if (vw % 8u == 0u && vh % 8u == 0u && vw >= 16u && vh >= 16u && fh->qp_base <= 95u &&
                         fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
                         D->deblock = 1u;
+                        D->db_unit = 8u;
                         D->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
                     } else {
                         D->tmp = 0u;
+                    }
+                }
+            }
+            if (ht != NULL) {
+                tcg_dplane* T = &ht[ip + p];
+                if (scaled && !dropped) {
+                    T->sample = 1u;
+                    T->dst = final_dst;
+                    T->stride = ost;
+                    T->vis_w = ow; T->vis_h = oh;
+                    T->maxv = D->maxv;
+                    T->kind = D->kind;
+                    max_px = ow * oh > max_px ? ow * oh : max_px;
+                    if (colour && db && p < 3u && half) {
+                        T->tmp = c->tgt.p + tgt_off;
+                        tgt_off += al256((size_t)ost * oh * 2u);
+                        if (ow % 4u == 0u && oh % 4u == 0u && ow >= 8u && oh >= 8u && fh->qp_base <= 95u &&
+                            fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
+                            T->deblock = 1u;
+                            T->db_unit = 4u;
+                            T->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
+                            const uint32_t t = (oh / 4u + 1u) * ((ow + 255u) / 256u);
+                            tdbt = t > tdbt ? t : tdbt;
+                        } else {
+                            T->tmp = 0u;
+                        }
                     }
                 }
             }
@@ -713,7 +794,6 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             dbt = t > dbt ? t : dbt;
         }
     }
-    (void)ndb;
     (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
     free(wkeys);
     info.work = (tcg_work*)(uintptr_t)(c->desc.p + o_wk);
@@ -729,6 +809,12 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                    (int32_t*)(uintptr_t)c->res.p, dbt, &info) != 0) {
         free(out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode launch failed: %s", tcg_last_error());
+    }
+    if (ht != NULL && tcg_decode_sample(c->g, (const tcg_dplane*)(uintptr_t)(c->desc.p + o_pl),
+                                        (const tcg_dplane*)(uintptr_t)(c->desc.p + o_tg), nplanes, max_px,
+                                        tdbt) != 0) {
+        free(out_dev);
+        return tc_fail(TC_ERR_OUT_OF_MEMORY, "target-size sample launch failed: %s", tcg_last_error());
     }
     if (tcg_d2h(c->g, c->hres, c->res.p, (size_t)nslices * sizeof(int32_t)) != 0) {
         free(out_dev);
@@ -755,9 +841,12 @@ int32_t tc_engine_decode_finish(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     for (uint32_t f = 0; f < n; ++f) {
         tc_dec_req* r = &reqs[f];
         const tc_view* v = r->view;
+        const tc_fhdr* fh = &v->fh;
         uint32_t conc = 0;
         for (uint32_t i = 0; i < v->nslices; ++i) {
-            const uint8_t s = st[sbase + i] != 0 ? TC_FRAME_SLICE_CONCEALED : TC_FRAME_SLICE_OK;
+            /* a dropped alpha plane's slices are not decoded for the caller: status OK */
+            const int dropped = r->skip_alpha && tc_is_alpha(fh, v->sl[i].plane);
+            const uint8_t s = st[sbase + i] != 0 && !dropped ? TC_FRAME_SLICE_CONCEALED : TC_FRAME_SLICE_OK;
             if (r->info) { r->info->slice_status[i] = s; }
             conc += s;
         }
@@ -765,11 +854,13 @@ int32_t tc_engine_decode_finish(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         if (r->info) { r->info->concealed_slices = (uint16_t)conc; }
         r->rc = conc ? tc_fail(TC_WARN_CONCEALED, "%u of %u slices concealed", conc, (unsigned)v->nslices) : TC_OK;
         if (r->dev_planes == NULL && r->planes != NULL) {
-            for (uint32_t p = 0; p < v->fh.plane_count; ++p) {
-                const size_t vw = v->fh.pvw[p], vh = v->fh.pvh[p];
-                const size_t dst_stride = r->strides && r->strides[p] ? r->strides[p] : vw;
-                if (tcg_d2h_2d(c->g, r->planes[p], dst_stride * 2u, out_dev[(size_t)f * 4u + p], vw * 2u, vw * 2u,
-                               vh) != 0) {
+            for (uint32_t p = 0; p < fh->plane_count; ++p) {
+                if (r->skip_alpha && tc_is_alpha(fh, p)) { continue; }
+                uint32_t ow = fh->pvw[p], oh = fh->pvh[p];
+                if (r->tw != 0u) { tc_target_plane_dims(fh, r->tw, r->th, p, &ow, &oh); }
+                const size_t dst_stride = r->strides && r->strides[p] ? r->strides[p] : ow;
+                if (tcg_d2h_2d(c->g, r->planes[p], dst_stride * 2u, out_dev[(size_t)f * 4u + p], (size_t)ow * 2u,
+                               (size_t)ow * 2u, oh) != 0) {
                     r->rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "plane download: %s", tcg_last_error());
                     break;
                 }

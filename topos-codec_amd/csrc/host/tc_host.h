@@ -150,8 +150,15 @@ typedef struct tc_dec_req {
     const size_t* dev_strides;
     topos_frame_output* info;
     const tc_view* view;              /* parsed structure */
+    /* target-size decode (0 = source size): output planes are tw x th (4:2:2
+     * chroma and RAW planes as the frame geometry derives them); limit =
+     * highest kept scan position (63 all); skip_alpha drops plane 3 */
+    uint32_t tw, th, limit, skip_alpha;
     int32_t rc;
 } tc_dec_req;
+
+/* output plane size of plane p for a target-size decode */
+void tc_target_plane_dims(const tc_fhdr* fh, uint32_t tw, uint32_t th, uint32_t p, uint32_t* w, uint32_t* h);
 
 int32_t tc_engine_decode(tc_ectx* c, tc_dec_req* reqs, uint32_t n);
 int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n);

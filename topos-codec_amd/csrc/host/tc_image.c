@@ -633,6 +633,9 @@ int32_t tc_image_decode(const topos_io* io, const topos_plane_view planes[TC_FRA
     return rc;
 }
 
+/* reference native/src/image/image_reader.c:490-583: a target matching a
+ * fixed ratio exactly goes through the reduced request, any other box through
+ * the scaled one; plane views are checked against the target geometry */
 int32_t tc_image_decode_preview(const topos_io* io, uint32_t target_width, uint32_t target_height,
                                 const topos_plane_view planes[TC_FRAME_MAX_PLANES], topos_frame_output* out_info)
 {
@@ -642,5 +645,59 @@ int32_t tc_image_decode_preview(const topos_io* io, uint32_t target_width, uint3
     topos_image_info info;
     int32_t rc = probe(io, &info);
     if (rc != TC_OK) { return rc; }
-    return tc_fail(TC_ERR_NOT_IMPLEMENTED, "reduced-size preview decode is not provided by the GPU backend");
+    if (target_width > (uint32_t)info.visible_width || target_height > (uint32_t)info.visible_height ||
+        target_width > 65535u || target_height > 65535u) {
+        return tc_fail(TC_ERR_INVALID_ARGUMENT, "image preview: target %ux%u exceeds source %ux%u", target_width,
+                       target_height, (unsigned)info.visible_width, (unsigned)info.visible_height);
+    }
+    uint8_t* pixl = (uint8_t*)malloc((size_t)info.pixl_size);
+    if (pixl == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "image preview PIXL alloc"); }
+    rc = read_at(io, info.pixl_offset, pixl, (size_t)info.pixl_size);
+    topos_decode_request req;
+    memset(&req, 0, sizeof(req));
+    req.struct_size = (uint32_t)sizeof(req);
+    req.abi_version = TOPOS_CODEC_ABI_VERSION;
+    req.mode = TC_DECODE_MODE_SCALED;
+    req.target_width = target_width;
+    req.target_height = target_height;
+    req.memory_type = TC_DECODE_MEMORY_CPU;
+    for (uint32_t s = TC_DECODE_SCALE_HALF; s <= TC_DECODE_SCALE_EIGHTH; ++s) {
+        uint32_t tw = 0, th = 0;
+        if (tc_decode_scale_dimensions(info.visible_width, info.visible_height, (tc_decode_scale)s, &tw, &th) ==
+                TC_OK && tw == target_width && th == target_height) {
+            req.mode = TC_DECODE_MODE_REDUCED;
+            req.scale = s;
+            req.target_width = 0u;
+            req.target_height = 0u;
+            break;
+        }
+    }
+    topos_frame_output tgt;
+    memset(&tgt, 0, sizeof(tgt));
+    if (rc == TC_OK) { rc = tc_frame_decode_request(pixl, (size_t)info.pixl_size, &req, NULL, NULL, &tgt); }
+    if (rc == TC_OK && (tgt.plane_count == 0u || tgt.plane_count > TC_FRAME_MAX_PLANES)) {
+        rc = tc_fail(TC_ERR_MALFORMED, "image preview: invalid source plane count");
+    }
+    uint16_t* ptrs[TC_FRAME_MAX_PLANES] = { NULL, NULL, NULL, NULL };
+    size_t strides[TC_FRAME_MAX_PLANES] = { 0, 0, 0, 0 };
+    for (uint32_t p = 0; rc == TC_OK && p < tgt.plane_count; ++p) {
+        uint32_t w = 0, h = 0;
+        rc = tc_frame_plane_geometry(&tgt, p, &w, &h);
+        if (rc == TC_OK && (planes[p].struct_size != (uint32_t)sizeof(topos_plane_view) ||
+                            planes[p].abi_version != TOPOS_CODEC_ABI_VERSION || planes[p].pixels == NULL)) {
+            rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "image preview: plane %u view is invalid", p);
+        }
+        if (rc == TC_OK && planes[p].stride != 0u && planes[p].stride < (size_t)w) {
+            rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "image preview: plane %u stride %zu < width %u", p, planes[p].stride,
+                         w);
+        }
+        ptrs[p] = planes[p].pixels;
+        strides[p] = planes[p].stride;
+    }
+    if (rc == TC_OK) {
+        rc = tc_frame_decode_request(pixl, (size_t)info.pixl_size, &req, ptrs, strides, &tgt);
+        if (rc >= 0) { *out_info = tgt; }
+    }
+    free(pixl);
+    return rc;
 }

@@ -384,7 +384,14 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
         const uint32_t j = t >> 3, u = t & 7u;
         if (j < nb) {
             const int4 r0 = pre.r0, r1 = pre.r1;
-            const int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
+            int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
+            if (P.limit != 0u && P.limit < 63u) {
+                // reduced decode: scan positions above the limit are dropped
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    if ((uint32_t)kZigzagInv[u * 8 + v] > P.limit) { qv[v] = 0; }
+                }
+            }
             acc_t G[8];
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
@@ -530,14 +537,18 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
     const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (!P.deblock) { return; }
+    // grid unit g (8 at full size, 4 on an exact 2:1 target): strip r holds rows
+    // [g r - 2, g r + g - 2), the rows of horizontal edge g r and the vertical
+    // edges of those rows; strips partition the plane
+    const uint32_t g = P.db_unit ? P.db_unit : 8u;
     const uint32_t w = P.vis_w, h = P.vis_h;
     const uint32_t col_tiles = (w + DB_COLS - 1u) / DB_COLS;
     const uint32_t local = blockIdx.x;
-    if (local >= (h / 8u + 1u) * col_tiles) { return; }
+    if (local >= (h / g + 1u) * col_tiles) { return; }
     const uint32_t r = local / col_tiles;
     const uint32_t c0 = (local % col_tiles) * DB_COLS;
-    const int32_t ys = (int32_t)(8u * r) - 2 < 0 ? 0 : (int32_t)(8u * r) - 2;
-    const int32_t ye = min((int32_t)h, (int32_t)(8u * r) + 6);
+    const int32_t ys = (int32_t)(g * r) - 2 < 0 ? 0 : (int32_t)(g * r) - 2;
+    const int32_t ye = min((int32_t)h, (int32_t)(g * r + g) - 2);
     const int32_t nrows = ye - ys;
     const int32_t xs = (int32_t)c0 - 2;
     const uint16_t* src = (const uint16_t*)P.tmp;
@@ -551,11 +562,12 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
         sP[rr][cc] = (rr < nrows && x >= 0 && x < (int32_t)w) ? (int32_t)src[(size_t)y * P.stride + x] : 0;
     }
     __syncthreads();
-    // vertical edges x0 = c0, c0+8, ..., c0+256 (x0 >= 8, x0 + 1 < w), all rows
-    for (uint32_t i = t; i < 8u * 33u; i += 256u) {
-        const int32_t rr = (int32_t)(i / 33u);
-        const int32_t x0 = (int32_t)c0 + 8 * (int32_t)(i % 33u);
-        if (rr < nrows && x0 >= 8 && x0 + 1 < (int32_t)w) {
+    // vertical edges x0 = c0, c0+g, ..., c0+DB_COLS (x0 >= g, x0 + 1 < w), all rows
+    const uint32_t ne = DB_COLS / g + 1u;
+    for (uint32_t i = t; i < 8u * ne; i += 256u) {
+        const int32_t rr = (int32_t)(i / ne);
+        const int32_t x0 = (int32_t)c0 + (int32_t)g * (int32_t)(i % ne);
+        if (rr < nrows && x0 >= (int32_t)g && x0 + 1 < (int32_t)w) {
             const int32_t l = x0 - xs;
             int32_t p1 = sP[rr][l - 2], p0 = sP[rr][l - 1], q0 = sP[rr][l], q1 = sP[rr][l + 1];
             db_edge(p1, p0, q0, q1, alpha, beta, clip, maxv);
@@ -563,8 +575,8 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
         }
     }
     __syncthreads();
-    // horizontal edge y0 = 8r for 1 <= r <= h/8 - 1
-    if (r >= 1u && r + 1u <= h / 8u) {
+    // horizontal edge y0 = g r for 1 <= r <= h/g - 1
+    if (r >= 1u && r + 1u <= h / g) {
         const int32_t x = (int32_t)c0 + (int32_t)t;
         if (x < (int32_t)w) {
             const int32_t l = x - xs;
@@ -580,6 +592,27 @@ __global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ 
         if (rr < nrows && x < (int32_t)w) {
             dst[(size_t)(ys + rr) * P.stride + x] = (uint16_t)sP[rr][x - xs];
         }
+    }
+}
+
+// Target-size decode: target sample (tx, ty) of a plane takes the source
+// sample (floor(tx vw / dw), floor(ty vh / dh)) of the reconstructed
+// source-size plane; the output is the target's tmp when it deblocks next.
+__global__ __launch_bounds__(256) void k_sample(const tcg_dplane* __restrict__ srcs,
+                                                const tcg_dplane* __restrict__ dsts)
+{
+    const uint32_t pi = blockIdx.y;
+    const tcg_dplane D = dsts[pi];
+    if (!D.sample) { return; }
+    const tcg_dplane S = srcs[pi];
+    const uint16_t* src = (const uint16_t*)S.dst;
+    uint16_t* out = (uint16_t*)(D.deblock ? D.tmp : D.dst);
+    const uint64_t n = (uint64_t)D.vis_w * D.vis_h;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) {
+        const uint32_t ty = (uint32_t)(i / D.vis_w), tx = (uint32_t)(i - (uint64_t)ty * D.vis_w);
+        const uint32_t sy = (uint32_t)(((uint64_t)ty * S.vis_h) / D.vis_h);
+        const uint32_t sx = (uint32_t)(((uint64_t)tx * S.vis_w) / D.vis_w);
+        out[(size_t)ty * D.stride + tx] = src[(size_t)sy * S.stride + sx];
     }
 }
 
@@ -648,4 +681,16 @@ uint32_t tcg_entropy_capacity_dec(uint32_t wpg, uint32_t max_cols)
         return 0u;
     }
     return (uint32_t)per_cu * tcg_cu_count();
+}
+
+extern "C" int32_t tcg_decode_sample(tcg_ctx* c, const tcg_dplane* src_dev, const tcg_dplane* dst_dev,
+                                     uint32_t nplanes, uint32_t max_px, uint32_t max_db_tiles)
+{
+    if (nplanes == 0u || max_px == 0u) { return 0; }
+    const uint32_t gx = min((max_px + 255u) / 256u, 8192u);
+    hipLaunchKernelGGL(k_sample, dim3(gx, nplanes), dim3(256), 0, c->stream, src_dev, dst_dev);
+    if (max_db_tiles > 0u) {
+        hipLaunchKernelGGL(k_deblock, dim3(max_db_tiles, nplanes), dim3(256), 0, c->stream, dst_dev);
+    }
+    return tcg_check("target-size sample launch");
 }

@@ -149,7 +149,9 @@ typedef struct tcg_dplane {
     uint32_t deblock;    /* 1 = run the output deblocking filter on this plane */
     uint32_t db_alpha;   /* deblock strength alpha (beta and clip derive from it) */
     uint32_t tile0;      /* first transform tile of this plane (ascending) */
-    uint32_t pad_;
+    uint32_t limit;      /* highest kept scan position; 0 or >= 63 keep all (reduced decode) */
+    uint32_t db_unit;    /* deblock grid in samples (4 or 8; 0 = 8) */
+    uint32_t sample;     /* target descriptor of a target-size decode: sample this plane */
 } tcg_dplane;
 
 /* per (plane of the batch, block row): the quantiser context and the slice
@@ -247,6 +249,13 @@ uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols);
 
 /* ---- conversions (queued on the context's stream) ---- */
 int32_t tcg_convert_rgb(tcg_ctx* c, const tcg_cvt* k);
+/* target-size decode, after tcg_decode on the same stream: every target
+ * descriptor with sample != 0 takes, per target sample (tx, ty), the source
+ * plane's sample (floor(tx * vw / dw), floor(ty * vh / dh)) (into tmp when the
+ * target deblocks), then the target deblock pass; max_px = largest target
+ * plane in samples, max_db_tiles = deblock grid extent of the targets */
+int32_t tcg_decode_sample(tcg_ctx* c, const tcg_dplane* src_dev, const tcg_dplane* dst_dev, uint32_t nplanes,
+                          uint32_t max_px, uint32_t max_db_tiles);
 int32_t tcg_convert_444_422(tcg_ctx* c, const tcg_p422* k);
 
 /* ---- encode ---- */
