@@ -418,6 +418,45 @@ def sized_bench(pkg, abi, codec, frames, in_addr, cfg, target=SIZED_TARGET):
     return res
 
 
+def preview_bench(pkg, abi, codec, host_pkts, pkt_bufs, sizes, orc, reps=5):
+    """Reduced 1/2 (AUTO_2K) decode of the resident 4K 422 HQ packets to 1920x1080
+    device planes (tcx_decode_batch_dev_request), frames/s over the batch, and
+    one frame's planes compared with the oracle's reduced decode."""
+    from helpers import orc_scaled
+    n = len(host_pkts)
+    req = abi.make_request(abi.DECODE_MODE_AUTO_2K)
+    dims = abi.target_plane_dims(W // 2, H // 2, 0, 3)
+    outs = [[codec.alloc(w * h * 2) for (w, h) in dims] for _ in range(n)]
+    host = (ctypes.c_void_p * n)()
+    szs = (ctypes.c_size_t * n)()
+    dpk = (ctypes.c_uint64 * n)()
+    planes = (ctypes.c_uint64 * (4 * n))()
+    keep = [np.frombuffer(p, np.uint8) for p in host_pkts]
+    for i in range(n):
+        host[i], szs[i], dpk[i] = keep[i].ctypes.data, sizes[i], pkt_bufs[i].addr
+        for p in range(3):
+            planes[4 * i + p] = outs[i][p].addr
+    infos = (abi.FrameOutput * n)()
+    rcs = (ctypes.c_int32 * n)()
+
+    def run():
+        st = codec._lib.tcx_decode_batch_dev_request(n, host, szs, dpk, planes, None, infos, rcs, ctypes.byref(req))
+        assert st == 0, codec.last_error()
+    run()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    rc, want, _, _ = orc_scaled(orc, host_pkts[0], W // 2, H // 2, abi.SCALE_LIMIT[1], dims)
+    got = [b.download(w * h * 2, np.uint16).reshape(h, w) for b, (w, h) in zip(outs[0], dims)]
+    t = float(np.median(times))
+    return {"workload": f"{n} x 3840x2160 422 HQ packets -> 1920x1080 (AUTO_2K: reduced 1/2, scan positions <= 24, "
+                        "deblock on the 4-sample grid), device-resident",
+            "ms": round(t * 1e3, 2), "frames_per_s": round(n / t, 1),
+            "equal_to_oracle": bool(rc == 0 and all(np.array_equal(a, b) for a, b in zip(got, want)))}
+
+
 def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
     """The mixed serving load: 60 % decodes / 40 % encodes over 1080p LT, 4K HQ,
     4K 4444 XQ + alpha and RAW 4096x3072 requests (shares 30/40/20/10 %), max
@@ -715,6 +754,7 @@ def main():
         configs = {"xq4444_alpha_16x4096x2160": batch_config_bench(pkg, abi, syn, codec, orc, XQ_BATCH, 3),
                    "uneven_batch": batch_config_bench(pkg, abi, syn, codec, orc, UNEVEN_BATCH, 3),
                    "sized_4k_422hq": sized_bench(pkg, abi, codec, frames, in_addr, cfg),
+                   "preview_auto2k_4k": preview_bench(pkg, abi, codec, host_pkts, pkt_bufs, sizes, orc),
                    "convert_bgra32_4k": convert_bench(pkg, abi, codec, orc, 2),
                    "convert_rgba64_4k": convert_bench(pkg, abi, codec, orc, 7),
                    "serving_mix": serve_mix_bench(pkg, abi, syn, codec, a.hbm_budget, a.max_batch, a.lanes,
