@@ -148,9 +148,10 @@ typedef struct topos_plane_view {
     size_t stride;
 } topos_plane_view;
 
-/* caller byte source: read exactly len bytes at an absolute offset, TC_OK or
- * TC_ERR_IO (reference topos_codec.h:646-669; the write callbacks are for the
- * reference's container writers and are not called here) */
+/* caller byte source / sink (reference topos_codec.h:646-669): read exactly
+ * len bytes at an absolute offset; write appends; seek_write overwrites len
+ * bytes at an absolute offset (the MOV writer patches the mdat size with it).
+ * Each returns TC_OK, or TC_ERR_IO on a short or failed transfer. */
 typedef int32_t (*tc_io_read_fn)(void* ctx, uint64_t offset, void* buf, size_t len);
 typedef int32_t (*tc_io_write_fn)(void* ctx, const void* data, size_t len);
 typedef int32_t (*tc_io_seek_write_fn)(void* ctx, uint64_t offset, const void* data, size_t len);
@@ -373,6 +374,162 @@ typedef struct topos_planar_444_to_422_params {
 /* reference topos_codec.h:1100-1101: (a + b + 1) >> 1 of horizontal pairs,
  * the edge sample repeated for an odd width */
 int32_t tc_convert_yuv444_to_422(const topos_planar_444_to_422_params* params);
+
+/* ---- MOV container, video track (reference topos_codec.h:670-1014) -------
+ * Mux: ftyp + mdat + moov, the mdat size patched at finish (the sink needs
+ * write and seek_write). Same packets and config give the same bytes as the
+ * reference writer. Demux: open parses the moov only and builds the sample
+ * index; packets are read and their headers checked against tpcC on demand.
+ * Sound and timecode tracks are not written (TC_ERR_NOT_IMPLEMENTED) and are
+ * skipped when read. */
+#define TC_MOVIE_MAX_SAMPLES 67108864u
+
+/* reference topos_codec.h:674-702; reserved[0..4] declare a sound track
+ * (all 0 = none), the only form accepted here */
+typedef struct topos_movie_config {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t visible_width;
+    uint16_t visible_height;
+    uint8_t profile;
+    uint8_t pixel_format;
+    uint8_t bit_depth;
+    uint8_t qmatrix_id;
+    uint8_t qp_base;
+    int8_t qp_delta_luma;
+    int8_t qp_delta_chroma;
+    uint8_t alpha_mode;
+    uint8_t alpha_bit_depth;
+    uint8_t alpha_premultiplied;
+    uint8_t color_range;
+    uint8_t color_primaries;
+    uint8_t color_transfer;
+    uint8_t color_matrix;
+    uint8_t chroma_siting;
+    uint16_t sar_num;
+    uint16_t sar_den;
+    uint32_t timescale;     /* 0 = 24000 */
+    uint32_t reserved[8];
+} topos_movie_config;
+
+/* reference topos_codec.h:730-760; reserved[0] = the track's bitstream major
+ * (tpcC version), reserved[1] = the tpcD tier (0 = none) */
+typedef struct topos_movie_info {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t visible_width;
+    uint16_t visible_height;
+    uint8_t profile;
+    uint8_t pixel_format;
+    uint8_t bit_depth;
+    uint8_t qmatrix_id;
+    uint8_t qp_base;
+    int8_t qp_delta_luma;
+    int8_t qp_delta_chroma;
+    uint8_t alpha_mode;
+    uint8_t alpha_bit_depth;
+    uint8_t alpha_premultiplied;
+    uint8_t color_range;
+    uint8_t color_primaries;
+    uint8_t color_transfer;
+    uint8_t color_matrix;
+    uint8_t chroma_siting;
+    uint16_t sar_num;
+    uint16_t sar_den;
+    uint32_t timescale;
+    uint32_t sample_count;
+    uint32_t faststart;     /* 1 = moov before mdat */
+    uint32_t index_bytes;   /* memory of the sample index */
+    uint32_t reserved[8];
+} topos_movie_info;
+
+/* tpcB alpha budget (reference topos_codec.h:762-780) */
+#define TC_ALPHA_BUDGET_RATIO_UNSET 0xFFFFu
+#define TC_ALPHA_BUDGET_FLAG_OVERRUN 0x0001u
+#define TC_ALPHA_BUDGET_FLAG_AUTHORIZED 0x0002u
+#define TC_ALPHA_BUDGET_FLAG_ADAPTED 0x0004u
+#define TC_ALPHA_BUDGET_FLAG_RATIO_SATURATED 0x0008u
+typedef struct topos_alpha_budget_info {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t target_ratio_bp;
+    uint16_t actual_ratio_bp;
+    uint16_t max_abs_error;
+    uint16_t flags;
+    uint32_t frame_count;   /* must equal the sample count when read */
+    uint32_t reserved[4];
+} topos_alpha_budget_info;
+
+typedef struct topos_mux topos_mux;
+typedef struct topos_movie topos_movie;
+
+/* tpcD movie metadata (reference topos_codec.h:784-825) */
+#define TC_TIER_NONE 0u
+#define TC_TIER_PROXY 1u
+#define TC_TIER_LT 2u
+#define TC_TIER_STANDARD 3u
+#define TC_TIER_HQ 4u
+#define TC_TIER_4444 5u
+#define TC_TIER_4444XQ 6u
+#define TC_TIER_LP 7u
+#define TC_TIER_RAW 8u
+typedef struct topos_movie_meta {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint8_t tier_id;
+    uint8_t reserved[3];    /* 0 */
+    char vendor[16];        /* NUL-terminated, at most 15 bytes */
+    char label[64];         /* NUL-terminated, at most 63 bytes */
+} topos_movie_meta;
+
+/* trwm payload, carried opaque (reference topos_codec.h:817-829) */
+#define TC_RAW_META_MAX 256u
+typedef struct topos_raw_meta {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint16_t payload_size;  /* 1..TC_RAW_META_MAX */
+    uint16_t reserved;      /* 0 */
+    uint8_t payload[TC_RAW_META_MAX];
+} topos_raw_meta;
+
+/* reference topos_codec.h:833-836 */
+int32_t tc_mux_create(const topos_movie_config* cfg, const topos_io* sink, topos_mux** out);
+int32_t tc_mux_add_packet(topos_mux* mux, const uint8_t* packet, size_t size, uint64_t pts_tick, uint32_t dur_tick);
+/* reference topos_codec.h:841-848 */
+int32_t tc_mux_set_alpha_budget(topos_mux* mux, const topos_alpha_budget_info* info);
+int32_t tc_mux_set_movie_meta(topos_mux* mux, const topos_movie_meta* meta);
+int32_t tc_mux_set_raw_meta(topos_mux* mux, const topos_raw_meta* meta);
+/* reference topos_codec.h:856-874: TC_ERR_NOT_IMPLEMENTED here */
+int32_t tc_mux_add_audio(topos_mux* mux, const uint8_t* data, size_t size, uint32_t num_samples);
+int32_t tc_mux_set_audio_asc(topos_mux* mux, const uint8_t* asc, size_t size);
+int32_t tc_mux_set_timecode(topos_mux* mux, uint32_t hh, uint32_t mm, uint32_t ss, uint32_t ff, uint32_t fps,
+                            uint32_t drop_frame);
+int32_t tc_mux_set_audio_priming(topos_mux* mux, uint32_t samples);
+/* reference topos_codec.h:875-876 */
+int32_t tc_mux_finish(topos_mux* mux);
+void tc_mux_free(topos_mux* mux);
+/* reference topos_codec.h:880 */
+int32_t tc_movie_faststart(const topos_io* src, const topos_io* dst);
+/* reference topos_codec.h:883-892 */
+int32_t tc_movie_open(const topos_io* io, topos_movie** out);
+int32_t tc_movie_open_fd(int fd, uint64_t length, topos_movie** out);
+int32_t tc_movie_info(const topos_movie* movie, topos_movie_info* info);
+int32_t tc_movie_alpha_budget(const topos_movie* movie, topos_alpha_budget_info* info);
+/* reference topos_codec.h:968 */
+int32_t tc_movie_read_raw_meta(const topos_movie* movie, topos_raw_meta* out);
+/* reference topos_codec.h:978-979: buf NULL or cap short -> BUFFER_TOO_SMALL
+ * with *need_size */
+int32_t tc_movie_packet(const topos_movie* movie, uint32_t sample_index, uint8_t* buf, size_t cap,
+                        size_t* need_size);
+/* reference topos_codec.h:1000-1003: samples [start, start + count) packed
+ * into arena, one io read when they are contiguous in the file */
+int32_t tc_movie_packet_batch(const topos_movie* movie, uint32_t start, uint32_t count, uint8_t* arena, size_t cap,
+                              size_t* offsets_out, size_t* sizes_out, size_t* need_total);
+/* reference topos_codec.h:1004-1014 */
+int32_t tc_movie_packet_pts(const topos_movie* movie, uint32_t sample_index, uint64_t* pts_tick, uint32_t* dur_tick);
+int32_t tc_movie_packet_sync(const topos_movie* movie, uint32_t sample_index, uint8_t* is_sync);
+int32_t tc_movie_prev_sync(const topos_movie* movie, uint32_t sample_index, uint32_t* sync_index);
+void tc_movie_close(topos_movie* movie);
 
 #ifdef __cplusplus
 }

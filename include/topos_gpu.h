@@ -49,6 +49,16 @@ int32_t tcx_decode_batch_dev(uint32_t n, const uint8_t* const* host_packets, con
                              const uint64_t* dev_packets, const uint64_t* dev_planes,
                              const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs);
 
+/* MOV samples [start, start + count) into HBM and decoded as one batch (no
+ * reference counterpart; it joins tc_movie_packet_batch, reference
+ * topos_codec.h:1000-1003, with tcx_decode_batch_dev): the packets are read
+ * with one io read when contiguous, header-checked, uploaded to dev_arena (a
+ * tcx_dev_alloc block of arena_cap >= their total bytes) in one copy, and
+ * decoded into dev_planes (count x 4). */
+int32_t tcx_movie_decode_batch_dev(const topos_movie* mv, uint32_t start, uint32_t count, uint64_t dev_arena,
+                                   size_t arena_cap, const uint64_t* dev_planes, const size_t* dev_strides,
+                                   topos_frame_output* infos, int32_t* rcs);
+
 /* device-resident batch decode through one decode request (reduced, scaled,
  * AUTO_2K, DROP_ALPHA): dev_planes receive the request's target geometry */
 int32_t tcx_decode_batch_dev_request(uint32_t n, const uint8_t* const* host_packets, const size_t* sizes,

@@ -104,8 +104,38 @@ def load_library(path=LIB_PATH):
     lib.tcx_sized_set_mode.restype = None
     lib.tcx_sized_set_seed.argtypes = [ctypes.c_int32, ctypes.c_int32]
     lib.tcx_sized_set_seed.restype = None
+    movie_signatures(lib)
+    lib.tcx_movie_decode_batch_dev.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                               ctypes.c_size_t, vp, vp, vp, vp]
     del u8p
     return lib
+
+
+def movie_signatures(lib):
+    """ctypes signatures of the MOV entry points (the same C ABI as the reference's)."""
+    vp, u32, u64, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t
+    lib.tc_last_error.restype = ctypes.c_char_p
+    lib.tc_mux_create.argtypes = [vp, vp, vp]
+    lib.tc_mux_add_packet.argtypes = [vp, vp, sz, u64, u32]
+    lib.tc_mux_set_alpha_budget.argtypes = [vp, vp]
+    lib.tc_mux_set_movie_meta.argtypes = [vp, vp]
+    lib.tc_mux_set_raw_meta.argtypes = [vp, vp]
+    lib.tc_mux_finish.argtypes = [vp]
+    lib.tc_mux_free.argtypes = [vp]
+    lib.tc_mux_free.restype = None
+    lib.tc_movie_faststart.argtypes = [vp, vp]
+    lib.tc_movie_open.argtypes = [vp, vp]
+    lib.tc_movie_open_fd.argtypes = [ctypes.c_int, u64, vp]
+    lib.tc_movie_info.argtypes = [vp, vp]
+    lib.tc_movie_alpha_budget.argtypes = [vp, vp]
+    lib.tc_movie_read_raw_meta.argtypes = [vp, vp]
+    lib.tc_movie_packet.argtypes = [vp, u32, vp, sz, vp]
+    lib.tc_movie_packet_batch.argtypes = [vp, u32, u32, vp, sz, vp, vp, vp]
+    lib.tc_movie_packet_pts.argtypes = [vp, u32, vp, vp]
+    lib.tc_movie_packet_sync.argtypes = [vp, u32, vp]
+    lib.tc_movie_prev_sync.argtypes = [vp, u32, vp]
+    lib.tc_movie_close.argtypes = [vp]
+    lib.tc_movie_close.restype = None
 
 
 def _ptr(a):
@@ -354,6 +384,198 @@ class MemoryIo:
         return ctypes.cast(ctypes.byref(self.io), ctypes.c_void_p)
 
 
+class MemorySink:
+    """topos_io sink appending to a bytearray (write + seek_write)."""
+
+    def __init__(self):
+        self.data = bytearray()
+
+        def _write(_ctx, buf, length):
+            self.data += ctypes.string_at(buf, length)
+            return abi.TC_OK
+
+        def _seek_write(_ctx, offset, buf, length):
+            if offset + length > len(self.data):
+                return abi.TC_ERR_IO
+            self.data[offset:offset + length] = ctypes.string_at(buf, length)
+            return abi.TC_OK
+
+        self._w = abi.IO_WRITE_FN(_write)
+        self._s = abi.IO_SEEK_WRITE_FN(_seek_write)
+        self.io = abi.sized(abi.TposIo, write=self._w, seek_write=self._s)
+
+    def ref(self):
+        return ctypes.cast(ctypes.byref(self.io), ctypes.c_void_p)
+
+
+class ToposMovieFile:
+    """MOV reader (reference python/topos_codec/topos_binding.py:1954-2455):
+    a file path (opened through tc_movie_open_fd) or the file's bytes."""
+
+    def __init__(self, lib, src):
+        self._lib = lib
+        self._fd = None
+        self._src = None
+        h = ctypes.c_void_p()
+        if isinstance(src, (bytes, bytearray, memoryview)):
+            self._src = MemoryIo(src)
+            st = int(lib.tc_movie_open(self._src.ref(), ctypes.byref(h)))
+        else:
+            self._fd = os.open(src, os.O_RDONLY)
+            st = int(lib.tc_movie_open_fd(self._fd, os.fstat(self._fd).st_size, ctypes.byref(h)))
+        if st != abi.TC_OK:
+            self._close_fd()
+            raise ToposCodecError(st, lib.tc_last_error().decode(errors="replace"))
+        self.handle = h
+
+    def _close_fd(self):
+        if self._fd is not None:
+            os.close(self._fd)
+            self._fd = None
+
+    def _check(self, st, what):
+        if st != abi.TC_OK:
+            raise ToposCodecError(st, f"{what}: " + self._lib.tc_last_error().decode(errors="replace"))
+
+    def info(self):
+        info = abi.sized(abi.MovieInfo)
+        self._check(int(self._lib.tc_movie_info(self.handle, ctypes.byref(info))), "tc_movie_info")
+        return info
+
+    def packet(self, index):
+        need = ctypes.c_size_t(0)
+        st = int(self._lib.tc_movie_packet(self.handle, index, None, 0, ctypes.byref(need)))
+        if st != abi.TC_ERR_BUFFER_TOO_SMALL:
+            self._check(st, "tc_movie_packet probe")
+        buf = ctypes.create_string_buffer(need.value)
+        self._check(int(self._lib.tc_movie_packet(self.handle, index, buf, need.value, None)), "tc_movie_packet")
+        return buf.raw
+
+    def packets_into_batch(self, start, count):
+        """(arena bytes, offsets, sizes) of samples [start, start + count)."""
+        need = ctypes.c_size_t(0)
+        st = int(self._lib.tc_movie_packet_batch(self.handle, start, count, None, 0, None, None, ctypes.byref(need)))
+        if st != abi.TC_ERR_BUFFER_TOO_SMALL:
+            self._check(st, "tc_movie_packet_batch probe")
+        offs = (ctypes.c_size_t * count)()
+        sizes = (ctypes.c_size_t * count)()
+        buf = ctypes.create_string_buffer(max(need.value, 1))
+        self._check(int(self._lib.tc_movie_packet_batch(self.handle, start, count, buf, need.value, offs, sizes,
+                                                         None)), "tc_movie_packet_batch")
+        return buf.raw[:need.value], list(offs), list(sizes)
+
+    def pts(self, index):
+        pts, dur = ctypes.c_uint64(0), ctypes.c_uint32(0)
+        self._check(int(self._lib.tc_movie_packet_pts(self.handle, index, ctypes.byref(pts), ctypes.byref(dur))),
+                    "tc_movie_packet_pts")
+        return pts.value, dur.value
+
+    def is_sync(self, index):
+        v = ctypes.c_uint8(0)
+        self._check(int(self._lib.tc_movie_packet_sync(self.handle, index, ctypes.byref(v))), "tc_movie_packet_sync")
+        return bool(v.value)
+
+    def prev_sync(self, index):
+        v = ctypes.c_uint32(0)
+        self._check(int(self._lib.tc_movie_prev_sync(self.handle, index, ctypes.byref(v))), "tc_movie_prev_sync")
+        return v.value
+
+    def alpha_budget(self):
+        """The tpcB record, or None when the file carries none."""
+        bi = abi.sized(abi.AlphaBudgetInfo)
+        st = int(self._lib.tc_movie_alpha_budget(self.handle, ctypes.byref(bi)))
+        if st == abi.TC_ERR_STATE:
+            return None
+        self._check(st, "tc_movie_alpha_budget")
+        return bi
+
+    def read_raw_meta(self):
+        rm = abi.sized(abi.RawMeta)
+        self._check(int(self._lib.tc_movie_read_raw_meta(self.handle, ctypes.byref(rm))), "tc_movie_read_raw_meta")
+        return bytes(rm.payload[:rm.payload_size])
+
+    def close(self):
+        if self.handle:
+            self._lib.tc_movie_close(self.handle)
+            self.handle = None
+        self._close_fd()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *_exc):
+        self.close()
+
+
+class ToposMuxFile:
+    """MOV writer (reference python/topos_codec/topos_binding.py:2457-2715):
+    to a file path, or to memory when path is None (the bytes in .data after
+    finish)."""
+
+    def __init__(self, lib, path, config):
+        self._lib = lib
+        self._path = path
+        self._sink = MemorySink()
+        h = ctypes.c_void_p()
+        st = int(lib.tc_mux_create(ctypes.byref(config), self._sink.ref(), ctypes.byref(h)))
+        if st != abi.TC_OK:
+            raise ToposCodecError(st, lib.tc_last_error().decode(errors="replace"))
+        self.handle = h
+
+    def _check(self, st, what):
+        if st != abi.TC_OK:
+            raise ToposCodecError(st, f"{what}: " + self._lib.tc_last_error().decode(errors="replace"))
+
+    def add_packet(self, packet, pts, dur=1):
+        self._check(int(self._lib.tc_mux_add_packet(self.handle, bytes(packet), len(packet), pts, dur)),
+                    "tc_mux_add_packet")
+
+    def set_alpha_budget(self, **kw):
+        self._check(int(self._lib.tc_mux_set_alpha_budget(self.handle, ctypes.byref(abi.sized(abi.AlphaBudgetInfo,
+                                                                                            **kw)))),
+                    "tc_mux_set_alpha_budget")
+
+    def set_movie_meta(self, tier_id=0, vendor="", label=""):
+        m = abi.sized(abi.MovieMeta, tier_id=tier_id, vendor=vendor.encode(), label=label.encode())
+        self._check(int(self._lib.tc_mux_set_movie_meta(self.handle, ctypes.byref(m))), "tc_mux_set_movie_meta")
+
+    def set_raw_meta(self, payload):
+        rm = abi.sized(abi.RawMeta, payload_size=len(payload))
+        ctypes.memmove(rm.payload, bytes(payload), len(payload))
+        self._check(int(self._lib.tc_mux_set_raw_meta(self.handle, ctypes.byref(rm))), "tc_mux_set_raw_meta")
+
+    @property
+    def data(self):
+        return bytes(self._sink.data)
+
+    def finish(self):
+        self._check(int(self._lib.tc_mux_finish(self.handle)), "tc_mux_finish")
+        if self._path is not None:
+            with open(self._path, "wb") as f:
+                f.write(self._sink.data)
+
+    def close(self):
+        if self.handle:
+            self._lib.tc_mux_free(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *_exc):
+        self.close()
+
+
+def movie_faststart(lib, data):
+    """tc_movie_faststart over bytes: the moov-first file."""
+    src = MemoryIo(data)
+    dst = MemorySink()
+    st = int(lib.tc_movie_faststart(src.ref(), dst.ref()))
+    if st != abi.TC_OK:
+        raise ToposCodecError(st, lib.tc_last_error().decode(errors="replace"))
+    return bytes(dst.data)
+
+
 def _image_source(src):
     if isinstance(src, (bytes, bytearray, memoryview)):
         return MemoryIo(src)
@@ -570,6 +792,23 @@ class DeviceCodec(ToposCodec):
         st = int(self._lib.tcx_encode_batch_dev(a["cfgs"], a["n"], a["planes"], a["strides"], a["outs"], a["caps"],
                                                 a["sizes"], a["rcs"], a["stats"]))
         return st, list(a["sizes"]), list(a["rcs"]), a["stats"]
+
+    def decode_movie_batch_dev(self, movie, start, count, arena, plane_addrs, strides=None):
+        """tcx_movie_decode_batch_dev: MOV samples [start, start + count) of a
+        ToposMovieFile uploaded into arena (a DeviceBuffer) and decoded into
+        plane_addrs (per frame, 3-4 device addresses). Returns (status, rcs, infos)."""
+        planes = (ctypes.c_uint64 * (4 * count))()
+        strd = (ctypes.c_size_t * (4 * count))() if strides is not None else None
+        for i, pa in enumerate(plane_addrs):
+            for p, addr in enumerate(pa):
+                planes[4 * i + p] = addr
+                if strides is not None:
+                    strd[4 * i + p] = strides[i][p]
+        infos = (abi.FrameOutput * count)()
+        rcs = (ctypes.c_int32 * count)()
+        st = int(self._lib.tcx_movie_decode_batch_dev(movie.handle, start, count, ctypes.c_uint64(arena.addr),
+                                                      arena.nbytes, planes, strd, infos, rcs))
+        return st, list(rcs), infos
 
     def decode_batch_dev(self, host_packets, dev_packets, plane_addrs, strides=None):
         a = self._dec_args(host_packets, dev_packets, plane_addrs, strides)

@@ -142,6 +142,9 @@ TC_IMG_ERR_METADATA_CONFLICT = -105
 TC_IMG_ERR_LIMIT = -106
 
 IO_READ_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_size_t)
+IO_WRITE_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+IO_SEEK_WRITE_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                    ctypes.c_size_t)
 
 
 class TposIo(ctypes.Structure):
@@ -150,11 +153,82 @@ class TposIo(ctypes.Structure):
         ("abi_version", ctypes.c_uint32),
         ("ctx", ctypes.c_void_p),
         ("read", IO_READ_FN),
-        ("write", ctypes.c_void_p),
-        ("seek_write", ctypes.c_void_p),
+        ("write", IO_WRITE_FN),
+        ("seek_write", IO_SEEK_WRITE_FN),
         ("length", ctypes.c_uint64),
         ("reserved", ctypes.c_uint32 * 4),
     ]
+
+
+# ---- MOV container (include/topos_codec.h, reference topos_codec.h:670-1014)
+TC_MOVIE_MAX_SAMPLES = 67108864
+TC_ALPHA_BUDGET_RATIO_UNSET = 0xFFFF
+TC_TIER_NONE, TC_TIER_PROXY, TC_TIER_LT, TC_TIER_STANDARD, TC_TIER_HQ = 0, 1, 2, 3, 4
+TC_TIER_4444, TC_TIER_4444XQ, TC_TIER_LP, TC_TIER_RAW = 5, 6, 7, 8
+TC_RAW_META_MAX = 256
+
+_MOVIE_FIELDS = [
+    ("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+    ("visible_width", ctypes.c_uint16), ("visible_height", ctypes.c_uint16),
+    ("profile", ctypes.c_uint8), ("pixel_format", ctypes.c_uint8), ("bit_depth", ctypes.c_uint8),
+    ("qmatrix_id", ctypes.c_uint8), ("qp_base", ctypes.c_uint8),
+    ("qp_delta_luma", ctypes.c_int8), ("qp_delta_chroma", ctypes.c_int8),
+    ("alpha_mode", ctypes.c_uint8), ("alpha_bit_depth", ctypes.c_uint8), ("alpha_premultiplied", ctypes.c_uint8),
+    ("color_range", ctypes.c_uint8), ("color_primaries", ctypes.c_uint8), ("color_transfer", ctypes.c_uint8),
+    ("color_matrix", ctypes.c_uint8), ("chroma_siting", ctypes.c_uint8),
+    ("sar_num", ctypes.c_uint16), ("sar_den", ctypes.c_uint16), ("timescale", ctypes.c_uint32),
+]
+
+
+class MovieConfig(ctypes.Structure):
+    _fields_ = _MOVIE_FIELDS + [("reserved", ctypes.c_uint32 * 8)]
+
+
+class MovieInfo(ctypes.Structure):
+    _fields_ = _MOVIE_FIELDS + [("sample_count", ctypes.c_uint32), ("faststart", ctypes.c_uint32),
+                                ("index_bytes", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 8)]
+
+
+class AlphaBudgetInfo(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+                ("target_ratio_bp", ctypes.c_uint16), ("actual_ratio_bp", ctypes.c_uint16),
+                ("max_abs_error", ctypes.c_uint16), ("flags", ctypes.c_uint16),
+                ("frame_count", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class MovieMeta(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+                ("tier_id", ctypes.c_uint8), ("reserved", ctypes.c_uint8 * 3),
+                ("vendor", ctypes.c_char * 16), ("label", ctypes.c_char * 64)]
+
+
+class RawMeta(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+                ("payload_size", ctypes.c_uint16), ("reserved", ctypes.c_uint16),
+                ("payload", ctypes.c_uint8 * TC_RAW_META_MAX)]
+
+
+def sized(cls, **kw):
+    """A zeroed ABI struct with struct_size / abi_version set and fields from kw."""
+    v = cls()
+    v.struct_size = ctypes.sizeof(cls)
+    v.abi_version = TOPOS_CODEC_ABI_VERSION
+    for k, x in kw.items():
+        setattr(v, k, x)
+    return v
+
+
+def movie_config(width, height, qp=24, profile=3, qmatrix=1, alpha_mode=0, alpha_bit_depth=16,
+                 alpha_premultiplied=False, color_range=1, color_primaries=1, color_transfer=1, color_matrix=1,
+                 chroma_siting=0, sar_num=1, sar_den=1, timescale=24000, bit_depth=10, pixel_format=0):
+    """topos_movie_config with the reference binding's defaults
+    (python/topos_codec/topos_binding.py:4126-4146)."""
+    return sized(MovieConfig, visible_width=width, visible_height=height, qp_base=qp, profile=profile,
+                 qmatrix_id=qmatrix, alpha_mode=alpha_mode, alpha_bit_depth=alpha_bit_depth if alpha_mode else 0,
+                 alpha_premultiplied=int(bool(alpha_premultiplied)), color_range=color_range,
+                 color_primaries=color_primaries, color_transfer=color_transfer, color_matrix=color_matrix,
+                 chroma_siting=chroma_siting, sar_num=sar_num, sar_den=sar_den, timescale=timescale,
+                 bit_depth=bit_depth, pixel_format=pixel_format)
 
 
 class ImagePreamble(ctypes.Structure):

@@ -20,7 +20,7 @@ ARCH = os.environ.get("TOPOS_GPU_ARCH", "gfx950")
 
 HOST_C = ["tc_status.c", "tc_format.c", "tc_vlc.c", "tc_engine.c", "tc_api.c", "tc_gpu_api.c", "tc_image.c",
           "tc_arena.c", "tc_serve.c", "tc_tune.c",
-          "tc_convert.c", "tc_sized.c", "tc_sized_search.c", "tc_preview.c"]
+          "tc_convert.c", "tc_sized.c", "tc_sized_search.c", "tc_preview.c", "tc_mov.c"]
 HIP_SRC = ["tc_runtime.hip", "tc_encode.hip", "tc_decode.hip", "tc_alpha.hip", "tc_bits.hip", "tc_convert.hip"]
 
 CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
