@@ -457,6 +457,51 @@ def preview_bench(pkg, abi, codec, host_pkts, pkt_bufs, sizes, orc, reps=5):
             "equal_to_oracle": bool(rc == 0 and all(np.array_equal(a, b) for a, b in zip(got, want)))}
 
 
+def mov_bench(pkg, abi, codec, host_pkts, cfg, dec0, reps=5):
+    """The 4K 422 HQ packets muxed into a MOV file (tc_mux_*, 24000/1001 fps),
+    opened through tc_movie_open_fd, and all samples read, uploaded in one
+    copy and decoded as one batch per step (tcx_movie_decode_batch_dev): the
+    file-to-HBM-planes rate, the mux rate, and sample 0 checked against the
+    resident decode of the same packet."""
+    import tempfile
+    n = len(host_pkts)
+    lib = codec._lib
+    t0 = time.perf_counter()
+    st, header = codec.query(host_pkts[0])
+    assert st == 0, codec.last_error()
+    with pkg.ToposMuxFile(lib, None, abi.movie_config_for(cfg, header, timescale=24000)) as mx:
+        for i, p in enumerate(host_pkts):
+            mx.add_packet(p, 1001 * i, 1001)
+        mx.finish()
+        data = mx.data
+    mux_s = time.perf_counter() - t0
+    dims = abi.plane_dims(W, H, 0, 0)
+    planes = [[codec.alloc(w * h * 2) for (w, h) in dims] for _ in range(n)]
+    addrs = [[b.addr for b in p] for p in planes]
+    with tempfile.NamedTemporaryFile(suffix=".mov") as f:
+        f.write(data)
+        f.flush()
+        with pkg.ToposMovieFile(lib, f.name) as mv:
+            arena = codec.alloc(len(data))
+
+            def run():
+                st, rcs, _ = codec.decode_movie_batch_dev(mv, 0, n, arena, addrs)
+                assert st == 0 and rcs == [0] * n, codec.last_error()
+            run()
+            times = []
+            for _ in range(reps):
+                t1 = time.perf_counter()
+                run()
+                times.append(time.perf_counter() - t1)
+    got = [b.download(w * h * 2, np.uint16).reshape(h, w) for b, (w, h) in zip(planes[0], dims)]
+    t = float(np.median(times))
+    return {"workload": f"{n} x 3840x2160 422 HQ samples: MOV file (page cache) -> pread -> one H2D copy -> "
+                        "batch decode into device planes",
+            "ms": round(t * 1e3, 2), "frames_per_s": round(n / t, 1), "file_bytes": len(data),
+            "mux_ms": round(mux_s * 1e3, 2),
+            "equal_to_resident_decode": bool(all(np.array_equal(a, b) for a, b in zip(got, dec0)))}
+
+
 def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
     """The mixed serving load: 60 % decodes / 40 % encodes over 1080p LT, 4K HQ,
     4K 4444 XQ + alpha and RAW 4096x3072 requests (shares 30/40/20/10 %), max
@@ -755,6 +800,7 @@ def main():
                    "uneven_batch": batch_config_bench(pkg, abi, syn, codec, orc, UNEVEN_BATCH, 3),
                    "sized_4k_422hq": sized_bench(pkg, abi, codec, frames, in_addr, cfg),
                    "preview_auto2k_4k": preview_bench(pkg, abi, codec, host_pkts, pkt_bufs, sizes, orc),
+                   "mov_batch_decode_4k": mov_bench(pkg, abi, codec, host_pkts, cfg, dec0),
                    "convert_bgra32_4k": convert_bench(pkg, abi, codec, orc, 2),
                    "convert_rgba64_4k": convert_bench(pkg, abi, codec, orc, 7),
                    "serving_mix": serve_mix_bench(pkg, abi, syn, codec, a.hbm_budget, a.max_batch, a.lanes,

@@ -231,6 +231,20 @@ def movie_config(width, height, qp=24, profile=3, qmatrix=1, alpha_mode=0, alpha
                  bit_depth=bit_depth, pixel_format=pixel_format)
 
 
+def movie_config_for(frame_cfg, header, timescale=24000):
+    """The movie config whose tpcC matches packets encoded with frame_cfg:
+    qmatrix, qp and premultiplication from the config, every field the encoder
+    normalises (depth, alpha, colour description, siting, sar) from header, the
+    FrameOutput of a query decode of one of the packets."""
+    names = [f for f, _ in _MOVIE_FIELDS if f not in ("struct_size", "abi_version", "timescale")]
+    v = sized(MovieConfig, timescale=timescale, **{k: getattr(frame_cfg, k) for k in names})
+    for k in ("visible_width", "visible_height", "profile", "pixel_format", "bit_depth", "alpha_mode",
+              "alpha_bit_depth", "color_range", "color_primaries", "color_transfer", "color_matrix",
+              "chroma_siting", "sar_num", "sar_den"):
+        setattr(v, k, getattr(header, k))
+    return v
+
+
 class ImagePreamble(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
