@@ -495,7 +495,7 @@ def mov_bench(pkg, abi, codec, host_pkts, cfg, dec0, reps=5):
                 times.append(time.perf_counter() - t1)
     got = [b.download(w * h * 2, np.uint16).reshape(h, w) for b, (w, h) in zip(planes[0], dims)]
     t = float(np.median(times))
-    return {"workload": f"{n} x 3840x2160 422 HQ samples: MOV file (page cache) -> pread -> one H2D copy -> "
+    return {"workload": f"{n} x 3840x2160 422 HQ samples: MOV file (page cache) -> parallel pread into pinned staging -> one DMA copy -> "
                         "batch decode into device planes",
             "ms": round(t * 1e3, 2), "frames_per_s": round(n / t, 1), "file_bytes": len(data),
             "mux_ms": round(mux_s * 1e3, 2),

@@ -366,3 +366,28 @@ def test_metadata_setter_rules(pkg, lib, reflib, abi, recs):
         return codes
 
     assert run(lib) == run(reflib)
+
+
+def test_large_movie_parallel_reads(pkg, lib, reflib, abi, syn, oracle, tmp_path):
+    """A 1080p movie over 8 MiB opened by descriptor: the batch read splits into
+    parallel preads; bytes and statuses equal the callback source and the
+    reference reader."""
+    from helpers import frame_for, orc_encode
+    cfg = abi.make_config(1920, 1080, 4, profile=3, entropy_mode=8, slice_rows=8)
+    rc, pkt = orc_encode(oracle, cfg, frame_for(pkg, syn, "422", 1920, 1080, 7))
+    assert rc == 0
+    st, header = pkg.ToposCodec.query(pkg.ToposCodec(), pkt)
+    assert st == 0
+    n = max(3, (9 << 20) // len(pkt) + 1)
+    with pkg.ToposMuxFile(lib, str(tmp_path / "big.mov"), abi.movie_config_for(cfg, header)) as mx:
+        for i in range(n):
+            mx.add_packet(pkt, i, 1)
+        mx.finish()
+        data = mx.data
+    assert len(data) > (8 << 20)
+    with pkg.ToposMovieFile(lib, str(tmp_path / "big.mov")) as mv:
+        arena, offs, sizes = mv.packets_into_batch(0, n)
+        tail = mv.packets_into_batch(1, n - 1)[0]
+    assert arena == pkt * n and tail == pkt * (n - 1) and sizes == [len(pkt)] * n
+    with pkg.ToposMovieFile(reflib, str(tmp_path / "big.mov")) as mv:
+        assert mv.packets_into_batch(0, n)[0] == arena

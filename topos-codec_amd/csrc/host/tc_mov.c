@@ -11,6 +11,7 @@
  * skips sound and timecode tracks. */
 #define _GNU_SOURCE
 #include <errno.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #include <unistd.h>
@@ -1327,12 +1328,53 @@ int32_t tc_movie_packet(const topos_movie* mv, uint32_t sample_index, uint8_t* b
     return rc != TC_OK ? rc : sample_check(mv, sample_index, buf, sz);
 }
 
-int32_t tc_movie_packet_batch(const topos_movie* mv, uint32_t start, uint32_t count, uint8_t* arena, size_t cap,
-                              size_t* offsets_out, size_t* sizes_out, size_t* need_total)
+/* parallel pread of one contiguous file range (descriptor sources only:
+ * pread needs no shared file position) */
+typedef struct pread_part { int fd; uint64_t off; uint8_t* dst; size_t len; int32_t rc; } pread_part;
+
+static void* pread_worker(void* arg)
 {
-    if (mv == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "movie NULL"); }
-    if (count == 0u) { return TC_OK; }
-    if (start >= mv->n || count > mv->n - start) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "sample range"); }
+    pread_part* p = (pread_part*)arg;
+    p->rc = fd_read((void*)(intptr_t)p->fd, p->off, p->dst, p->len);
+    return NULL;
+}
+
+#define MOV_READ_THREADS 8
+#define MOV_READ_SPLIT ((size_t)4 << 20)
+
+static int32_t read_span(const topos_movie* mv, uint64_t off, uint8_t* dst, size_t len)
+{
+    if (mv->fd < 0 || len < 2u * MOV_READ_SPLIT) { return io_read(mv, off, dst, len); }
+    if (off > mv->io.length || len > mv->io.length - off) { return tc_fail(TC_ERR_TRUNCATED, "read past end"); }
+    size_t parts = len / MOV_READ_SPLIT;
+    if (parts > MOV_READ_THREADS) { parts = MOV_READ_THREADS; }
+    pread_part pp[MOV_READ_THREADS];
+    pthread_t th[MOV_READ_THREADS];
+    int started[MOV_READ_THREADS] = { 0 };
+    const size_t step = (len / parts + 4095u) & ~(size_t)4095u;
+    size_t o = 0;
+    for (size_t k = 0; k < parts; ++k) {
+        const size_t n = k + 1u == parts ? len - o : step;
+        pp[k] = (pread_part){ mv->fd, off + o, dst + o, n, TC_OK };
+        o += n;
+        /* part 0 runs on this thread; a part whose thread cannot start runs here too */
+        started[k] = k > 0u && pthread_create(&th[k], NULL, pread_worker, &pp[k]) == 0;
+    }
+    int32_t rc = TC_OK;
+    for (size_t k = 0; k < parts; ++k) {
+        if (!started[k]) { pread_worker(&pp[k]); }
+    }
+    for (size_t k = 0; k < parts; ++k) {
+        if (started[k]) { pthread_join(th[k], NULL); }
+        if (pp[k].rc != TC_OK && rc == TC_OK) { rc = tc_fail(TC_ERR_IO, "io read failed"); }
+    }
+    return rc;
+}
+
+/* bounds of samples [start, start + count) and their packed layout */
+static int32_t batch_layout(const topos_movie* mv, uint32_t start, uint32_t count, size_t* offsets_out,
+                            size_t* sizes_out, uint64_t* total_out)
+{
     uint64_t total = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const uint64_t off = mv->offsets[start + i], sz = mv->sizes[start + i];
@@ -1343,15 +1385,22 @@ int32_t tc_movie_packet_batch(const topos_movie* mv, uint32_t start, uint32_t co
         if (sizes_out != NULL) { sizes_out[i] = (size_t)sz; }
         total += sz;
     }
-    if (need_total != NULL) { *need_total = (size_t)total; }
-    if (arena == NULL || cap < total) { return TC_ERR_BUFFER_TOO_SMALL; }
+    if (total > (uint64_t)SIZE_MAX) { return tc_fail(TC_ERR_MALFORMED, "range bytes overflow"); }
+    *total_out = total;
+    return TC_OK;
+}
+
+/* the packed samples into arena (one read when contiguous in the file), then
+ * every header checked against the track */
+static int32_t batch_fill(const topos_movie* mv, uint32_t start, uint32_t count, uint8_t* arena, uint64_t total)
+{
     int contiguous = 1;
     for (uint32_t i = 1; i < count && contiguous; ++i) {
         contiguous = mv->offsets[start + i] == mv->offsets[start + i - 1u] + mv->sizes[start + i - 1u];
     }
     int32_t rc = TC_OK;
     if (contiguous) {
-        rc = io_read(mv, mv->offsets[start], arena, (size_t)total);
+        rc = total == 0u ? TC_OK : read_span(mv, mv->offsets[start], arena, (size_t)total);
     } else {
         size_t o = 0;
         for (uint32_t i = 0; i < count && rc == TC_OK; ++i) {
@@ -1365,6 +1414,20 @@ int32_t tc_movie_packet_batch(const topos_movie* mv, uint32_t start, uint32_t co
         o += mv->sizes[start + i];
     }
     return rc;
+}
+
+int32_t tc_movie_packet_batch(const topos_movie* mv, uint32_t start, uint32_t count, uint8_t* arena, size_t cap,
+                              size_t* offsets_out, size_t* sizes_out, size_t* need_total)
+{
+    if (mv == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "movie NULL"); }
+    if (count == 0u) { return TC_OK; }
+    if (start >= mv->n || count > mv->n - start) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "sample range"); }
+    uint64_t total = 0;
+    int32_t rc = batch_layout(mv, start, count, offsets_out, sizes_out, &total);
+    if (rc != TC_OK) { return rc; }
+    if (need_total != NULL) { *need_total = (size_t)total; }
+    if (arena == NULL || cap < total) { return TC_ERR_BUFFER_TOO_SMALL; }
+    return batch_fill(mv, start, count, arena, total);
 }
 
 int32_t tc_movie_packet_pts(const topos_movie* mv, uint32_t sample_index, uint64_t* pts_tick, uint32_t* dur_tick)
@@ -1482,9 +1545,51 @@ int32_t tc_movie_faststart(const topos_io* src, const topos_io* dst)
     return rc;
 }
 
+/* page-locked staging for tcx_movie_decode_batch_dev: a few grow-only
+ * buffers shared by all movies, one per concurrent call (a call that finds
+ * every slot busy stages through ordinary memory) */
+#define MOV_PIN_SLOTS 4
+static struct { void* p; size_t cap; int busy; } g_pin[MOV_PIN_SLOTS];
+static pthread_mutex_t g_pin_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int pin_take(size_t bytes, uint8_t** out)
+{
+    int slot = -1;
+    pthread_mutex_lock(&g_pin_mu);
+    for (int k = 0; k < MOV_PIN_SLOTS; ++k) {
+        if (!g_pin[k].busy && (slot < 0 || (g_pin[k].cap >= bytes && g_pin[slot].cap < bytes))) { slot = k; }
+    }
+    if (slot >= 0) { g_pin[slot].busy = 1; }
+    pthread_mutex_unlock(&g_pin_mu);
+    if (slot < 0) { return -1; }
+    if (g_pin[slot].cap < bytes) {
+        tcg_pinned_free(g_pin[slot].p);
+        const size_t cap = (bytes + ((size_t)1 << 20) - 1u) & ~(((size_t)1 << 20) - 1u);
+        g_pin[slot].p = tcg_pinned_alloc(cap);
+        g_pin[slot].cap = g_pin[slot].p != NULL ? cap : 0u;
+    }
+    if (g_pin[slot].p == NULL) {
+        pthread_mutex_lock(&g_pin_mu);
+        g_pin[slot].busy = 0;
+        pthread_mutex_unlock(&g_pin_mu);
+        return -1;
+    }
+    *out = (uint8_t*)g_pin[slot].p;
+    return slot;
+}
+
+static void pin_give(int slot)
+{
+    if (slot < 0) { return; }
+    pthread_mutex_lock(&g_pin_mu);
+    g_pin[slot].busy = 0;
+    pthread_mutex_unlock(&g_pin_mu);
+}
+
 /* a run of samples straight into one HBM arena and decoded as one batch (no
  * reference counterpart): the packets of samples [start, start + count) are
- * read, checked, uploaded once, then decoded into dev_planes (count x 4) */
+ * read into page-locked staging (in parallel for descriptor sources), checked,
+ * uploaded with one DMA copy, then decoded into dev_planes (count x 4) */
 int32_t tcx_movie_decode_batch_dev(const topos_movie* mv, uint32_t start, uint32_t count, uint64_t dev_arena,
                                    size_t arena_cap, const uint64_t* dev_planes, const size_t* dev_strides,
                                    topos_frame_output* infos, int32_t* rcs)
@@ -1492,35 +1597,39 @@ int32_t tcx_movie_decode_batch_dev(const topos_movie* mv, uint32_t start, uint32
     if (mv == NULL || count == 0u || dev_planes == NULL || infos == NULL || rcs == NULL) {
         return tc_fail(TC_ERR_INVALID_ARGUMENT, "movie batch arguments");
     }
-    size_t total = 0;
+    if (start >= mv->n || count > mv->n - start) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "sample range"); }
+    uint64_t total = 0;
     size_t* offs = (size_t*)calloc(count, sizeof(size_t));
     size_t* sizes = (size_t*)calloc(count, sizeof(size_t));
-    uint8_t** hp = (uint8_t**)calloc(count, sizeof(uint8_t*));
+    const uint8_t** hp = (const uint8_t**)calloc(count, sizeof(uint8_t*));
     uint64_t* dp = (uint64_t*)calloc(count, sizeof(uint64_t));
     uint8_t* host = NULL;
+    int slot = -1;
     int32_t rc = offs && sizes && hp && dp ? TC_OK : tc_fail(TC_ERR_OUT_OF_MEMORY, "movie batch");
-    if (rc == TC_OK) {
-        rc = tc_movie_packet_batch(mv, start, count, NULL, 0u, offs, sizes, &total);
-        if (rc == TC_ERR_BUFFER_TOO_SMALL) { rc = TC_OK; }
-    }
+    if (rc == TC_OK) { rc = batch_layout(mv, start, count, offs, sizes, &total); }
     /* the arena comes from tcx_dev_alloc, whose tail slack covers the
      * decoder's read-ahead past the last packet */
     if (rc == TC_OK && total > arena_cap) {
-        rc = tc_fail(TC_ERR_BUFFER_TOO_SMALL, "device arena %zu < %zu bytes", arena_cap, total);
+        rc = tc_fail(TC_ERR_BUFFER_TOO_SMALL, "device arena %zu < %llu bytes", arena_cap, (unsigned long long)total);
     }
-    if (rc == TC_OK && (host = (uint8_t*)malloc(total ? total : 1u)) == NULL) {
+    if (rc == TC_OK && (slot = pin_take((size_t)total, &host)) < 0 &&
+        (host = (uint8_t*)malloc(total ? (size_t)total : 1u)) == NULL) {
         rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "movie batch staging");
     }
-    if (rc == TC_OK) { rc = tc_movie_packet_batch(mv, start, count, host, total, offs, sizes, &total); }
-    if (rc == TC_OK) { rc = tcx_h2d(dev_arena, host, total); }
+    if (rc == TC_OK) { rc = batch_fill(mv, start, count, host, total); }
+    if (rc == TC_OK) { rc = tcx_h2d(dev_arena, host, (size_t)total); }
     if (rc == TC_OK) {
         for (uint32_t i = 0; i < count; ++i) {
             hp[i] = host + offs[i];
             dp[i] = dev_arena + offs[i];
         }
-        rc = tcx_decode_batch_dev(count, (const uint8_t* const*)hp, sizes, dp, dev_planes, dev_strides, infos, rcs);
+        rc = tcx_decode_batch_dev(count, hp, sizes, dp, dev_planes, dev_strides, infos, rcs);
     }
-    free(host);
+    if (slot >= 0) {
+        pin_give(slot);
+    } else {
+        free(host);
+    }
     free(offs);
     free(sizes);
     free(hp);

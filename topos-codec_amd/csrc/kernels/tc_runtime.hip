@@ -171,6 +171,21 @@ int32_t tcg_d2d(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes)
     return e == hipSuccess ? 0 : fail("device copy", e);
 }
 
+void* tcg_pinned_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void tcg_pinned_free(void* p)
+{
+    if (p != nullptr) { (void)hipHostFree(p); }
+}
+
 int32_t tcg_malloc(uint64_t* dptr, size_t bytes)
 {
     void* p = nullptr;

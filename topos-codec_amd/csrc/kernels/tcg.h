@@ -209,6 +209,9 @@ int32_t tcg_query(tcg_ctx* c);
 int32_t tcg_copy(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes);
 /* byte-exact device-to-device copy, queued on the context's stream */
 int32_t tcg_d2d(tcg_ctx* c, uint64_t dst, uint64_t src, size_t bytes);
+/* page-locked host memory (hipHostMalloc), NULL on failure */
+void* tcg_pinned_alloc(size_t bytes);
+void tcg_pinned_free(void* p);
 /* upload the (12 - log2 f) cost table, f = 0..4096 (entry 0 unused), and
  * the CRC32 tables (bytewise table, x^(2^k) mod P) */
 int32_t tcg_set_tables(const double* cost4097, const uint32_t* crc256, const uint32_t* x2n32);
