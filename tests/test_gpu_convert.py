@@ -50,6 +50,21 @@ def test_convert_every_8bit_colour(oracle, abi, codec):
             assert np.array_equal(a, b), (mode, bd, fr)
 
 
+def test_convert_every_16bit_sample(oracle, abi, codec):
+    """Every 16-bit value in every channel (R, G, B, A permuted differently) of
+    an RGBA64 frame, all three output modes with alpha, against the oracle."""
+    v = np.arange(1 << 16, dtype=np.uint32)
+    px = np.stack([v, (v * 40503) & 0xFFFF, 0xFFFF - v, (v * 7919) & 0xFFFF], 1).astype("<u2")
+    src = px.view(np.uint8).reshape(256, 256 * 8)
+    for mode, bd, fr in ((0, 10, 0), (1, 12, 1), (2, 12, 0)):
+        kw = dict(bit_depth=bd, full_range=fr, with_alpha=True)
+        rc_o, _, fo = cvt_call(oracle.orc_convert_packed_rgb, abi, src, 256, 256, abi.CVT_RGBA64, mode, **kw)
+        rc_g, _, fg = cvt_call(codec._lib.tc_convert_packed_rgb, abi, src, 256, 256, abi.CVT_RGBA64, mode, **kw)
+        assert rc_o == 0 and rc_g == 0
+        for k, (a, b) in enumerate(zip(fo, fg)):
+            assert np.array_equal(a, b), (mode, bd, fr, k)
+
+
 @pytest.mark.parametrize("fmt", [2, 7])
 def test_convert_4k_binding_and_device(oracle, abi, codec, fmt):
     rng = np.random.default_rng(12 + fmt)

@@ -201,3 +201,44 @@ def test_ctx_log2_matches_reference(pkg, ref):
         assert lib.tc_ctx_log2(float(1 << k)) == float(k)
     for f in range(1, 4097, 37):
         assert abs(lib.tc_ctx_log2(float(f)) - math.log2(f)) < 1e-7  # series to z^13
+
+
+def _round_f32(x):
+    """Nearest float32 (ties to even) of an exact rational; normal range only."""
+    from fractions import Fraction
+    if x == 0:
+        return np.float32(0.0)
+    sign = -1 if x < 0 else 1
+    x = abs(x)
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    while Fraction(2) ** e > x:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= x:
+        e += 1
+    assert -126 <= e <= 127
+    m = x / Fraction(2) ** (e - 23)
+    n = m.numerator // m.denominator
+    rem = m - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return np.float32(sign * n * 2.0 ** (e - 23))
+
+
+@pytest.mark.parametrize("d", [255, 65535])
+def test_conversion_normalisation_exact(d):
+    """The conversion kernel's c / d for integer samples (tc_convert.hip cvt_px):
+    q0 = c * r, res = fma(-q0, d, c), q = fma(res, r, q0) with r = 1 / d
+    correctly rounded, each fma rounded once, equals the IEEE quotient for every
+    c in 0..d."""
+    from fractions import Fraction
+    df = np.float32(d)
+    r = np.float32(1.0) / df
+    c = np.arange(d + 1, dtype=np.float32)
+    want = c / df
+    q0 = c * r
+    fr, fd = Fraction(float(r)), Fraction(d)
+    for i in range(d + 1):
+        ci, q = Fraction(int(c[i])), Fraction(float(q0[i]))
+        res = _round_f32(ci - q * fd)
+        got = _round_f32(Fraction(float(res)) * fr + q)
+        assert got == want[i], (d, i)

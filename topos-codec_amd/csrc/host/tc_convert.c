@@ -44,6 +44,7 @@ static int32_t cvt_plan(const topos_cvt_params* p, tcg_cvt* k)
     k->alpha_shift = p->alpha_shift > 15u ? 15u : p->alpha_shift;
     k->a_nmax = (float)(0xFFFFu >> k->alpha_shift);
     k->norm_div = k->smp == 2u ? 65535.0f : 255.0f;
+    k->norm_rcp = 1.0f / k->norm_div;
     if (p->format == TOPOS_CVT_BGR24 || p->format == TOPOS_CVT_BGRA32 || p->format == TOPOS_CVT_BGR48 ||
         p->format == TOPOS_CVT_BGRA64) {
         k->ro = 2u; k->go = 1u; k->bo = 0u; k->ao = 3u;

@@ -7,20 +7,15 @@
 //   planar 444->422 native/src/common/planar_convert.c:26-45
 //
 // Float32 throughout in the reference's operation order; this file is built
-// with contraction off (no FMA) and HIP's IEEE-correct single-precision
-// division, so every rounding matches the host build. rintf rounds half to
+// with contraction off (no implicit FMA) and HIP's IEEE-correct single-precision
+// division, so every rounding matches the host build; the sample
+// normalisation uses an exact reciprocal form (cvt_norm). rintf rounds half to
 // even (v_rndne_f32).
 #include <hip/hip_runtime.h>
 
 #include "tcg.h"
 
 namespace {
-
-__device__ inline float cvt_load(const uint8_t* row, uint32_t off, uint32_t smp)
-{
-    if (smp == 1u) { return (float)row[off]; }
-    return (float)((uint32_t)row[off] | ((uint32_t)row[off + 1u] << 8));
-}
 
 // clip(rint(v), 0, scale)
 __device__ inline uint16_t cvt_q(float v, float scale)
@@ -38,16 +33,81 @@ __device__ inline uint16_t cvt_box2(uint16_t a, uint16_t b)
     return (uint16_t)rintf(((float)a + (float)b) * 0.5f);
 }
 
-__device__ inline float cvt_px(const tcg_cvt& k, const uint8_t* row, uint32_t ch, uint32_t x)
+// sample / norm_div, correctly rounded like the IEEE division, for integer
+// samples 0..norm_div: q0 = c r (r the correctly rounded 1 / d) is within an
+// ulp of c / d, the residual c - q0 d is exact in one fma, and q0 + residual r
+// in one more fma rounds to the nearest float (Markstein's correction); equal
+// to c / d for every 8-bit and 16-bit sample (checked exhaustively)
+__device__ inline float cvt_norm(const tcg_cvt& k, uint32_t v)
 {
-    return cvt_load(row, (x * k.nch + ch) * k.smp, k.smp);
+#pragma clang fp contract(off)
+    const float c = (float)v;
+    const float q0 = c * k.norm_rcp;
+    const float res = __builtin_fmaf(-q0, k.norm_div, c);
+    return __builtin_fmaf(res, k.norm_rcp, q0);
 }
 
-__device__ inline uint16_t cvt_alpha(const tcg_cvt& k, const uint8_t* row, uint32_t x)
+// the pixel pair's channels (x0 and, when two, x0 + 1), normalised, in
+// storage order: 4-channel rows with a 16-byte aligned base and stride are
+// read with one aligned vector load per pair, other rows sample by sample
+__device__ inline void cvt_pair(const tcg_cvt& k, const uint8_t* row, uint32_t x0, bool two, float c[2][4])
+{
+    if (k.vec && k.smp == 1u) {
+        uint32_t w0, w1 = 0u;
+        if (two) {
+            const uint2 w = *(const uint2*)(row + x0 * 4u);
+            w0 = w.x;
+            w1 = w.y;
+        } else {
+            w0 = *(const uint32_t*)(row + x0 * 4u);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j) {
+            c[0][j] = cvt_norm(k, (w0 >> (8u * j)) & 255u);
+            c[1][j] = cvt_norm(k, (w1 >> (8u * j)) & 255u);
+        }
+    } else if (k.vec) {
+        uint32_t w[4] = { 0u, 0u, 0u, 0u };
+        if (two) {
+            const uint4 v = *(const uint4*)(row + x0 * 8u);
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+        } else {
+            const uint2 v = *(const uint2*)(row + x0 * 8u);
+            w[0] = v.x; w[1] = v.y;
+        }
+#pragma unroll
+        for (uint32_t p = 0; p < 2u; ++p) {
+            c[p][0] = cvt_norm(k, w[2u * p] & 0xFFFFu);
+            c[p][1] = cvt_norm(k, w[2u * p] >> 16);
+            c[p][2] = cvt_norm(k, w[2u * p + 1u] & 0xFFFFu);
+            c[p][3] = cvt_norm(k, w[2u * p + 1u] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t p = 0; p < 2u; ++p) {
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j) {
+                const bool live = (p == 0u || two) && j < k.nch;
+                const uint32_t off = ((x0 + p) * k.nch + j) * k.smp;
+                uint32_t v = 0u;
+                if (live) { v = k.smp == 1u ? row[off] : ((uint32_t)row[off] | ((uint32_t)row[off + 1u] << 8)); }
+                c[p][j] = cvt_norm(k, v);
+            }
+        }
+    }
+}
+
+// channel i (0..3) of a pixel without dynamic register indexing
+__device__ inline float cvt_ch(const float* c, uint32_t i)
+{
+    return i == 0u ? c[0] : (i == 1u ? c[1] : (i == 2u ? c[2] : c[3]));
+}
+
+__device__ inline uint16_t cvt_alpha(const tcg_cvt& k, const float* c)
 {
 #pragma clang fp contract(off)
     if (k.opaque || k.nch == 3u) { return (uint16_t)(0xFFFFu << k.alpha_shift); }
-    const float af = cvt_px(k, row, k.ao, x) / k.norm_div;
+    const float af = cvt_ch(c, k.ao);
     float v = rintf(af * k.a_nmax);
     if (v < 0.0f) { v = 0.0f; }
     if (v > k.a_nmax) { v = k.a_nmax; }
@@ -55,13 +115,12 @@ __device__ inline uint16_t cvt_alpha(const tcg_cvt& k, const uint8_t* row, uint3
 }
 
 // y = kr r + kf g, then + kb b (left to right); u = (b - y) / du, v = (r - y) / dv
-__device__ inline void cvt_yuv(const tcg_cvt& k, const uint8_t* row, uint32_t x, uint16_t& yo, uint16_t& uo,
-                               uint16_t& vo)
+__device__ inline void cvt_yuv(const tcg_cvt& k, const float* c, uint16_t& yo, uint16_t& uo, uint16_t& vo)
 {
 #pragma clang fp contract(off)
-    const float rf = cvt_px(k, row, k.ro, x) / k.norm_div;
-    const float gf = cvt_px(k, row, k.go, x) / k.norm_div;
-    const float bf = cvt_px(k, row, k.bo, x) / k.norm_div;
+    const float rf = cvt_ch(c, k.ro);
+    const float gf = cvt_ch(c, k.go);
+    const float bf = cvt_ch(c, k.bo);
     const float p0 = k.kr * rf;
     const float p1 = k.kf * gf;
     float y = p0 + p1;
@@ -91,24 +150,26 @@ __global__ __launch_bounds__(256) void k_convert_rgb(const tcg_cvt k)
     uint16_t* ao = k.a ? (uint16_t*)k.a + (size_t)y * k.a_st : nullptr;
     const uint32_t x0 = cx * 2u;
     const bool two = x0 + 1u < k.width;
+    float c[2][4];
+    cvt_pair(k, row, x0, two, c);
     if (k.mode == 2u) {
-        for (uint32_t x = x0; x < x0 + (two ? 2u : 1u); ++x) {
-            const float gf = cvt_px(k, row, k.go, x) / k.norm_div;
-            const float bf = cvt_px(k, row, k.bo, x) / k.norm_div;
-            const float rf = cvt_px(k, row, k.ro, x) / k.norm_div;
-            yo[x] = cvt_q(gf * k.scale, k.scale);
-            uo[x] = cvt_q(bf * k.scale, k.scale);
-            vo[x] = cvt_q(rf * k.scale, k.scale);
+#pragma unroll
+        for (uint32_t p = 0; p < 2u; ++p) {
+            if (p == 1u && !two) { break; }
+            yo[x0 + p] = cvt_q(cvt_ch(c[p], k.go) * k.scale, k.scale);
+            uo[x0 + p] = cvt_q(cvt_ch(c[p], k.bo) * k.scale, k.scale);
+            vo[x0 + p] = cvt_q(cvt_ch(c[p], k.ro) * k.scale, k.scale);
         }
     } else if (k.mode == 1u) {
-        for (uint32_t x = x0; x < x0 + (two ? 2u : 1u); ++x) { cvt_yuv(k, row, x, yo[x], uo[x], vo[x]); }
+        cvt_yuv(k, c[0], yo[x0], uo[x0], vo[x0]);
+        if (two) { cvt_yuv(k, c[1], yo[x0 + 1u], uo[x0 + 1u], vo[x0 + 1u]); }
     } else {
         uint16_t y0v, u0v, v0v;
-        cvt_yuv(k, row, x0, y0v, u0v, v0v);
+        cvt_yuv(k, c[0], y0v, u0v, v0v);
         yo[x0] = y0v;
         if (two) {
             uint16_t y1v, u1v, v1v;
-            cvt_yuv(k, row, x0 + 1u, y1v, u1v, v1v);
+            cvt_yuv(k, c[1], y1v, u1v, v1v);
             yo[x0 + 1u] = y1v;
             uo[cx] = cvt_box2(u0v, u1v);
             vo[cx] = cvt_box2(v0v, v1v);
@@ -118,8 +179,8 @@ __global__ __launch_bounds__(256) void k_convert_rgb(const tcg_cvt k)
         }
     }
     if (ao != nullptr) {
-        ao[x0] = cvt_alpha(k, row, x0);
-        if (two) { ao[x0 + 1u] = cvt_alpha(k, row, x0 + 1u); }
+        ao[x0] = cvt_alpha(k, c[0]);
+        if (two) { ao[x0 + 1u] = cvt_alpha(k, c[1]); }
     }
 }
 
@@ -145,7 +206,10 @@ extern "C" int32_t tcg_convert_rgb(tcg_ctx* c, const tcg_cvt* k)
 {
     c->kind = 3;
     tcg_mark(c, 0);
-    hipLaunchKernelGGL(k_convert_rgb, dim3((k->cw + 255u) / 256u, k->height), dim3(256), 0, c->stream, *k);
+    tcg_cvt kv = *k;
+    // aligned vector loads: 4-channel rows whose base and stride are 16-byte aligned
+    kv.vec = kv.nch == 4u && (kv.src & 15u) == 0u && (kv.src_stride & 15u) == 0u;
+    hipLaunchKernelGGL(k_convert_rgb, dim3((kv.cw + 255u) / 256u, kv.height), dim3(256), 0, c->stream, kv);
     tcg_mark(c, 1);
     return tcg_check("convert launch");
 }

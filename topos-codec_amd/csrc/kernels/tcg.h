@@ -171,7 +171,9 @@ typedef struct tcg_cvt {
     uint32_t nch, smp;          /* channels, bytes per sample */
     uint32_t ro, go, bo, ao;    /* channel offsets in samples */
     uint32_t alpha_shift, opaque;
+    uint32_t vec;               /* set by the launcher: aligned 4-channel rows */
     float scale, kr, kb, kf, du, dv, y_mult, y_off, c_mult, c_off, a_nmax, norm_div;
+    float norm_rcp;             /* 1 / norm_div, correctly rounded */
 } tcg_cvt;
 
 /* 4:4:4 -> 4:2:2 chroma (device addresses, strides in elements) */
