@@ -99,6 +99,40 @@ def test_convert_4k_binding_and_device(oracle, abi, codec, fmt):
         assert np.array_equal(got, a)
 
 
+# (format, width, row padding): rows of 270 / 8012 / 534 bytes are read sample
+# by sample; 544 / 4016 / 4016 bytes take the vector loads (odd and even widths)
+@pytest.mark.parametrize("fmt,w,pad", [(2, 67, 2), (6, 1001, 4), (7, 66, 6), (7, 67, 8), (3, 1001, 12),
+                                       (2, 1000, 16)])
+def test_convert_device_strides(oracle, abi, codec, fmt, w, pad):
+    """Device-resident 4-channel sources whose row stride is not a multiple of
+    16 bytes (read sample by sample) and one that is (vector loads), odd and
+    even widths, every output mode, against the oracle."""
+    rng = np.random.default_rng(40 + fmt + w)
+    h = 6
+    src, _ = cvt_source(rng, fmt, abi, w, h)
+    nch, smp = abi.cvt_channels(fmt), abi.cvt_sample_bytes(fmt)
+    pitch = w * nch * smp + pad
+    assert (pitch % 16 == 0) == (pad in (8, 12, 16))
+    padded = np.zeros((h, pitch), np.uint8)
+    padded[:, : w * nch * smp] = src
+    dsrc = codec.alloc(pitch * h)
+    dsrc.upload(padded)
+    for mode in (0, 1, 2):
+        rc, _, fo = cvt_call(oracle.orc_convert_packed_rgb, abi, src, w, h, fmt, mode, with_alpha=True)
+        assert rc == 0
+        shapes = abi.cvt_plane_shapes(w, h, mode, True)
+        outs = [codec.alloc(s[1] * s[0] * 2) for s in shapes]
+        p = abi.CvtParams()
+        p.format, p.width, p.height, p.out_mode, p.bit_depth = fmt, w, h, mode, 10
+        p.kr, p.kb = 0.2126, 0.0722
+        p.src, p.src_stride = dsrc.addr, pitch
+        p.y_out, p.u_out, p.v_out, p.a_out = (b.addr for b in outs)
+        assert codec.convert_packed_rgb_dev(p) == 0, codec.last_error()
+        for b, sh, want in zip(outs, shapes, fo):
+            got = b.download(sh[1] * sh[0] * 2, np.uint16).reshape(sh[0], sh[1])
+            assert np.array_equal(got, want), (fmt, w, pad, mode)
+
+
 def test_planar_444_to_422(oracle, abi, codec):
     rng = np.random.default_rng(13)
     for w, h in ((1, 3), (3, 2), (257, 5), (3840, 2160)):
