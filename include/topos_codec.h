@@ -177,8 +177,78 @@ typedef struct topos_decoder_config {
 
 typedef struct tc_decoder tc_decoder;
 
+/* library version (reference topos_codec.h:58-67); the strings are static */
+typedef struct topos_version_info {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t version_major;
+    uint32_t version_minor;
+    uint32_t version_patch;
+    const char* git_commit;
+    const char* build_target;
+    uint32_t reserved[4];
+} topos_version_info;
+
+/* host CPU feature bits (reference topos_codec.h:69-80); informational only */
+#define TOPOS_CPU_X86_AVX2 (1u << 0)
+#define TOPOS_CPU_X86_AVX512F (1u << 1)
+#define TOPOS_CPU_ARM_NEON (1u << 2)
+#define TOPOS_CPU_X86_FMA (1u << 3)
+
+typedef struct topos_cpu_features {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t flags;
+    uint32_t reserved[5];
+} topos_cpu_features;
+
+/* codec capability bits (reference topos_codec.h:82-104) */
+#define TC_CODEC_CAP_FULL_DECODE (1u << 0)
+#define TC_CODEC_CAP_FIXED_REDUCED_DECODE (1u << 1)
+#define TC_CODEC_CAP_AUTO_2K_DECODE (1u << 2)
+#define TC_CODEC_CAP_CPU_SURFACE_OUTPUT (1u << 3)
+#define TC_CODEC_CAP_HOST_VISIBLE_GPU_SURFACE (1u << 4)
+#define TC_CODEC_CAP_V7A_BAND_DECODE (1u << 5)
+#define TC_CODEC_CAP_V7B_SCALABLE_BASE (1u << 6)
+#define TC_CODEC_CAP_DECODE_DROP_ALPHA (1u << 7)
+
+#define TC_CODEC_MAX_DIM 16384u
+#define TC_CODEC_AUTO_2K_MAX_DIM 2048u
+
+typedef struct topos_codec_capabilities {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t flags;
+    uint32_t max_width;
+    uint32_t max_height;
+    uint32_t auto2k_max_dim;
+    uint32_t reserved[4];
+} topos_codec_capabilities;
+
+/* release state of the scalable (V7-B) layout (reference topos_codec.h:110-115) */
+typedef enum topos_scalable_status {
+    TC_SCALABLE_STATUS_EXPERIMENTAL = 0,
+    TC_SCALABLE_STATUS_REFERENCE = 1,
+    TC_SCALABLE_STATUS_ELIGIBLE = 2,
+    TC_SCALABLE_STATUS_DEFAULT = 3
+} topos_scalable_status;
+
 /* reference topos_codec.h:125 */
 int32_t tc_abi_version(void);
+/* reference topos_codec.h:128: TC_ERR_INVALID_ARGUMENT when out == NULL */
+int32_t tc_version(topos_version_info* out);
+/* reference topos_codec.h:131 */
+int32_t tc_query_cpu_features(topos_cpu_features* out);
+/* reference topos_codec.h:135: the decode paths this library provides */
+int32_t tc_query_capabilities(topos_codec_capabilities* out);
+/* reference topos_codec.h:142-143: TC_OK when a stream of this profile /
+ * pixel format / bit depth / alpha mode decodes here, else the
+ * TC_ERR_UNSUPPORTED_* code of the first field that fails */
+int32_t tc_query_support(uint32_t profile, uint32_t pixel_format, uint32_t bit_depth, uint32_t alpha_mode);
+/* reference topos_codec.h:118, 122: V7-B is not written or read by this
+ * library, so the state reported is TC_SCALABLE_STATUS_EXPERIMENTAL */
+int32_t tc_query_scalable_status(topos_scalable_status* out);
+const char* tc_scalable_status_name(topos_scalable_status status);
 /* reference topos_codec.h:146 */
 const char* tc_status_message(int32_t status);
 /* reference topos_codec.h:150 */
@@ -303,6 +373,30 @@ int32_t tc_decoder_decode_request(tc_decoder* dec, const uint8_t* packet, size_t
 int32_t tc_decoder_decode_batch_request(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
                                         const topos_decode_request* request, const topos_plane_view* out,
                                         topos_frame_output* infos);
+
+/* caller-owned output surface (reference topos_codec.h:495-500); the planes
+ * are written during the call only and no pointer is kept */
+typedef struct topos_decode_surface {
+    uint32_t struct_size;
+    uint32_t abi_version;
+    uint32_t memory_type;   /* TC_DECODE_MEMORY_* */
+    uint32_t reserved;
+    topos_plane_view planes[TC_FRAME_MAX_PLANES];
+} topos_decode_surface;
+
+/* reference topos_codec.h:521: the context's worker budget (0 = default).
+ * Slices of a frame run on the GPU in one launch, so the value is kept and
+ * reported but does not change the result or the launch. */
+int32_t tc_decoder_set_max_slice_workers(tc_decoder* dec, uint32_t workers);
+/* reference topos_codec.h:563-565: tc_decoder_decode into surface->planes
+ * after the surface's memory type and struct_size / abi_version are checked */
+int32_t tc_decoder_decode_surface(tc_decoder* dec, const uint8_t* packet, size_t size,
+                                  const topos_decode_surface* surface, topos_frame_output* info);
+/* reference topos_codec.h:568-571: request->memory_type must equal the
+ * surface's */
+int32_t tc_decoder_decode_surface_request(tc_decoder* dec, const uint8_t* packet, size_t size,
+                                          const topos_decode_request* request,
+                                          const topos_decode_surface* surface, topos_frame_output* info);
 
 /* ---- packed RGB(A) -> planar conversion (reference topos_codec.h:1016-1102) ----
  * The step before encode in every export: packed 8-bit or 16-bit RGB(A)

@@ -264,6 +264,13 @@ void tcx_graph_stats(uint64_t* replays, uint64_t* captures);
 void tcx_set_timing(int on);
 void tcx_get_timing(float out[7]);
 
+/* the host-built constant tables the kernels receive, for checking them:
+ * tcx_ctx_log2 is the model-choice log2 series (f in 1..4096 gives the
+ * per-symbol cost 12 - log2 f); tcx_crc_tables gives the byte table of the
+ * reflected CRC32 (0xEDB88320) and x^(2^k) mod P for k = 0..31 */
+double tcx_ctx_log2(double x);
+void tcx_crc_tables(uint32_t crc256[256], uint32_t x2n32[32]);
+
 #ifdef __cplusplus
 }
 #endif

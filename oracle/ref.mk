@@ -1,4 +1,5 @@
-# Builds the reference C11 library straight from its sources under
+# Builds the reference C11 library (Release flags: -O3 -DNDEBUG, as its CMake
+# Release configuration) straight from its sources under
 # /root/reference/native (read-only) into oracle/_ref/ for use as a checker.
 # No reference file is copied; only the shared object lands in oracle/_ref/.
 # Usage: make -f oracle/ref.mk [-j8]
@@ -21,7 +22,7 @@ SRCS := version.c common/error.c common/alloc.c common/color_convert.c \
         image/half_map.c simd/dispatch.c simd/transform_avx2.c \
         simd/color_convert_avx2.c simd/planar_convert_avx2.c simd/transform_neon.c
 OBJS := $(patsubst %.c,$(OUT)/obj/%.o,$(SRCS))
-CFLAGS := -O2 -std=c11 -fPIC -I$(REF)/include -I$(REF)/src \
+CFLAGS := -O3 -DNDEBUG -std=c11 -fPIC -I$(REF)/include -I$(REF)/src \
           -DTOPOS_GIT_COMMIT=\"reference\" -DTOPOS_BUILD_TARGET=\"x86_64-gcc\" -D_GNU_SOURCE
 
 all: $(OUT)/libtopos_ref.so

@@ -141,7 +141,7 @@ def test_crc_combine_matches_zlib(mc, pkg):
     lib = pkg.load_library()
     t = (ctypes.c_uint32 * 256)()
     x2n = (ctypes.c_uint32 * 32)()
-    lib.tc_crc_tables(t, x2n)
+    lib.tcx_crc_tables(t, x2n)
     assert list(t) == crc_tables()
     rng = np.random.default_rng(3)
     for n in [0, 1, 2, 63, 64, 65, 255, 1000, 4097, 80000]:
@@ -188,19 +188,19 @@ def test_row_tables_match_reference(mc, ref):
 
 def test_ctx_log2_matches_reference(pkg, ref):
     lib = pkg.load_library()
-    lib.tc_ctx_log2.restype = ctypes.c_double
-    lib.tc_ctx_log2.argtypes = [ctypes.c_double]
+    lib.tcx_ctx_log2.restype = ctypes.c_double
+    lib.tcx_ctx_log2.argtypes = [ctypes.c_double]
     ref.tc_ctx_log2.restype = ctypes.c_double
     ref.tc_ctx_log2.argtypes = [ctypes.c_double]
     for f in range(1, 4097):
-        a, b = lib.tc_ctx_log2(float(f)), ref.tc_ctx_log2(float(f))
+        a, b = lib.tcx_ctx_log2(float(f)), ref.tc_ctx_log2(float(f))
         assert a.hex() == b.hex(), f
     # closed form sanity: exact at powers of two, close to log2 elsewhere
     import math
     for k in range(13):
-        assert lib.tc_ctx_log2(float(1 << k)) == float(k)
+        assert lib.tcx_ctx_log2(float(1 << k)) == float(k)
     for f in range(1, 4097, 37):
-        assert abs(lib.tc_ctx_log2(float(f)) - math.log2(f)) < 1e-7  # series to z^13
+        assert abs(lib.tcx_ctx_log2(float(f)) - math.log2(f)) < 1e-7  # series to z^13
 
 
 def _round_f32(x):

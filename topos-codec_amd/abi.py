@@ -504,3 +504,43 @@ def target_plane_dims(tw, th, pixel_format, plane_count):
         return [((tw + 1) // 2, (th + 1) // 2)] * plane_count
     cw = (tw + 1) // 2 if pixel_format == 0 else tw
     return [(tw, th), (cw, th), (cw, th), (tw, th)][:plane_count]
+
+
+# ---- version / capability negotiation and decode surfaces (include/topos_codec.h) ----
+class VersionInfo(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+                ("version_major", ctypes.c_uint32), ("version_minor", ctypes.c_uint32),
+                ("version_patch", ctypes.c_uint32), ("git_commit", ctypes.c_char_p),
+                ("build_target", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class CpuFeatures(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
+
+
+class CodecCapabilities(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("max_width", ctypes.c_uint32), ("max_height", ctypes.c_uint32),
+                ("auto2k_max_dim", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
+TC_DECODE_MEMORY_CPU = 0
+TC_DECODE_MEMORY_HOST_VISIBLE_GPU = 1
+
+
+class DecodeSurface(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+                ("memory_type", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("planes", PlaneView * 4)]
+
+
+def plane_views(arrays):
+    """Four topos_plane_view entries over numpy planes (tight strides)."""
+    views = (PlaneView * 4)()
+    for i, a in enumerate(arrays):
+        views[i].struct_size = ctypes.sizeof(PlaneView)
+        views[i].abi_version = TOPOS_CODEC_ABI_VERSION
+        views[i].pixels = a.ctypes.data
+        views[i].stride = a.shape[1]
+    return views

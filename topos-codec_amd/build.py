@@ -8,6 +8,7 @@ Run:  python topos-codec_amd/build.py [--jobs N]
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -18,7 +19,8 @@ BUILD = os.path.join(PKG, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TOPOS_GPU_ARCH", "gfx950")
 
-HOST_C = ["tc_status.c", "tc_format.c", "tc_vlc.c", "tc_engine.c", "tc_api.c", "tc_gpu_api.c", "tc_image.c",
+HOST_C = ["tc_status.c", "tc_format.c", "tc_vlc.c", "tc_engine.c", "tc_api.c", "tc_decoder.c", "tc_gpu_api.c",
+          "tc_image.c",
           "tc_arena.c", "tc_serve.c", "tc_tune.c",
           "tc_convert.c", "tc_sized.c", "tc_sized_search.c", "tc_preview.c", "tc_mov.c"]
 HIP_SRC = ["tc_runtime.hip", "tc_encode.hip", "tc_decode.hip", "tc_alpha.hip", "tc_bits.hip", "tc_convert.hip"]
@@ -29,6 +31,31 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-W
 HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "kernels")]
+
+
+def exported_names(include_dir=os.path.join(ROOT, "include")):
+    """Function names declared in the public headers: the library's whole
+    dynamic symbol table (everything else is bound locally at link time)."""
+    names = set()
+    for h in sorted(os.listdir(include_dir)):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(include_dir, h)).read()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        # a declaration "type name(" -- not a function-pointer typedef "(*name)("
+        for m in re.finditer(r"(?<!\(\*)\b(tcx?_[A-Za-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def version_script(path):
+    with open(path, "w") as f:
+        f.write("{\n  global:\n")
+        for n in exported_names():
+            f.write(f"    {n};\n")
+        f.write("  local: *;\n};\n")
+    return path
 
 
 def run(cmd):
@@ -56,7 +83,12 @@ def build(jobs=8, verbose=False):
                 print(o)
     objs = [os.path.join(BUILD, f + ".o") for f in HOST_C + HIP_SRC]
     lib = os.path.join(PKG, "libtopos_codec_amd.so")
-    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib + ".tmp", *objs, "-lpthread"])
+    # exports = the functions include/*.h declares; calls between them bind
+    # inside the library (-Bsymbolic), so a host that also loads another
+    # Topos library globally never diverts this library's internal calls
+    vs = version_script(os.path.join(BUILD, "exports.map"))
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib + ".tmp", *objs, "-lpthread",
+         "-Wl,--version-script=" + vs, "-Wl,-Bsymbolic"])
     os.replace(lib + ".tmp", lib)
     synth = os.path.join(PKG, "libtopos_synth.so")
     run(["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-shared", "-o", synth + ".tmp",

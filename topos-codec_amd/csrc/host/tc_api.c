@@ -117,7 +117,7 @@ int32_t tc_frame_encode(const topos_frame_config* cfg, const topos_frame_input* 
     return rc == TC_OK ? r.rc : rc;
 }
 
-static int32_t dec_prepare(const uint8_t* data, size_t size, topos_frame_output* info, tc_view* v)
+int32_t tc_dec_prepare(const uint8_t* data, size_t size, topos_frame_output* info, tc_view* v)
 {
     if (info == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "out_info == NULL"); }
     memset(info, 0, sizeof(*info));
@@ -130,7 +130,7 @@ static int32_t dec_prepare(const uint8_t* data, size_t size, topos_frame_output*
     return TC_OK;
 }
 
-static int32_t dec_supported(const tc_view* v)
+int32_t tc_dec_supported(const tc_view* v)
 {
     const tc_fhdr* fh = &v->fh;
     if (fh->major == 7u && fh->entropy_mode == 8u && fh->frame_type != 0u) {
@@ -150,8 +150,8 @@ static int32_t dec_supported(const tc_view* v)
 
 int32_t tc_dec_prepare_check(const uint8_t* data, size_t size, topos_frame_output* info, tc_view* v)
 {
-    int32_t rc = dec_prepare(data, size, info, v);
-    return rc == TC_OK ? dec_supported(v) : rc;
+    int32_t rc = tc_dec_prepare(data, size, info, v);
+    return rc == TC_OK ? tc_dec_supported(v) : rc;
 }
 
 int32_t tc_frame_decode(const uint8_t* data, size_t size, uint16_t* const planes_out[TC_FRAME_MAX_PLANES],
@@ -159,9 +159,9 @@ int32_t tc_frame_decode(const uint8_t* data, size_t size, uint16_t* const planes
 {
     tc_view* v = (tc_view*)malloc(sizeof(tc_view));
     if (v == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "packet view"); }
-    int32_t rc = dec_prepare(data, size, out_info, v);
+    int32_t rc = tc_dec_prepare(data, size, out_info, v);
     if (rc == TC_OK && planes_out != NULL) {
-        rc = dec_supported(v);
+        rc = tc_dec_supported(v);
         for (uint32_t p = 0; rc == TC_OK && p < v->fh.plane_count; ++p) {
             if (planes_out[p] == NULL) { rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "planes_out[%u] == NULL", p); }
             else if (strides && strides[p] != 0u && strides[p] < v->fh.pvw[p]) {
@@ -184,52 +184,11 @@ int32_t tc_frame_decode(const uint8_t* data, size_t size, uint16_t* const planes
     return rc;
 }
 
+/* reference codec.c:9792-9798 (the batch contract at topos_codec.h:374-400) */
 int32_t tc_frame_decode_batch(const topos_batch_packet* packets, uint32_t count, uint16_t* const* planes_out,
                               const size_t* strides, topos_frame_output* infos)
 {
-    if (count == 0u) { return TC_OK; }
-    if (packets == NULL || infos == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "batch args NULL"); }
-    tc_view* views = (tc_view*)malloc(sizeof(tc_view) * count);
-    tc_dec_req* reqs = (tc_dec_req*)calloc(count, sizeof(tc_dec_req));
-    if (views == NULL || reqs == NULL) { free(views); free(reqs); return tc_fail(TC_ERR_OUT_OF_MEMORY, "batch"); }
-    int32_t rc = TC_OK;
-    for (uint32_t i = 0; i < count && rc == TC_OK; ++i) {
-        rc = dec_prepare(packets[i].data, packets[i].size, &infos[i], &views[i]);
-    }
-    if (rc != TC_OK || planes_out == NULL) { free(views); free(reqs); return rc; }
-    uint32_t nrun = 0;
-    int32_t first = TC_OK;
-    tc_dec_req* run = (tc_dec_req*)calloc(count, sizeof(tc_dec_req));
-    uint32_t* map = (uint32_t*)calloc(count, sizeof(uint32_t));
-    for (uint32_t i = 0; i < count; ++i) {
-        tc_dec_req* r = &reqs[i];
-        r->data = packets[i].data; r->size = packets[i].size; r->view = &views[i]; r->info = &infos[i];
-        r->planes = planes_out + (size_t)i * TC_FRAME_MAX_PLANES;
-        r->strides = strides ? strides + (size_t)i * TC_FRAME_MAX_PLANES : NULL;
-        r->rc = dec_supported(&views[i]);
-        for (uint32_t p = 0; r->rc == TC_OK && p < views[i].fh.plane_count; ++p) {
-            if (r->planes[p] == NULL) { r->rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "planes_out[%u][%u] == NULL", i, p); }
-            else if (r->strides && r->strides[p] != 0u && r->strides[p] < views[i].fh.pvw[p]) {
-                r->rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "frame %u plane %u stride too small", i, p);
-            }
-        }
-        if (r->rc == TC_OK) { map[nrun] = i; run[nrun++] = *r; }
-    }
-    if (nrun > 0u) {
-        tc_ectx* ec = NULL;
-        if ((rc = tc_engine_acquire(&ec)) == TC_OK) {
-            rc = tc_engine_decode(ec, run, nrun);
-            tc_engine_release(ec);
-            for (uint32_t k = 0; k < nrun; ++k) { reqs[map[k]].rc = rc < TC_OK ? rc : run[k].rc; }
-        } else {
-            for (uint32_t k = 0; k < nrun; ++k) { reqs[map[k]].rc = rc; }
-        }
-    }
-    for (uint32_t i = 0; i < count; ++i) {
-        if (first == TC_OK && reqs[i].rc != TC_OK) { first = reqs[i].rc; }
-    }
-    free(run); free(map); free(views); free(reqs);
-    return first;
+    return tc_decode_batch_run(packets, count, (uint32_t)TC_DECODE_SCALE_FULL, NULL, planes_out, strides, infos);
 }
 
 int32_t tc_frame_plane_geometry(const topos_frame_output* info, uint32_t plane, uint32_t* width, uint32_t* height)
@@ -249,76 +208,4 @@ int32_t tc_frame_plane_geometry(const topos_frame_output* info, uint32_t plane, 
     *width = (plane == 1u || plane == 2u) ? cw : info->visible_width;
     *height = info->visible_height;
     return TC_OK;
-}
-
-/* ------------------------------------------------------------ contexts */
-
-struct tc_decoder { uint32_t workers; };
-
-int32_t tc_decoder_create(const topos_decoder_config* cfg, tc_decoder** out)
-{
-    if (out == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "out == NULL"); }
-    if (cfg != NULL && ((cfg->struct_size != 0u && cfg->struct_size != (uint32_t)sizeof(topos_decoder_config)) ||
-                        (cfg->abi_version != 0u && cfg->abi_version != (uint32_t)TOPOS_CODEC_ABI_VERSION))) {
-        return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder config struct_size/abi_version mismatch");
-    }
-    tc_decoder* d = (tc_decoder*)calloc(1, sizeof(tc_decoder));
-    if (d == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "decoder context"); }
-    d->workers = cfg ? cfg->max_slice_workers : 0u;
-    *out = d;
-    return TC_OK;
-}
-
-void tc_decoder_destroy(tc_decoder* dec) { free(dec); }
-
-int32_t tc_decoder_prepare(tc_decoder* dec, const uint8_t* packet, size_t size, topos_frame_output* info)
-{
-    if (dec == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder == NULL"); }
-    return tc_frame_decode(packet, size, NULL, NULL, info);
-}
-
-static int32_t views_to_planes(const topos_plane_view* out, uint32_t n, uint16_t** planes, size_t* strides)
-{
-    for (uint32_t p = 0; p < n; ++p) {
-        if ((out[p].struct_size != 0u && out[p].struct_size != (uint32_t)sizeof(topos_plane_view)) ||
-            (out[p].abi_version != 0u && out[p].abi_version != (uint32_t)TOPOS_CODEC_ABI_VERSION)) {
-            return tc_fail(TC_ERR_INVALID_ARGUMENT, "plane view %u struct_size/abi_version mismatch", p);
-        }
-        planes[p] = out[p].pixels;
-        strides[p] = out[p].stride;
-    }
-    return TC_OK;
-}
-
-int32_t tc_decoder_decode(tc_decoder* dec, const uint8_t* packet, size_t size,
-                          const topos_plane_view out[TC_FRAME_MAX_PLANES], topos_frame_output* info)
-{
-    if (dec == NULL || out == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder/out == NULL"); }
-    uint16_t* planes[TC_FRAME_MAX_PLANES] = { 0 };
-    size_t strides[TC_FRAME_MAX_PLANES] = { 0 };
-    topos_frame_output tmp;
-    int32_t rc = tc_frame_decode(packet, size, NULL, NULL, info ? info : &tmp);
-    if (rc != TC_OK) { return rc; }
-    const uint32_t np = (info ? info : &tmp)->plane_count;
-    if ((rc = views_to_planes(out, np, planes, strides)) != TC_OK) { return rc; }
-    return tc_frame_decode(packet, size, planes, strides, info ? info : &tmp);
-}
-
-int32_t tc_decoder_decode_batch(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
-                                const topos_plane_view* out, topos_frame_output* infos)
-{
-    if (dec == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder == NULL"); }
-    if (out == NULL) { return tc_frame_decode_batch(packets, count, NULL, NULL, infos); }
-    uint16_t** planes = (uint16_t**)calloc((size_t)count * TC_FRAME_MAX_PLANES + 1u, sizeof(uint16_t*));
-    size_t* strides = (size_t*)calloc((size_t)count * TC_FRAME_MAX_PLANES + 1u, sizeof(size_t));
-    if (planes == NULL || strides == NULL) { free(planes); free(strides); return tc_fail(TC_ERR_OUT_OF_MEMORY, "batch"); }
-    int32_t rc = TC_OK;
-    for (uint32_t i = 0; i < count && rc == TC_OK; ++i) {
-        rc = views_to_planes(out + (size_t)i * TC_FRAME_MAX_PLANES, TC_FRAME_MAX_PLANES,
-                             planes + (size_t)i * TC_FRAME_MAX_PLANES, strides + (size_t)i * TC_FRAME_MAX_PLANES);
-    }
-    if (rc == TC_OK) { rc = tc_frame_decode_batch(packets, count, planes, strides, infos); }
-    free(planes);
-    free(strides);
-    return rc;
 }

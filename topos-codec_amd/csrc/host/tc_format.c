@@ -14,6 +14,7 @@
  *   quantiser context   native/src/transform/quant.c:217-308
  */
 #include "tc_host.h"
+#include "topos_gpu.h"
 #include "../kernels/tc_math.h"
 
 #include <string.h>
@@ -729,3 +730,8 @@ void tc_fill_output_info(const tc_fhdr* fh, topos_frame_output* info)
     info->sar_den = fh->sar_den;
     info->slice_count = fh->slice_count;
 }
+
+/* exported views of the tables above (include/topos_gpu.h) */
+double tcx_ctx_log2(double x) { return tc_ctx_log2(x); }
+
+void tcx_crc_tables(uint32_t crc256[256], uint32_t x2n32[32]) { tc_crc_tables(crc256, x2n32); }

@@ -151,7 +151,6 @@ static int32_t dec_batch_prepare(dec_batch* b, uint32_t n, const uint8_t* const*
     if (host_packets == NULL || sizes == NULL || dev_packets == NULL || dev_planes == NULL || infos == NULL) {
         return tc_fail(TC_ERR_INVALID_ARGUMENT, "batch decode args NULL");
     }
-    extern int32_t tc_dec_prepare_check(const uint8_t*, size_t, topos_frame_output*, tc_view*);
     b->n = n;
     b->views = (tc_view*)malloc(sizeof(tc_view) * n);
     b->reqs = (tc_dec_req*)calloc(n, sizeof(tc_dec_req));

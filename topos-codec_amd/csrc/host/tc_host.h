@@ -85,6 +85,20 @@ void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bit_de
 uint32_t tc_deblock_alpha(uint32_t tbl, uint32_t qp);
 void tc_fill_output_info(const tc_fhdr* fh, topos_frame_output* info);
 
+/* decode-side checks (tc_api.c): tc_dec_prepare parses a packet's structure
+ * and stamps info (a failure rejects the packet); tc_dec_supported says
+ * whether the frame's bitstream generation decodes here; _check is both */
+int32_t tc_dec_prepare(const uint8_t* data, size_t size, topos_frame_output* info, tc_view* v);
+int32_t tc_dec_supported(const tc_view* v);
+int32_t tc_dec_prepare_check(const uint8_t* data, size_t size, topos_frame_output* info, tc_view* v);
+
+/* request rules of topos_decode_request and the batch decode core
+ * (tc_preview.c); scale is a tc_decode_scale used when request == NULL */
+int32_t tc_decode_request_validate(const topos_decode_request* q);
+int32_t tc_decode_batch_run(const topos_batch_packet* packets, uint32_t count, uint32_t scale,
+                            const topos_decode_request* request, uint16_t* const* planes_out,
+                            const size_t* strides, topos_frame_output* infos);
+
 /* encode argument checks of tc_frame_encode (tc_api.c): host planes or device planes */
 int32_t tc_enc_check_host(const topos_frame_config* cfg, const topos_frame_input* input);
 int32_t tc_enc_check_dev(const topos_frame_config* cfg, const uint64_t* dev_planes, const size_t* dev_strides);

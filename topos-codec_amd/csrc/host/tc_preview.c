@@ -16,8 +16,6 @@
 #define TC_BATCH_MAX_FRAMES 65536u
 #define AUTO_2K_MAX_DIM 2048u
 
-int32_t tc_dec_prepare_check(const uint8_t* data, size_t size, topos_frame_output* info, tc_view* v);
-
 /* reference codec.c:9174-9194 */
 int32_t tc_decode_scale_dimensions(uint32_t source_width, uint32_t source_height, tc_decode_scale scale,
                                    uint32_t* target_width, uint32_t* target_height)
@@ -45,7 +43,7 @@ typedef struct dec_plan {
 } dec_plan;
 
 /* reference codec.c:9208-9273 */
-static int32_t request_validate(const topos_decode_request* q)
+int32_t tc_decode_request_validate(const topos_decode_request* q)
 {
     if (q == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decode request == NULL"); }
     if (q->struct_size != (uint32_t)sizeof(topos_decode_request) ||
@@ -94,7 +92,7 @@ static int32_t request_validate(const topos_decode_request* q)
 /* reference codec.c:9277-9357 */
 static int32_t request_resolve(const tc_fhdr* fh, const topos_decode_request* q, dec_plan* plan)
 {
-    int32_t rc = request_validate(q);
+    int32_t rc = tc_decode_request_validate(q);
     if (rc != TC_OK) { return rc; }
     memset(plan, 0, sizeof(*plan));
     plan->tw = fh->visible_w;
@@ -245,7 +243,7 @@ int32_t tc_frame_decode_request(const uint8_t* data, size_t size, const topos_de
                                 const size_t strides[TC_FRAME_MAX_PLANES], topos_frame_output* out_info)
 {
     int32_t rc = begin(out_info, data);
-    if (rc == TC_OK) { rc = request_validate(request); }
+    if (rc == TC_OK) { rc = tc_decode_request_validate(request); }
     if (rc != TC_OK) { return rc; }
     tc_view* v = (tc_view*)malloc(sizeof(tc_view));
     if (v == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "packet view"); }
@@ -305,11 +303,18 @@ int32_t tc_frame_decode_reduced(const uint8_t* data, size_t size, tc_decode_scal
     return rc;
 }
 
-/* a batch through per-frame plans (codec.c:9640-9790); request NULL = the
- * fixed ratio `scale` for every frame */
-static int32_t batch_run(const topos_batch_packet* packets, uint32_t count, uint32_t scale,
-                         const topos_decode_request* request, uint16_t* const* planes_out, const size_t* strides,
-                         topos_frame_output* infos)
+/* A batch through per-frame plans (codec.c:9640-9790); request NULL = the
+ * fixed ratio `scale` for every frame (TC_DECODE_SCALE_FULL: the plain
+ * tc_frame_decode_batch). The contract of topos_codec.h:374-400: a packet
+ * whose structure does not parse rejects the whole batch before any frame is
+ * decoded; an inter frame rejects the whole batch (TC_ERR_STATE); a frame
+ * with a missing plane, a stride below its width or a bitstream generation
+ * this library does not decode is rejected alone; every other frame is
+ * decoded in the same launch sequence (concealed frames delivered) and the
+ * first non-OK status in frame order is returned. */
+int32_t tc_decode_batch_run(const topos_batch_packet* packets, uint32_t count, uint32_t scale,
+                            const topos_decode_request* request, uint16_t* const* planes_out, const size_t* strides,
+                            topos_frame_output* infos)
 {
     if (count == 0u) { return TC_OK; }
     if (count > TC_BATCH_MAX_FRAMES) {
@@ -321,7 +326,7 @@ static int32_t batch_run(const topos_batch_packet* packets, uint32_t count, uint
         return tc_fail(TC_ERR_INVALID_ARGUMENT, "invalid decode scale %u", scale);
     }
     if (request != NULL) {
-        int32_t rc = request_validate(request);
+        int32_t rc = tc_decode_request_validate(request);
         if (rc != TC_OK) { return rc; }
     }
     for (uint32_t i = 0; i < count; ++i) {
@@ -338,7 +343,7 @@ static int32_t batch_run(const topos_batch_packet* packets, uint32_t count, uint
     }
     /* structure of every packet first: one failure rejects the whole batch */
     for (uint32_t i = 0; i < count && rc == TC_OK; ++i) {
-        rc = tc_dec_prepare_check(packets[i].data, packets[i].size, &infos[i], &views[i]);
+        rc = tc_dec_prepare(packets[i].data, packets[i].size, &infos[i], &views[i]);
         if (rc == TC_OK) {
             dec_plan* pl = &plans[i];
             if (request != NULL) {
@@ -360,7 +365,8 @@ static int32_t batch_run(const topos_batch_packet* packets, uint32_t count, uint
             if (rc != TC_OK) { break; }
             uint16_t* const* pl = planes_out + (size_t)i * TC_FRAME_MAX_PLANES;
             const size_t* st = strides ? strides + (size_t)i * TC_FRAME_MAX_PLANES : NULL;
-            rcs[i] = check_outputs(&views[i], &plans[i], infos[i].plane_count, pl, st);
+            rcs[i] = tc_dec_supported(&views[i]);
+            if (rcs[i] == TC_OK) { rcs[i] = check_outputs(&views[i], &plans[i], infos[i].plane_count, pl, st); }
             if (rcs[i] == TC_OK) {
                 fill_req(&reqs[nrun], packets[i].data, packets[i].size, &views[i], &plans[i], pl, st, &infos[i]);
                 map[nrun++] = i;
@@ -390,69 +396,7 @@ static int32_t batch_run(const topos_batch_packet* packets, uint32_t count, uint
 int32_t tc_frame_decode_batch_reduced(const topos_batch_packet* packets, uint32_t count, tc_decode_scale scale,
                                       uint16_t* const* planes_out, const size_t* strides, topos_frame_output* infos)
 {
-    return batch_run(packets, count, (uint32_t)scale, NULL, planes_out, strides, infos);
-}
-
-/* ---- decoder-context request forms (topos_codec.h:529-559) ---- */
-
-static int32_t views_to_planes(const topos_plane_view* out, uint32_t n, uint16_t** planes, size_t* strides)
-{
-    for (uint32_t p = 0; p < n; ++p) {
-        if ((out[p].struct_size != 0u && out[p].struct_size != (uint32_t)sizeof(topos_plane_view)) ||
-            (out[p].abi_version != 0u && out[p].abi_version != (uint32_t)TOPOS_CODEC_ABI_VERSION)) {
-            return tc_fail(TC_ERR_INVALID_ARGUMENT, "plane view %u struct_size/abi_version mismatch", p);
-        }
-        planes[p] = out[p].pixels;
-        strides[p] = out[p].stride;
-    }
-    return TC_OK;
-}
-
-int32_t tc_decoder_prepare_request(tc_decoder* dec, const uint8_t* packet, size_t size,
-                                   const topos_decode_request* request, topos_frame_output* info)
-{
-    if (dec == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder == NULL"); }
-    return tc_frame_decode_request(packet, size, request, NULL, NULL, info);
-}
-
-int32_t tc_decoder_decode_request(tc_decoder* dec, const uint8_t* packet, size_t size,
-                                  const topos_decode_request* request, const topos_plane_view out[TC_FRAME_MAX_PLANES],
-                                  topos_frame_output* info)
-{
-    if (dec == NULL || out == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder/out == NULL"); }
-    topos_frame_output tmp;
-    topos_frame_output* inf = info ? info : &tmp;
-    int32_t rc = tc_frame_decode_request(packet, size, request, NULL, NULL, inf);
-    if (rc != TC_OK) { return rc; }
-    uint16_t* planes[TC_FRAME_MAX_PLANES] = { 0 };
-    size_t strides[TC_FRAME_MAX_PLANES] = { 0 };
-    if ((rc = views_to_planes(out, inf->plane_count, planes, strides)) != TC_OK) { return rc; }
-    return tc_frame_decode_request(packet, size, request, planes, strides, inf);
-}
-
-int32_t tc_decoder_decode_batch_request(tc_decoder* dec, const topos_batch_packet* packets, uint32_t count,
-                                        const topos_decode_request* request, const topos_plane_view* out,
-                                        topos_frame_output* infos)
-{
-    if (dec == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decoder == NULL"); }
-    if (request == NULL) { return tc_fail(TC_ERR_INVALID_ARGUMENT, "decode request == NULL"); }
-    if (out == NULL) { return batch_run(packets, count, 0u, request, NULL, NULL, infos); }
-    uint16_t** planes = (uint16_t**)calloc((size_t)count * TC_FRAME_MAX_PLANES + 1u, sizeof(uint16_t*));
-    size_t* strides = (size_t*)calloc((size_t)count * TC_FRAME_MAX_PLANES + 1u, sizeof(size_t));
-    if (planes == NULL || strides == NULL) {
-        free(planes);
-        free(strides);
-        return tc_fail(TC_ERR_OUT_OF_MEMORY, "batch");
-    }
-    int32_t rc = TC_OK;
-    for (uint32_t i = 0; i < count && rc == TC_OK; ++i) {
-        rc = views_to_planes(out + (size_t)i * TC_FRAME_MAX_PLANES, TC_FRAME_MAX_PLANES,
-                             planes + (size_t)i * TC_FRAME_MAX_PLANES, strides + (size_t)i * TC_FRAME_MAX_PLANES);
-    }
-    if (rc == TC_OK) { rc = batch_run(packets, count, 0u, request, planes, strides, infos); }
-    free(planes);
-    free(strides);
-    return rc;
+    return tc_decode_batch_run(packets, count, (uint32_t)scale, NULL, planes_out, strides, infos);
 }
 
 /* device-resident batch through one request (no reference counterpart) */
@@ -466,7 +410,7 @@ int32_t tcx_decode_batch_dev_request(uint32_t n, const uint8_t* const* host_pack
         rcs == NULL) {
         return tc_fail(TC_ERR_INVALID_ARGUMENT, "batch args NULL");
     }
-    int32_t rc = request_validate(request);
+    int32_t rc = tc_decode_request_validate(request);
     if (rc != TC_OK) { return rc; }
     tc_view* views = (tc_view*)malloc(sizeof(tc_view) * n);
     dec_plan* plans = (dec_plan*)calloc(n, sizeof(dec_plan));

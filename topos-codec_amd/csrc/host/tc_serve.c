@@ -490,7 +490,6 @@ int32_t tcx_server_submit(tcx_server* s, const tcx_request* req, uint64_t* id)
             rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "dev_out given with capacity 0");
         }
     } else if (req->kind == TCX_REQ_DECODE) {
-        extern int32_t tc_dec_prepare_check(const uint8_t*, size_t, topos_frame_output*, tc_view*);
         q->view = (tc_view*)malloc(sizeof(tc_view));
         if (q->view == NULL) {
             rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "packet view");
