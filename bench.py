@@ -66,61 +66,69 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(abi, frames, cfg, budget_s):
-    """CPU leg timed on this host's cores on a bounded sample of the same
-    workload (whole frames, encode + decode each, until budget_s has passed).
-    Uses the reference C library built from the reference sources
-    (oracle/_ref, kind "reference", its own slice thread pool) when present,
-    else the single-threaded C restatement in oracle/ (kind "port").
+    """CPU leg on this host's cores, on a bounded sample of the same workload:
+    the C restatement in oracle/ ("port", bit-exact with the reference by the
+    committed fixtures) encoding and decoding whole frames, frame-parallel on
+    one thread per host core, until budget_s has passed. The reference itself
+    is timed only where it was built (oracle/time_cpu.py writes
+    oracle/reference_timing.json there); its numbers and the port/reference
+    ratio measured on that machine are reported beside this one.
     Returns (record, packet of frame 0)."""
-    ref_path = os.path.join(ROOT, "oracle", "_ref", "libtopos_ref.so")
+    import concurrent.futures as cf
+    from helpers import orc_decode, orc_encode
+    orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
     dims = abi.plane_dims(W, H, 0, 0)
-    if os.path.exists(ref_path):
-        kind, threads = "reference", min(16, os.cpu_count() or 1)
-        os.environ["TOPOS_SLICE_THREADS"] = str(threads)
-        ref = ctypes.CDLL(ref_path)
-        ref.tc_frame_packet_bound.restype = ctypes.c_size_t
-        bound = ref.tc_frame_packet_bound(ctypes.byref(cfg))
-        out = ctypes.create_string_buffer(bound)
-        dec = [np.zeros((h, w), np.uint16) for (w, h) in dims]
-        dptr = (ctypes.c_void_p * 4)(*([d.ctypes.data for d in dec] + [None]))
+    threads = len(os.sched_getaffinity(0))
+    done = [0] * threads
+    pk0 = []
+    t_end = time.perf_counter() + budget_s
 
-        def one(planes):
-            inp = abi.FrameInput()
-            inp.struct_size, inp.abi_version = ctypes.sizeof(inp), 2
-            for i, p in enumerate(planes):
-                inp.planes[i] = p.ctypes.data
-            st = abi.FrameStats()
-            rc = ref.tc_frame_encode(ctypes.byref(cfg), ctypes.byref(inp), out, ctypes.c_size_t(bound),
-                                     ctypes.byref(st))
-            assert rc == 0
-            pkt = out.raw[: st.packet_size]
-            info = abi.FrameOutput()
-            assert ref.tc_frame_decode(pkt, ctypes.c_size_t(len(pkt)), dptr, None, ctypes.byref(info)) == 0
-            return pkt
-        what = f"reference C library built from the reference sources, {threads} slice threads"
-    else:
-        kind, threads = "port", 1
-        orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
-        from helpers import orc_decode, orc_encode
-
-        def one(planes):
-            rc, pkt = orc_encode(orc, cfg, planes)
+    def work(k):
+        i = k
+        while True:
+            rc, pkt = orc_encode(orc, cfg, frames[i % len(frames)])
             assert rc == 0
             assert orc_decode(orc, pkt, dims)[0] == 0
-            return pkt
-        what = "single-threaded C restatement (oracle/)"
-    done, pk0, t0 = 0, None, time.perf_counter()
-    while True:
-        pkt = one(frames[done % len(frames)])
-        if done == 0:
-            pk0 = pkt
-        done += 1
-        if time.perf_counter() - t0 >= budget_s and done >= 2:
-            break
+            if i == 0:
+                pk0.append(pkt)
+            done[k] += 1
+            i += threads
+            if time.perf_counter() >= t_end:
+                return
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(threads)))
     dt = time.perf_counter() - t0
-    return {"value": round(done / dt, 3), "unit": "frames/s enc+dec", "cores": threads, "kind": kind,
-            "sample": f"{done} frames of the 4K 422 HQ qp{QP} workload (encode+decode each), {what}"}, pk0
+    value = sum(done) / dt
+    rec = {"value": round(value, 3), "unit": "frames/s enc+dec", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": f"{sum(done)} frames of the 4K 422 HQ qp{QP} workload (encode+decode each) by the C "
+                     f"restatement in oracle/, one frame per thread on {threads} threads, {dt:.1f} s"}
+    tpath = os.path.join(ROOT, "oracle", "reference_timing.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        ref8 = max(t["reference"].values(), key=lambda v: v["frames_per_s_enc_dec"])
+        rec["reference_where_built"] = {
+            "cpu_model": t["cpu_model"], "cores": t["cores"],
+            "frames_per_s_enc_dec": t["reference_best_frames_per_s"],
+            "per_frame_encode_ms": ref8["encode_ms"], "per_frame_decode_ms": ref8["decode_ms"],
+            "port_frames_per_s_enc_dec": t["port"]["all_cores"]["frames_per_s_enc_dec"],
+            "port_over_reference": t["port_over_reference"],
+            "what": "compiled reference (-O3 -DNDEBUG, its slice thread pool) and the same port, timed on the "
+                    "build machine by oracle/time_cpu.py"}
+        rec["reference_estimate_here"] = round(value / t["port_over_reference"], 3)
+    return rec, pk0[0]
 
 
 def server_run(pkg, srv, enc_req, dec_req, nsteps, nf, sizes):
@@ -360,9 +368,8 @@ def sized_bench(pkg, abi, codec, frames, in_addr, cfg, target=SIZED_TARGET):
     """Target-size encode of 4K 422 HQ frames to target bytes (qp 0..95):
     tc_frame_encode_sized on one frame (host buffers, cold seed and seeded
     repeat) and tcx_encode_sized_batch_dev on the resident batch (one seed
-    state per frame, cold and seeded); the compiled reference's
-    tc_frame_encode_sized on the same frame when it is built (its own slice
-    threads). The chosen qp and packet agree with the reference's."""
+    state per frame, cold and seeded). The chosen qp and packet agree with
+    the oracle's target-size search on the same frame."""
     codec.sized_set_mode(0)
     res = {"workload": f"3840x2160 422 HQ frames to a {target}-byte target, qp 0..95, strict search"}
     times, pk0 = [], None
@@ -400,21 +407,12 @@ def sized_bench(pkg, abi, codec, frames, in_addr, cfg, target=SIZED_TARGET):
     res["batch"] = {"frames": n, "cold_ms": round(bt[0] * 1e3, 2), "seeded_ms": round(float(np.median(bt[1:])) * 1e3, 2),
                     "frames_per_s_seeded": round(n / float(np.median(bt[1:])), 2),
                     "encodes_per_frame": [int(r.encodes) for r in reqs]}
-    ref_path = os.path.join(ROOT, "oracle", "_ref", "libtopos_ref.so")
-    if os.path.exists(ref_path):
-        from helpers import lib_sized
-        threads = min(16, os.cpu_count() or 1)
-        os.environ["TOPOS_SLICE_THREADS"] = str(threads)
-        ref = ctypes.CDLL(ref_path)
-        ref.tc_frame_packet_bound.restype = ctypes.c_size_t
-        rt = []
-        for rep in range(2):
-            t0 = time.perf_counter()
-            r = lib_sized(ref, abi, cfg, frames[0], target, 0, 95)
-            rt.append(time.perf_counter() - t0)
-        res["cpu_reference_ms"] = {"first": round(rt[0] * 1e3, 1), "repeat": round(rt[1] * 1e3, 1),
-                                   "threads": threads}
-        res["equal_to_reference"] = bool(r[0] == 0 and r[1] == q0 and r[2] == pk0)
+    # the oracle's target-size search on the same frame picks the same qp and packet
+    orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
+    orc.orc_sized_reset()
+    from helpers import orc_sized
+    r = orc_sized(orc, cfg, frames[0], target, 0, 95)
+    res["equal_to_oracle"] = bool(r[0] == 0 and r[1] == q0 and r[2] == pk0)
     return res
 
 
@@ -604,6 +602,16 @@ def main():
     t_gen = time.perf_counter()
     frames = [pkg.synth_frame422(syn, W, H, seed) for seed in range(nf)]
     print(f"# generated {nf} frames in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    # the compiled reference's committed hashes of these frames (tests/golden/
+    # baseline_ref_hashes.json): the generator must reproduce the inputs here
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import baseline_cases as bc
+    fixture = bc.load_fixture()
+    ref_frames = {c[4]: fixture[bc.case_id(c)] for c in bc.cases() if c[0] == "hq_4k"}
+    bitexact = {"inputs_equal_reference_fixture": all(
+        seed in ref_frames and [bc.sha(p) for p in fr] == ref_frames[seed]["input_sha256"]
+        for seed, fr in enumerate(frames))}
+    assert bitexact["inputs_equal_reference_fixture"], "frozen generator differs on this host"
 
     # resident inputs, packet buffers and output planes in HBM
     dims = abi.plane_dims(W, H, 0, 0)
@@ -790,7 +798,7 @@ def main():
         now = [bufs[i].download(sizes[i]).tobytes() for i in all_idx]
         assert [hashlib.sha256(p).hexdigest() for p in now] == ref_hash
     dec0 = [b.download(w * h * 2, np.uint16).reshape(h, w) for b, (w, h) in zip(out_planes[0], dims)]
-    bitexact = {"packets_stable": True}
+    bitexact["packets_stable"] = True
 
     configs = None
     if not a.no_configs:
@@ -814,9 +822,11 @@ def main():
     # decode of packet 0 against the oracle's decode of the same packet
     orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liborc.so"))
     from helpers import orc_decode, orc_encode
-    if "packet0_equals_reference" not in bitexact:
-        rc, opk = orc_encode(orc, cfg, frames[0])
-        bitexact["packet0_equals_oracle"] = rc == 0 and opk == host_pkts[0]
+    rc, opk = orc_encode(orc, cfg, frames[0])
+    bitexact["packet0_equals_oracle"] = rc == 0 and opk == host_pkts[0]
+    bitexact["packets_equal_reference"] = all(bc.sha(host_pkts[i]) == ref_frames[i]["packet_sha256"]
+                                              for i in range(nf) if i in ref_frames)
+    bitexact["decode0_equals_reference"] = [bc.sha(x) for x in dec0] == ref_frames[0]["decode_sha256"]
     rc, oplanes, _, _ = orc_decode(orc, host_pkts[0], dims)
     bitexact["decode0_equals_oracle"] = rc == 0 and all((x == y).all() for x, y in zip(oplanes, dec0))
     bitexact["served_packets_equal_batched"] = True  # every warm-up packet hash checked above

@@ -30,8 +30,9 @@ static uint32_t bitlen(uint32_t m) { return m == 0u ? 0u : 32u - (uint32_t)__bui
 /* zlib CRC32, reflected poly 0xEDB88320, bytewise (native/src/common/crc32.c:7-10,105). */
 uint32_t orc_crc32(const uint8_t* p, size_t n)
 {
-    static uint32_t tab[256];
-    static int ready = 0;
+    /* per thread: the restatement is called from concurrent threads */
+    static _Thread_local uint32_t tab[256];
+    static _Thread_local int ready = 0;
     if (!ready) {
         for (uint32_t i = 0; i < 256u; ++i) {
             uint32_t c = i;
@@ -698,7 +699,7 @@ static int rans2_payload(const toks* t, bytes_t* out)
     const uint32_t lvl_n = lvl_model == 1 ? 4u : lvl_model == 2 ? 5u : 1u;
     const uint32_t dc_n = dc_model ? 5u : 1u;
 
-    static rmodel dcm[5], lvm[5], rnm;
+    static _Thread_local rmodel dcm[5], lvm[5], rnm;   /* per thread: too large for the stack frame's budget */
     bpush(out, (uint8_t)(lvl_model | (dc_model << 2)));
     const uint32_t* dc_rows = dc_model ? dc_prev : t->dc_hist;
     for (uint32_t c = 0; c < dc_n; ++c) {
@@ -1195,7 +1196,7 @@ static int decode_colour(const orc_hdr* h, const orc_slice* s, const uint8_t* pl
     const uint32_t dc_n = dc_model ? 5u : 1u;
     const size_t prefix = 1u + dc_n * 29u + 64u + lvl_n * 28u;
     if (n < prefix + 4u) { return ORC_ETRUNC; }
-    static rmodel dcm[5], lvm[5], rnm;
+    static _Thread_local rmodel dcm[5], lvm[5], rnm;
     size_t off = 1;
     for (uint32_t c = 0; c < dc_n; ++c, off += 29) { if (row_model(pl + off, 29, &dcm[c])) { return ORC_EMALFORMED; } }
     if (row_model(pl + off, 64, &rnm)) { return ORC_EMALFORMED; }

@@ -251,3 +251,57 @@ def lib_decode_request(lib, abi, packet, request):
     rc = lib.tc_frame_decode_request(packet, ctypes.c_size_t(len(packet)), ctypes.byref(request), ptrs, None,
                                      ctypes.byref(info))
     return rc, outs, info
+
+
+# ---------------------------------------------------------------- robustness corpus
+import os as _os
+
+_GOLDEN = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "golden")
+
+
+def golden_bitstream_records():
+    """The 56 records of golden_bitstream_v1.bin (reference
+    native/tests/conformance/golden_bitstream.c:209-230: 8 frame headers, 30
+    Rice streams, 12 block payloads, 6 complete packets) and the stored fold."""
+    data = open(_os.path.join(_GOLDEN, "golden_bitstream_v1.bin"), "rb").read()
+    assert data[:4] == b"TPB1" and int.from_bytes(data[4:8], "big") == 1
+    count = int.from_bytes(data[8:12], "big")
+    fold = int.from_bytes(data[16:24], "big")
+    off, recs = 24, []
+    for _ in range(count):
+        n = int.from_bytes(data[off:off + 4], "big")
+        recs.append(data[off + 4:off + 4 + n])
+        off += 4 + n
+    assert off == len(data)
+    return recs, fold
+
+
+def splitmix_fold(records):
+    """The record fold of golden_bitstream.c:31-49 (splitmix finaliser per byte)."""
+    M = (1 << 64) - 1
+
+    def mix(h, v):
+        h = (h + 0x9E3779B97F4A7C15) & M
+        z = v
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return h ^ (z ^ (z >> 31))
+
+    g = 0x5A5A5A5A5A5A5A5A
+    for r in records:
+        g = mix(g, len(r))
+        for b in r:
+            g = mix(g, b)
+    return g
+
+
+def corpus_packets():
+    """(name, bytes) of the reference's bitstream robustness corpus
+    (native/tests/fuzz/seed_corpus_bitstream: valid seeds, bit flips,
+    truncations, forged and over-long fields) and the six complete packets of
+    golden_bitstream_v1.bin."""
+    d = _os.path.join(_GOLDEN, "seed_corpus_bitstream")
+    out = [(f, open(_os.path.join(d, f), "rb").read()) for f in sorted(_os.listdir(d))]
+    recs, _ = golden_bitstream_records()
+    out += [(f"golden_bitstream_packet_{i}", r) for i, r in enumerate(recs[50:])]
+    return out

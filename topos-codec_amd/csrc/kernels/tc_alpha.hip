@@ -436,6 +436,17 @@ __global__ __launch_bounds__(64) void k_alpha_decode(const TcgConsts* __restrict
             if (__ballot(bad != 0u) != 0ull) { rc = -9; break; }
         }
     }
+    if (rc != 0) {
+        // concealment: the band's visible rows take the fill value (0xFFFF),
+        // over anything a failed reconstruction already stored
+        // (native/src/codec/codec.c:7975-7985)
+        uint16_t* dst = (uint16_t*)P.dst;
+        const uint32_t py0 = S.y0 * 8u;
+        const uint32_t nrows = py0 < P.vis_h ? min(rows, P.vis_h - py0) : 0u;
+        for (uint32_t y = 0; y < nrows; ++y) {
+            for (uint32_t x = lane; x < P.vis_w; x += 64u) { dst[(size_t)(py0 + y) * P.stride + x] = (uint16_t)P.fill; }
+        }
+    }
     if (lane == 0u) { status[blockIdx.x] = rc == 0 ? 0 : 1; }
 }
 
