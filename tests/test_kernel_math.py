@@ -242,3 +242,52 @@ def test_conversion_normalisation_exact(d):
         res = _round_f32(ci - q * fd)
         got = _round_f32(Fraction(float(res)) * fr + q)
         assert got == want[i], (d, i)
+
+
+def _idct8_f64(v):
+    """The kernels' f64 inverse butterfly (tc_math.h TC_IDCT8_BODY), in the same
+    operation order, on Python floats (IEEE double)."""
+    p, q = 13.0 * v[0] + 9.0 * v[4], 13.0 * v[0] - 9.0 * v[4]
+    r, t = 12.0 * v[2] + 5.0 * v[6], 5.0 * v[2] - 12.0 * v[6]
+    e0, e3, e1, e2 = p + r, p - r, q + t, q - t
+    o0 = 13.0 * v[1] + 11.0 * v[3] + 8.0 * v[5] + 1.0 * v[7]
+    o1 = 11.0 * v[1] - 4.0 * v[3] - 13.0 * v[5] - 7.0 * v[7]
+    o2 = 7.0 * v[1] - 13.0 * v[3] + 4.0 * v[5] + 11.0 * v[7]
+    o3 = 1.0 * v[1] - 8.0 * v[3] + 11.0 * v[5] - 13.0 * v[7]
+    return [e0 + o0, e1 + o1, e2 + o2, e3 + o3, e3 - o3, e2 - o2, e1 - o1, e0 - o0]
+
+
+def _ref_round(v):
+    # reference sign-split rounding of an exact integer (transform.c:69-117)
+    return (v + (1 << 31)) >> 32 if v >= 0 else -((-v + (1 << 31)) >> 32)
+
+
+def test_f64_inverse_transform_worst_case(oracle):
+    """Dequantised coefficients at the clamp (|G| = f_clamp W, f_clamp = 2^25
+    for bit depths <= 12) with the signs that maximise each output sample:
+    every f64 step of the kernel's dequant / IDCT / rounding stays an exact
+    integer below 2^53, so it equals the exact integer transform and the
+    reference rounding (the kernel's one-FMA rounding is checked too)."""
+    import math
+    M = np.ctypeslib.as_array((ctypes.c_int16 * 64).in_dll(oracle, "orc_M")).astype(np.int64).reshape(8, 8)
+    W = np.ctypeslib.as_array((ctypes.c_uint32 * 64).in_dll(oracle, "orc_W")).astype(np.int64).reshape(8, 8)
+    fc = 1 << 25
+    worst = 0.0
+    for y in range(8):
+        for x in range(8):
+            su = np.sign(M[:, y]).astype(np.int64)
+            sv = np.sign(M[:, x]).astype(np.int64)
+            G = (su[:, None] * sv[None, :]) * fc * W          # exact integers (< 2^39)
+            # pass 1 (rows u): B[u][x'] = sum_v G[u][v] M[v][x']
+            B = [_idct8_f64([float(g) for g in G[u]]) for u in range(8)]
+            cols = [_idct8_f64([B[u][xx] for u in range(8)]) for xx in range(8)]
+            exact = M.T @ G @ M                                   # X[y][x] = sum_uv M[u][y] G[u][v] M[v][x]
+            for yy in range(8):
+                for xx in range(8):
+                    v = cols[xx][yy]
+                    worst = max(worst, abs(v))
+                    assert v == float(exact[yy, xx]) and abs(v) < 2.0 ** 53
+                    r = int(math.floor(abs(v) * 2.0 ** -32 + 0.5))  # the kernel's FMA form (exact, unfused too)
+                    assert (r if v >= 0 else -r) == _ref_round(int(exact[yy, xx]))
+            assert all(abs(b) < 2.0 ** 53 for row in B for b in row)
+    assert worst > 2.0 ** 45    # the test reaches the range the argument is about

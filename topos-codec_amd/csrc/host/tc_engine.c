@@ -28,7 +28,7 @@ typedef struct enc_frame_plan {
 struct tc_ectx {
     pthread_mutex_t mu;
     tcg_ctx* g;
-    dbuf in, q, dcm, scratch, pkt, desc, res, out, tmp, src, tgt;
+    dbuf in, q, dcm, scratch, pkt, desc, res, out, src;
     uint8_t* hdesc;    /* pinned staging for descriptor uploads */
     size_t hdesc_cap;
     uint8_t* hres;     /* pinned staging for result downloads */
@@ -40,7 +40,9 @@ struct tc_ectx {
     uint32_t nslices;
 };
 
-#define TC_MAX_CTX 8u
+/* at most the hardware queues HIP gives a process on this machine class
+ * (GPU_MAX_HW_QUEUES = 4): more streams would share queues and serialise */
+#define TC_MAX_CTX 4u
 
 static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
 static int g_state = 0;       /* 0 uninitialised, 1 ready, -1 failed */
@@ -194,6 +196,7 @@ int32_t tc_engine_acquire(tc_ectx** out)
 }
 
 void tc_engine_release(tc_ectx* c) { pthread_mutex_unlock(&c->mu); }
+
 
 int32_t tc_engine_staging(tc_ectx* c, size_t in_bytes, size_t out_bytes, uint64_t* in, uint64_t* out)
 {
@@ -563,7 +566,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
 {
     c->out_dev = NULL;
     uint32_t nplanes = 0, nslices = 0, nblocks = 0, ntiles = 0, maxrows = 1, nscaled = 0;
-    size_t pkt_bytes = 0, out_bytes = 0, tmp_bytes = 0, src_bytes = 0, tgt_bytes = 0;
+    size_t pkt_bytes = 0, out_bytes = 0, src_bytes = 0;
     for (uint32_t f = 0; f < n; ++f) {
         const tc_dec_req* r = &reqs[f];
         const tc_fhdr* fh = &r->view->fh;
@@ -587,9 +590,6 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             if (dropped) { continue; }
             const size_t ost = dec_out_stride(r, p, ow);
             if (r->dev_planes == NULL) { out_bytes += al256((size_t)ow * oh * 2u); }
-            if (db && p < 3u) {
-                if (scaled) { tgt_bytes += al256(ost * oh * 2u); } else { tmp_bytes += al256(ost * oh * 2u); }
-            }
         }
     }
     const size_t o_q = 0;
@@ -610,9 +610,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     }
     if (pkt_bytes && (rc = dbuf_need(c, &c->pkt, pkt_bytes)) != TC_OK) { return rc; }
     if (out_bytes && (rc = dbuf_need(c, &c->out, out_bytes)) != TC_OK) { return rc; }
-    if (tmp_bytes && (rc = dbuf_need(c, &c->tmp, tmp_bytes)) != TC_OK) { return rc; }
     if (src_bytes && (rc = dbuf_need(c, &c->src, src_bytes)) != TC_OK) { return rc; }
-    if (tgt_bytes && (rc = dbuf_need(c, &c->tgt, tgt_bytes)) != TC_OK) { return rc; }
 
     memset(c->hdesc, 0, desc_bytes);
     tcg_qctx* hq = (tcg_qctx*)(c->hdesc + o_q);
@@ -621,7 +619,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     tcg_rowrec* hrs = (tcg_rowrec*)(c->hdesc + o_rs);
     tcg_dplane* ht = nscaled ? (tcg_dplane*)(c->hdesc + o_tg) : NULL;
     uint32_t ip = 0, is = 0, iq = 0, blk = 0, tile = 0, row = 0, max_px = 0, tdbt = 0;
-    size_t pkt_off = 0, out_off = 0, tmp_off = 0, src_off = 0, tgt_off = 0;
+    size_t pkt_off = 0, out_off = 0, src_off = 0;
     tcg_batch_info info;
     memset(&info, 0, sizeof(info));
     info.max_cols = 1u;
@@ -694,17 +692,11 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                 D->fill = 1u << (fh->bit_depth - 1u);
                 D->maxv = (1u << fh->bit_depth) - 1u;
                 tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
-                if (db && p < 3u && !scaled) {
-                    D->tmp = c->tmp.p + tmp_off;
-                    tmp_off += al256((size_t)ost * vh * 2u);
-                    if (vw % 8u == 0u && vh % 8u == 0u && vw >= 16u && vh >= 16u && fh->qp_base <= 95u &&
-                        fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
-                        D->deblock = 1u;
-                        D->db_unit = 8u;
-                        D->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
-                    } else {
-                        D->tmp = 0u;
-                    }
+                if (db && p < 3u && !scaled && vw % 8u == 0u && vh % 8u == 0u && vw >= 16u && vh >= 16u &&
+                    fh->qp_base <= 95u && fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
+                    D->deblock = 1u;
+                    D->db_unit = 8u;
+                    D->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
                 }
             }
             if (ht != NULL) {
@@ -717,19 +709,12 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                     T->maxv = D->maxv;
                     T->kind = D->kind;
                     max_px = ow * oh > max_px ? ow * oh : max_px;
-                    if (colour && db && p < 3u && half) {
-                        T->tmp = c->tgt.p + tgt_off;
-                        tgt_off += al256((size_t)ost * oh * 2u);
-                        if (ow % 4u == 0u && oh % 4u == 0u && ow >= 8u && oh >= 8u && fh->qp_base <= 95u &&
-                            fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
-                            T->deblock = 1u;
-                            T->db_unit = 4u;
-                            T->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
-                            const uint32_t t = (oh / 4u + 1u) * ((ow + 255u) / 256u);
-                            tdbt = t > tdbt ? t : tdbt;
-                        } else {
-                            T->tmp = 0u;
-                        }
+                    if (colour && db && p < 3u && half && ow % 4u == 0u && oh % 4u == 0u && ow >= 8u &&
+                        oh >= 8u && fh->qp_base <= 95u && fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
+                        T->deblock = 1u;
+                        T->db_unit = 4u;
+                        T->db_alpha = tc_deblock_alpha(tbl, fh->qp_base);
+                        tdbt = tcg_db_grid_max(tdbt, tcg_db_grid(ow, oh, 4u));
                     }
                 }
             }
@@ -786,13 +771,10 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         }
         ip += fh->plane_count;
     }
-    /* deblock grid extent: strips x 256-column tiles of the largest filtered plane */
+    /* deblock grid extent over the filtered planes */
     uint32_t dbt = 0;
     for (uint32_t i = 0; i < ip; ++i) {
-        if (hp[i].deblock) {
-            const uint32_t t = (hp[i].vis_h / 8u + 1u) * ((hp[i].vis_w + 255u) / 256u);
-            dbt = t > dbt ? t : dbt;
-        }
+        if (hp[i].deblock) { dbt = tcg_db_grid_max(dbt, tcg_db_grid(hp[i].vis_w, hp[i].vis_h, 8u)); }
     }
     (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
     free(wkeys);

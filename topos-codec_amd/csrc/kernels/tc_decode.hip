@@ -357,18 +357,53 @@ __device__ static inline void iq_load(const tcg_dplane& P, uint32_t brow, uint32
     }
 }
 
+// Exact rounding of the f64 transform output v (an integer below 2^53) by
+// the reference's sign-split rule r = sign(v) floor((|v| + 2^31) / 2^32): |v|
+// 2^-32 is exact (a power-of-two scale) and adding 0.5 is exact below 2^52,
+// so one FMA forms (|v| + 2^31) / 2^32 and the truncating conversion floors it.
+__device__ static inline int32_t idct_round(double v)
+{
+    const int32_t m = (int32_t)__builtin_fma(__builtin_fabs(v), 0x1p-32, 0.5);
+    return v < 0.0 ? -m : m;
+}
+
+__device__ static inline int32_t idct_round(int64_t v)
+{
+    return v >= 0 ? (int32_t)((v + ((int64_t)1 << 31)) >> 32) : -(int32_t)((-v + ((int64_t)1 << 31)) >> 32);
+}
+
+// LDS of one IDCT tile: the first pass's output (f64 or int64, 65-word pitch
+// per block), reused after the second pass as the tile's output samples
+// (8 rows of 32 blocks x 8 u16) for the row stores
+struct IqLds {
+    union {
+        uint64_t B[IQ_BLOCKS * 65];
+        uint16_t out[8][IQ_BLOCKS * 8 + 8];
+    };
+};
+
 template <bool WIDE>
 __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __restrict__ qc,
                                          uint32_t brow, uint32_t bx0, uint32_t nb, bool conceal,
-                                         const IqRow& pre, void* sBraw, uint32_t* sQ, const double* sW)
+                                         const IqRow& pre, IqLds& L, uint32_t* sQ, double* sQW, int32_t* sQlim,
+                                         const double* sW)
 {
     typedef typename std::conditional<WIDE, int64_t, double>::type acc_t;
-    acc_t* sB = (acc_t*)sBraw;
+    acc_t* sB = (acc_t*)L.B;
     const uint32_t t = threadIdx.x;
-    if (t < 64u) { sQ[t] = qc->Q[t]; }
     const int32_t mid = qc->mid, maxv = qc->maxv;
     const int64_t fclamp = qc->fclamp;
-    uint16_t* out = (uint16_t*)(P.deblock ? P.tmp : P.dst);
+    if (t < 64u) {
+        const uint32_t Q = qc->Q[t];
+        sQ[t] = Q;
+        if (!WIDE) {
+            // Q W < 2^40 is exact; |q| <= floor(fclamp / Q) keeps |q Q| within
+            // the clamp, so such a coefficient dequantises as q (Q W) exactly
+            sQW[t] = (double)Q * sW[t];
+            sQlim[t] = (int32_t)((uint64_t)fclamp / Q);
+        }
+    }
+    uint16_t* out = (uint16_t*)P.dst;   // deblocking, if any, runs in place afterwards
     const uint32_t y0 = brow * 8u;
     if (conceal) {
         // fill the visible part of the tile with the neutral value
@@ -382,29 +417,43 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
     __syncthreads();
     {
         const uint32_t j = t >> 3, u = t & 7u;
+        const int4 r0 = pre.r0, r1 = pre.r1;
+        int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
+        if (P.limit != 0u && P.limit < 63u) {
+            // reduced decode: scan positions above the limit are dropped
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                if ((uint32_t)kZigzagInv[u * 8 + v] > P.limit) { qv[v] = 0; }
+            }
+        }
         if (j < nb) {
-            const int4 r0 = pre.r0, r1 = pre.r1;
-            int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
-            if (P.limit != 0u && P.limit < 63u) {
-                // reduced decode: scan positions above the limit are dropped
+            bool big = false;
+            if (!WIDE) {
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
-                    if ((uint32_t)kZigzagInv[u * 8 + v] > P.limit) { qv[v] = 0; }
+                    const int32_t lim = sQlim[u * 8 + v];
+                    big |= qv[v] > lim || qv[v] < -lim;
                 }
             }
             acc_t G[8];
+            if (WIDE) {
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                if (WIDE) {
+                for (int v = 0; v < 8; ++v) {
                     int64_t d = (int64_t)qv[v] * (int64_t)sQ[u * 8 + v];
                     d = d > fclamp ? fclamp : (d < -fclamp ? -fclamp : d);
                     d = d > (1 << 30) ? (1 << 30) : (d < -(1 << 30) ? -(1 << 30) : d);
                     G[v] = (acc_t)d * (acc_t)kTW[u * 8 + v];
-                } else {
-                    // |q| < 2^27 and Q < 2^26, so q Q is an integer below 2^53 and
-                    // exact in f64; the clamp bounds it by 2^25 and the weight by
-                    // 2^14, so the product with the weight is exact as well
-                    const double fc = (double)fclamp;
+                }
+            } else if (__builtin_expect(__ballot(big) == 0ull, 1)) {
+#pragma unroll
+                for (int v = 0; v < 8; ++v) { G[v] = (acc_t)((double)qv[v] * sQW[u * 8 + v]); }
+            } else {
+                // |q| < 2^27 and Q < 2^26, so q Q is an integer below 2^53 and
+                // exact in f64; the clamp bounds it by 2^25 and the weight by
+                // 2^14, so the product with the weight is exact as well
+                const double fc = (double)fclamp;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
                     double d = (double)qv[v] * (double)sQ[u * 8 + v];
                     d = fmin(fmax(d, -fc), fc);
                     G[v] = (acc_t)(d * sW[u * 8 + v]);
@@ -417,32 +466,38 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
         }
     }
     __syncthreads();
+    int32_t col[8];
+    const uint32_t j = t >> 3, x = t & 7u;
+    if (j < nb) {
+        acc_t B[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { B[u] = sB[j * 65 + u * 8 + x]; }
+        acc_t Xv[8];
+        idct8(B, Xv);
+#pragma unroll
+        for (int y = 0; y < 8; ++y) {
+            const int32_t sv = idct_round(Xv[y]) + mid;
+            col[y] = sv < 0 ? 0 : (sv > maxv ? maxv : sv);
+        }
+    }
+    __syncthreads();   // every thread is done reading sB
+    if (j < nb) {
+#pragma unroll
+        for (int y = 0; y < 8; ++y) { L.out[y][j * 8 + x] = (uint16_t)col[y]; }
+    }
+    __syncthreads();
+    // thread t stores row y = t / 32 of block jj = t % 32: 16 contiguous bytes,
+    // a half-wave one 512-byte row span of the tile
     {
-        const uint32_t j = t >> 3, x = t & 7u;
-        if (j < nb) {
-            acc_t B[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) { B[u] = sB[j * 65 + u * 8 + x]; }
-            const uint32_t px = (bx0 + j) * 8u + x;
-            acc_t Xv[8];
-            idct8(B, Xv);
-#pragma unroll
-            for (int y = 0; y < 8; ++y) {
-                const acc_t acc = Xv[y];
-                int32_t r;
-                if (WIDE) {
-                    const int64_t v = (int64_t)acc;
-                    r = v >= 0 ? (int32_t)((v + ((int64_t)1 << 31)) >> 32)
-                               : -(int32_t)((-v + ((int64_t)1 << 31)) >> 32);
-                } else {
-                    const double v = (double)acc;
-                    r = v >= 0.0 ? (int32_t)floor((v + 2147483648.0) * (1.0 / 4294967296.0))
-                                 : -(int32_t)floor((-v + 2147483648.0) * (1.0 / 4294967296.0));
-                }
-                int32_t s = r + mid;
-                s = s < 0 ? 0 : (s > maxv ? maxv : s);
-                const uint32_t py = y0 + (uint32_t)y;
-                if (py < P.vis_h && px < P.vis_w) { out[(size_t)py * P.stride + px] = (uint16_t)s; }
+        const uint32_t y = t >> 5, jj = t & 31u;
+        const uint32_t py = y0 + y, px = (bx0 + jj) * 8u;
+        if (jj < nb && py < P.vis_h) {
+            uint16_t* dst = out + (size_t)py * P.stride + px;
+            const uint16_t* src = &L.out[y][jj * 8];
+            if (px + 8u <= P.vis_w && ((uintptr_t)dst & 15u) == 0u) {
+                *(uint4*)dst = *(const uint4*)src;
+            } else {
+                for (uint32_t k = 0; k < 8u && px + k < P.vis_w; ++k) { dst[k] = src[k]; }
             }
         }
     }
@@ -460,9 +515,10 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
                                                        const int32_t* __restrict__ status,
                                                        const int32_t* __restrict__ qbuf)
 {
-    __shared__ uint64_t sB[IQ_BLOCKS * 65];
+    __shared__ IqLds L;
     __shared__ uint32_t sQ[64];
-    __shared__ double sW[64];
+    __shared__ double sW[64], sQW[64];
+    __shared__ int32_t sQlim[64];
     const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (P.kind != 0u) { return; }
@@ -489,25 +545,31 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
         const tcg_rowrec rr = rowrec[(size_t)pi * max_rows + brow];
         const tcg_qctx* qc = &qctxs[rr.qctx];
         const bool conceal = status[rr.slice] != 0;
-        if (k > 0u) { __syncthreads(); }  // the previous tile is done with sQ and sB
+        if (k > 0u) { __syncthreads(); }  // the previous tile is done with sQ and the tile LDS
         if (qc->bit_depth > 12u) {
-            dequant_idct_tile<true>(P, qc, brow, bx0, nb, conceal, pre[k], sB, sQ, sW);
+            dequant_idct_tile<true>(P, qc, brow, bx0, nb, conceal, pre[k], L, sQ, sQW, sQlim, sW);
         } else {
-            dequant_idct_tile<false>(P, qc, brow, bx0, nb, conceal, pre[k], sB, sQ, sW);
+            dequant_idct_tile<false>(P, qc, brow, bx0, nb, conceal, pre[k], L, sQ, sQW, sQlim, sW);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// K7: output deblocking, vertical edges then horizontal edges, fused per
-// strip. Strip r covers rows [8r-2, 8r+6): every horizontal edge y0 = 8r
-// reads and writes rows y0-2..y0+1 only, so the strips are independent once
-// each has applied the vertical-edge filter to its own rows. A tile is one
-// strip x 256 columns plus a 2-column halo on each side; reads come from the
-// unfiltered plane and writes go to the output plane.
+// K7: output deblocking, in place. Vertical edges sit at x0 = g k (g = 8, or 4
+// on an exact 2:1 target), horizontal edges at y0 = g r, and an edge reads and
+// writes the two samples on each side of it. The plane therefore splits into
+// g x g tiles offset by g/2 in both directions, (x0 - g/2 .. x0 + g/2) x
+// (y0 - g/2 .. y0 + g/2), each holding exactly one vertical edge segment (g
+// rows) and one horizontal edge segment (g columns) and every sample those
+// segments read: filtering the vertical segment of a tile and then its
+// horizontal segment gives the result of the whole-plane vertical pass
+// followed by the whole-plane horizontal pass (deblock.c:163-212), and tiles
+// share no sample, so they run independently in place. One thread per tile:
+// its g rows of g samples arrive with 8-byte (g = 8) or 4-byte (g = 4) vector
+// loads when the plane's base and row pitch allow it; a wave covers 64
+// consecutive tiles of a tile row, so each of its row loads is one contiguous
+// span of 64 g samples.
 // ---------------------------------------------------------------------------
-#define DB_COLS 256
-
 __device__ static inline uint32_t db_absu(int32_t v) { return (uint32_t)(v < 0 ? -v : v); }
 __device__ static inline int32_t db_clampv(int32_t v, int32_t m) { return v < 0 ? 0 : (v > m ? m : v); }
 
@@ -518,6 +580,7 @@ __device__ static inline void db_edge(int32_t& p1, int32_t& p0, int32_t& q0, int
           db_absu(q1 - q0) < (uint32_t)beta)) {
         return;
     }
+    // arithmetic shifts of negative values, as the reference's gcc build
     int32_t d = (12 * (q0 - p0) + 4 * (p1 - q1) + 8) >> 4;
     const int32_t delta = q0 - p0;
     const int32_t half = delta >= 0 ? (delta >> 1) : -((-delta) >> 1);
@@ -530,74 +593,101 @@ __device__ static inline void db_edge(int32_t& p1, int32_t& p0, int32_t& q0, int
     p0 = np0; q0 = nq0; p1 = np1; q1 = nq1;
 }
 
-// grid: x = tile (strip x 256 columns) within the plane, y = plane of the batch
-__global__ __launch_bounds__(256) void k_deblock(const tcg_dplane* __restrict__ planes)
+#define DB_TX TCG_DB_TX   // tiles per tile row of a workgroup (one wave)
+#define DB_TY TCG_DB_TY   // tile rows per workgroup
+
+// grid: x = group of DB_TX tiles along a tile row, y = group of DB_TY tile
+// rows, z = plane of the batch
+template <uint32_t G>
+__global__ __launch_bounds__(DB_TX * DB_TY) void k_deblock(const tcg_dplane* __restrict__ planes)
 {
-    __shared__ int32_t sP[8][DB_COLS + 4];
-    const uint32_t pi = blockIdx.y;
-    const tcg_dplane P = planes[pi];
-    if (!P.deblock) { return; }
-    // grid unit g (8 at full size, 4 on an exact 2:1 target): strip r holds rows
-    // [g r - 2, g r + g - 2), the rows of horizontal edge g r and the vertical
-    // edges of those rows; strips partition the plane
-    const uint32_t g = P.db_unit ? P.db_unit : 8u;
-    const uint32_t w = P.vis_w, h = P.vis_h;
-    const uint32_t col_tiles = (w + DB_COLS - 1u) / DB_COLS;
-    const uint32_t local = blockIdx.x;
-    if (local >= (h / g + 1u) * col_tiles) { return; }
-    const uint32_t r = local / col_tiles;
-    const uint32_t c0 = (local % col_tiles) * DB_COLS;
-    const int32_t ys = (int32_t)(g * r) - 2 < 0 ? 0 : (int32_t)(g * r) - 2;
-    const int32_t ye = min((int32_t)h, (int32_t)(g * r + g) - 2);
-    const int32_t nrows = ye - ys;
-    const int32_t xs = (int32_t)c0 - 2;
-    const uint16_t* src = (const uint16_t*)P.tmp;
-    uint16_t* dst = (uint16_t*)P.dst;
+    const tcg_dplane P = planes[blockIdx.z];
+    if (!P.deblock || (P.db_unit ? P.db_unit : 8u) != G) { return; }
+    const int32_t w = (int32_t)P.vis_w, h = (int32_t)P.vis_h;
+    const int32_t k = (int32_t)(blockIdx.x * DB_TX + (threadIdx.x % DB_TX));
+    const int32_t r = (int32_t)(blockIdx.y * DB_TY + (threadIdx.x / DB_TX));
+    constexpr int32_t HG = (int32_t)G / 2;
+    if (k > w / (int32_t)G || r > h / (int32_t)G) { return; }
+    const int32_t x0 = k * (int32_t)G - HG, y0 = r * (int32_t)G - HG;  // tile origin
+    uint16_t* base = (uint16_t*)P.dst;
+    const size_t stride = P.stride;
+    // vector access needs the tile's half-rows (HG samples) aligned: the
+    // plane's base and pitch at HG * 2 bytes (w and h are multiples of g)
+    const bool vec = ((((uintptr_t)base) | (stride * 2u)) & (uintptr_t)(HG * 2 - 1)) == 0u;
+    int32_t v[G][G];
+#pragma unroll
+    for (int32_t i = 0; i < (int32_t)G; ++i) {
+        const int32_t y = y0 + i;
+        const bool row_in = y >= 0 && y < h;
+        const uint16_t* row = base + (size_t)(row_in ? y : 0) * stride;
+#pragma unroll
+        for (int32_t half = 0; half < 2; ++half) {
+            const int32_t xs = x0 + half * HG;
+            const bool in = row_in && xs >= 0 && xs < w;   // a half-row is all in or all out
+            if (G == 8u) {
+                uint2 d = make_uint2(0u, 0u);
+                if (in) {
+                    if (vec) { d = *(const uint2*)(row + xs); }
+                    else { d = make_uint2(row[xs] | (uint32_t)row[xs + 1] << 16, row[xs + 2] | (uint32_t)row[xs + 3] << 16); }
+                }
+                v[i][half * 4 + 0] = (int32_t)(d.x & 0xFFFFu); v[i][half * 4 + 1] = (int32_t)(d.x >> 16);
+                v[i][half * 4 + 2] = (int32_t)(d.y & 0xFFFFu); v[i][half * 4 + 3] = (int32_t)(d.y >> 16);
+            } else {
+                uint32_t d = 0u;
+                if (in) { d = vec ? *(const uint32_t*)(row + xs) : (row[xs] | (uint32_t)row[xs + 1] << 16); }
+                v[i][half * 2 + 0] = (int32_t)(d & 0xFFFFu); v[i][half * 2 + 1] = (int32_t)(d >> 16);
+            }
+        }
+    }
     const int32_t alpha = (int32_t)P.db_alpha, beta = alpha / 2 + 2, clip = alpha >> 1;
     const int32_t maxv = (int32_t)P.maxv;
-    const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < 8u * (DB_COLS + 4u); i += 256u) {
-        const int32_t rr = (int32_t)(i / (DB_COLS + 4u)), cc = (int32_t)(i % (DB_COLS + 4u));
-        const int32_t y = ys + rr, x = xs + cc;
-        sP[rr][cc] = (rr < nrows && x >= 0 && x < (int32_t)w) ? (int32_t)src[(size_t)y * P.stride + x] : 0;
-    }
-    __syncthreads();
-    // vertical edges x0 = c0, c0+g, ..., c0+DB_COLS (x0 >= g, x0 + 1 < w), all rows
-    const uint32_t ne = DB_COLS / g + 1u;
-    for (uint32_t i = t; i < 8u * ne; i += 256u) {
-        const int32_t rr = (int32_t)(i / ne);
-        const int32_t x0 = (int32_t)c0 + (int32_t)g * (int32_t)(i % ne);
-        if (rr < nrows && x0 >= (int32_t)g && x0 + 1 < (int32_t)w) {
-            const int32_t l = x0 - xs;
-            int32_t p1 = sP[rr][l - 2], p0 = sP[rr][l - 1], q0 = sP[rr][l], q1 = sP[rr][l + 1];
-            db_edge(p1, p0, q0, q1, alpha, beta, clip, maxv);
-            sP[rr][l - 2] = p1; sP[rr][l - 1] = p0; sP[rr][l] = q0; sP[rr][l + 1] = q1;
+    // vertical edge x = g k (g <= x, x + 1 < w), every row of the tile
+    if (k >= 1 && (k + 1) * (int32_t)G <= w) {
+#pragma unroll
+        for (int32_t i = 0; i < (int32_t)G; ++i) {
+            if (y0 + i >= 0 && y0 + i < h) {
+                db_edge(v[i][HG - 2], v[i][HG - 1], v[i][HG], v[i][HG + 1], alpha, beta, clip, maxv);
+            }
         }
     }
-    __syncthreads();
-    // horizontal edge y0 = g r for 1 <= r <= h/g - 1
-    if (r >= 1u && r + 1u <= h / g) {
-        const int32_t x = (int32_t)c0 + (int32_t)t;
-        if (x < (int32_t)w) {
-            const int32_t l = x - xs;
-            // local rows of y0-2 .. y0+1 are 0..3 (ys = y0 - 2)
-            int32_t p1 = sP[0][l], p0 = sP[1][l], q0 = sP[2][l], q1 = sP[3][l];
-            db_edge(p1, p0, q0, q1, alpha, beta, clip, maxv);
-            sP[0][l] = p1; sP[1][l] = p0; sP[2][l] = q0; sP[3][l] = q1;
+    // horizontal edge y = g r (1 <= r <= h/g - 1), every column of the tile
+    if (r >= 1 && (r + 1) * (int32_t)G <= h) {
+#pragma unroll
+        for (int32_t j = 0; j < (int32_t)G; ++j) {
+            if (x0 + j >= 0 && x0 + j < w) {
+                db_edge(v[HG - 2][j], v[HG - 1][j], v[HG][j], v[HG + 1][j], alpha, beta, clip, maxv);
+            }
         }
     }
-    __syncthreads();
-    for (uint32_t i = t; i < 8u * DB_COLS; i += 256u) {
-        const int32_t rr = (int32_t)(i / DB_COLS), x = (int32_t)c0 + (int32_t)(i % DB_COLS);
-        if (rr < nrows && x < (int32_t)w) {
-            dst[(size_t)(ys + rr) * P.stride + x] = (uint16_t)sP[rr][x - xs];
+#pragma unroll
+    for (int32_t i = 0; i < (int32_t)G; ++i) {
+        const int32_t y = y0 + i;
+        if (y < 0 || y >= h) { continue; }
+        uint16_t* row = base + (size_t)y * stride;
+#pragma unroll
+        for (int32_t half = 0; half < 2; ++half) {
+            const int32_t xs = x0 + half * HG;
+            if (xs < 0 || xs >= w) { continue; }
+            if (G == 8u) {
+                const uint2 d = make_uint2((uint32_t)v[i][half * 4 + 0] | (uint32_t)v[i][half * 4 + 1] << 16,
+                                           (uint32_t)v[i][half * 4 + 2] | (uint32_t)v[i][half * 4 + 3] << 16);
+                if (vec) { *(uint2*)(row + xs) = d; }
+                else {
+                    row[xs] = (uint16_t)d.x; row[xs + 1] = (uint16_t)(d.x >> 16);
+                    row[xs + 2] = (uint16_t)d.y; row[xs + 3] = (uint16_t)(d.y >> 16);
+                }
+            } else {
+                const uint32_t d = (uint32_t)v[i][half * 2 + 0] | (uint32_t)v[i][half * 2 + 1] << 16;
+                if (vec) { *(uint32_t*)(row + xs) = d; }
+                else { row[xs] = (uint16_t)d; row[xs + 1] = (uint16_t)(d >> 16); }
+            }
         }
     }
 }
 
 // Target-size decode: target sample (tx, ty) of a plane takes the source
 // sample (floor(tx vw / dw), floor(ty vh / dh)) of the reconstructed
-// source-size plane; the output is the target's tmp when it deblocks next.
+// source-size plane (deblocked in place afterwards when the target filters).
 __global__ __launch_bounds__(256) void k_sample(const tcg_dplane* __restrict__ srcs,
                                                 const tcg_dplane* __restrict__ dsts)
 {
@@ -606,7 +696,7 @@ __global__ __launch_bounds__(256) void k_sample(const tcg_dplane* __restrict__ s
     if (!D.sample) { return; }
     const tcg_dplane S = srcs[pi];
     const uint16_t* src = (const uint16_t*)S.dst;
-    uint16_t* out = (uint16_t*)(D.deblock ? D.tmp : D.dst);
+    uint16_t* out = (uint16_t*)D.dst;
     const uint64_t n = (uint64_t)D.vis_w * D.vis_h;
     for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u) {
         const uint32_t ty = (uint32_t)(i / D.vis_w), tx = (uint32_t)(i - (uint64_t)ty * D.vis_w);
@@ -659,7 +749,8 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         }
         tcg_mark(c, 2);
         if (max_db_tiles > 0u) {
-            hipLaunchKernelGGL(k_deblock, dim3(max_db_tiles, nplanes), dim3(256), 0, st, planes_dev);
+            hipLaunchKernelGGL(k_deblock<8>, dim3(max_db_tiles >> 16, max_db_tiles & 0xFFFFu, nplanes),
+                               dim3(DB_TX * DB_TY), 0, st, planes_dev);
         }
         tcg_mark(c, 3);
     };
@@ -690,7 +781,8 @@ extern "C" int32_t tcg_decode_sample(tcg_ctx* c, const tcg_dplane* src_dev, cons
     const uint32_t gx = min((max_px + 255u) / 256u, 8192u);
     hipLaunchKernelGGL(k_sample, dim3(gx, nplanes), dim3(256), 0, c->stream, src_dev, dst_dev);
     if (max_db_tiles > 0u) {
-        hipLaunchKernelGGL(k_deblock, dim3(max_db_tiles, nplanes), dim3(256), 0, c->stream, dst_dev);
+        hipLaunchKernelGGL(k_deblock<4>, dim3(max_db_tiles >> 16, max_db_tiles & 0xFFFFu, nplanes),
+                           dim3(DB_TX * DB_TY), 0, c->stream, dst_dev);
     }
     return tcg_check("target-size sample launch");
 }

@@ -134,8 +134,7 @@ typedef struct tcg_dslice {
 } tcg_dslice;
 
 typedef struct tcg_dplane {
-    uint64_t dst;        /* device u16 output, visible area */
-    uint64_t tmp;        /* unfiltered plane (same layout) when deblock != 0 */
+    uint64_t dst;        /* device u16 output, visible area (deblocked in place) */
     uint32_t stride;     /* elements */
     uint32_t vis_w, vis_h;
     uint32_t bcols, brows;
@@ -252,13 +251,31 @@ typedef struct tcg_batch_info {
 uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols);
 
 
+/* Deblocking runs one thread per g x g tile centred on a block corner, in
+ * workgroups of TCG_DB_TX tiles along a tile row x TCG_DB_TY tile rows. The
+ * grid extent for a w x h plane, packed as (x groups << 16 | y groups); a
+ * batch launches the component-wise maximum over its planes. */
+#define TCG_DB_TX 64u
+#define TCG_DB_TY 4u
+static inline uint32_t tcg_db_grid(uint32_t w, uint32_t h, uint32_t g)
+{
+    const uint32_t gx = (w / g + 1u + TCG_DB_TX - 1u) / TCG_DB_TX, gy = (h / g + 1u + TCG_DB_TY - 1u) / TCG_DB_TY;
+    return gx << 16 | gy;
+}
+static inline uint32_t tcg_db_grid_max(uint32_t a, uint32_t b)
+{
+    const uint32_t x = (a >> 16) > (b >> 16) ? (a >> 16) : (b >> 16);
+    const uint32_t y = (a & 0xFFFFu) > (b & 0xFFFFu) ? (a & 0xFFFFu) : (b & 0xFFFFu);
+    return x << 16 | y;
+}
+
 /* ---- conversions (queued on the context's stream) ---- */
 int32_t tcg_convert_rgb(tcg_ctx* c, const tcg_cvt* k);
 /* target-size decode, after tcg_decode on the same stream: every target
  * descriptor with sample != 0 takes, per target sample (tx, ty), the source
- * plane's sample (floor(tx * vw / dw), floor(ty * vh / dh)) (into tmp when the
- * target deblocks), then the target deblock pass; max_px = largest target
- * plane in samples, max_db_tiles = deblock grid extent of the targets */
+ * plane's sample (floor(tx * vw / dw), floor(ty * vh / dh)), then the target
+ * deblock pass in place; max_px = largest target plane in samples,
+ * max_db_tiles = deblock grid extent of the targets (tcg_db_grid, g = 4) */
 int32_t tcg_decode_sample(tcg_ctx* c, const tcg_dplane* src_dev, const tcg_dplane* dst_dev, uint32_t nplanes,
                           uint32_t max_px, uint32_t max_db_tiles);
 int32_t tcg_convert_444_422(tcg_ctx* c, const tcg_p422* k);
@@ -271,8 +288,9 @@ int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const
                    tcg_eframe_out* frame_out_dev, const tcg_batch_info* info);
 
 /* ---- decode ---- */
-/* rowrec_dev: nplanes x max_rows records; max_plane_tiles / max_db_tiles:
- * largest transform / deblock tile count of one plane (grid x extents) */
+/* rowrec_dev: nplanes x max_rows records; max_plane_tiles: largest transform
+ * tile count of one plane; max_db_tiles: deblock grid extent over the
+ * filtered planes (tcg_db_grid, g = 8) */
 int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t nplanes, const tcg_dplane* planes_dev,
                    uint32_t nslices, const tcg_dslice* slices_dev, const tcg_rowrec* rowrec_dev,
                    uint32_t max_rows, uint64_t qbuf, uint32_t max_plane_tiles, int32_t* slice_status_dev,
