@@ -49,6 +49,21 @@ int32_t tcx_decode_batch_dev(uint32_t n, const uint8_t* const* host_packets, con
                              const uint64_t* dev_packets, const uint64_t* dev_planes,
                              const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code running under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same decode as tcx_decode_batch_dev,
+ * with every intermediate buffer (coefficients, launch descriptors, slice
+ * status) inside one fixed 66 MiB device array of the library and all host
+ * staging in fixed arrays: the call itself allocates nothing (the first call
+ * in the process creates the stream it runs on). Limits per call: 16 frames,
+ * 262,144 8x8 blocks over all planes (one 3840x2160 4:2:2 frame), 4096
+ * slices; packets and planes resident in HBM. Beyond a limit the call returns
+ * TC_ERR_LIMIT_EXCEEDED before any device work. Calls are serialised inside
+ * the library, and the fixed array keeps the last call's contents. */
+int32_t tcx_decode_batch_dev_noalloc(uint32_t n, const uint8_t* const* host_packets, const size_t* sizes,
+                                     const uint64_t* dev_packets, const uint64_t* dev_planes,
+                                     const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs);
+
 /* MOV samples [start, start + count) into HBM and decoded as one batch (no
  * reference counterpart; it joins tc_movie_packet_batch, reference
  * topos_codec.h:1000-1003, with tcx_decode_batch_dev): the packets are read

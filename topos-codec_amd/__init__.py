@@ -54,6 +54,7 @@ def load_library(path=LIB_PATH):
     lib.tcx_d2h.argtypes = [vp, ctypes.c_uint64, ctypes.c_size_t]
     lib.tcx_encode_batch_dev.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
     lib.tcx_decode_batch_dev.argtypes = [ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
+    lib.tcx_decode_batch_dev_noalloc.argtypes = [ctypes.c_uint32, vp, vp, vp, vp, vp, vp, vp]
     lib.tcx_copy.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_size_t]
     lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
@@ -814,6 +815,13 @@ class DeviceCodec(ToposCodec):
         a = self._dec_args(host_packets, dev_packets, plane_addrs, strides)
         st = int(self._lib.tcx_decode_batch_dev(a["n"], a["hp"], a["sizes"], a["dp"], a["planes"], a["strides"],
                                                 a["infos"], a["rcs"]))
+        return st, list(a["rcs"]), a["infos"]
+
+    def decode_batch_dev_noalloc(self, host_packets, dev_packets, plane_addrs, strides=None):
+        """tcx_decode_batch_dev_noalloc: the same decode, with no allocation in the call."""
+        a = self._dec_args(host_packets, dev_packets, plane_addrs, strides)
+        st = int(self._lib.tcx_decode_batch_dev_noalloc(a["n"], a["hp"], a["sizes"], a["dp"], a["planes"],
+                                                        a["strides"], a["infos"], a["rcs"]))
         return st, list(a["rcs"]), a["infos"]
 
     def autotune_batch(self, cfgs, plane_addrs, out_bufs, reps=2, strides=None):

@@ -38,6 +38,12 @@ struct tc_ectx {
     size_t o_res, o_fo, o_fr;
     uint64_t* out_dev;
     uint32_t nslices;
+    /* no-allocation context: buffers are fixed views of the static device
+     * scratch and static host arrays; a need beyond them fails instead of
+     * allocating */
+    int noalloc;
+    void* na_keys;          /* work-list keys (TC_NA_MAX_SLICES) */
+    uint64_t* na_out_dev;   /* (TC_NA_MAX_FRAMES x 4 + 1) output plane addresses */
 };
 
 /* at most the hardware queues HIP gives a process on this machine class
@@ -58,6 +64,10 @@ static size_t al256(size_t v) { return (v + 255u) & ~(size_t)255u; }
 static int32_t dbuf_need(tc_ectx* c, dbuf* b, size_t bytes)
 {
     if (bytes <= b->cap) { return TC_OK; }
+    if (c->noalloc) {
+        return tc_fail(TC_ERR_LIMIT_EXCEEDED, "no-allocation decode: %zu bytes needed, the fixed buffer holds %zu",
+                       bytes, b->cap);
+    }
     if (b->p) { (void)tcg_sync(c->g); (void)tcg_free(b->p); b->p = 0; b->cap = 0; }
     size_t cap = bytes + bytes / 4u + 4096u;
     if (tcg_malloc(&b->p, cap) != 0) {
@@ -71,6 +81,10 @@ static int32_t dbuf_need(tc_ectx* c, dbuf* b, size_t bytes)
 static int32_t host_need(tc_ectx* c, uint8_t** p, size_t* cap, size_t bytes)
 {
     if (bytes <= *cap) { return TC_OK; }
+    if (c->noalloc) {
+        return tc_fail(TC_ERR_LIMIT_EXCEEDED, "no-allocation decode: %zu host bytes needed, the fixed buffer holds %zu",
+                       bytes, *cap);
+    }
     if (*p) { (void)tcg_sync(c->g); (void)tcg_host_free(*p); *p = NULL; *cap = 0; }
     size_t n = bytes + bytes / 4u + 4096u;
     if (tcg_host_alloc((void**)p, n) != 0) {
@@ -79,6 +93,13 @@ static int32_t host_need(tc_ectx* c, uint8_t** p, size_t* cap, size_t bytes)
     }
     *cap = n;
     return TC_OK;
+}
+
+/* host arrays of a decode plan: heap blocks, or the no-allocation context's
+ * fixed arrays, which are never freed */
+static void plan_free(const tc_ectx* c, void* p)
+{
+    if (!c->noalloc) { free(p); }
 }
 
 /* work list of the persistent rANS2 launch: slice indices by descending
@@ -197,6 +218,57 @@ int32_t tc_engine_acquire(tc_ectx** out)
 
 void tc_engine_release(tc_ectx* c) { pthread_mutex_unlock(&c->mu); }
 
+/* The no-allocation context: one per process, created on its first use (its
+ * stream and events), launching eagerly (no graph instantiation), with the
+ * device buffers q / desc / res fixed inside the static scratch
+ * (tcg_noalloc_scratch) and the host staging in static arrays. Calls through
+ * it are serialised by its lock; every other buffer has capacity 0, so a
+ * decode that would need one (host packets or planes, target-size or
+ * alpha-dropping decodes) fails with TC_ERR_LIMIT_EXCEEDED. */
+static tc_ectx g_na_ctx;
+static int g_na_ready = 0;
+static uint8_t g_na_hdesc[TCG_NOALLOC_DESC_BYTES];
+static uint8_t g_na_hres[TCG_NOALLOC_RES_BYTES];
+static work_key g_na_keys[TC_NA_MAX_SLICES];
+static uint64_t g_na_out_dev[TC_NA_MAX_FRAMES * 4u + 1u];
+
+int32_t tc_engine_acquire_noalloc(tc_ectx** out)
+{
+    pthread_mutex_lock(&g_mu);
+    int32_t rc = engine_init_locked();
+    if (rc == TC_OK && !g_na_ready) {
+        tc_ectx* c = &g_na_ctx;
+        memset(c, 0, sizeof(*c));
+        uint64_t base = 0u;
+        if (tcg_noalloc_scratch(&base) != 0) {
+            rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "no-allocation scratch: %s", tcg_last_error());
+        } else if (tcg_ctx_create(&c->g) != 0) {
+            rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "stream creation failed: %s", tcg_last_error());
+        } else {
+            tcg_ctx_set_eager(c->g, 1);
+            c->noalloc = 1;
+            c->q.p = base;
+            c->q.cap = TCG_NOALLOC_Q_BYTES;
+            c->desc.p = base + TCG_NOALLOC_Q_BYTES;
+            c->desc.cap = TCG_NOALLOC_DESC_BYTES;
+            c->res.p = base + TCG_NOALLOC_Q_BYTES + TCG_NOALLOC_DESC_BYTES;
+            c->res.cap = TCG_NOALLOC_RES_BYTES;
+            c->hdesc = g_na_hdesc;
+            c->hdesc_cap = sizeof(g_na_hdesc);
+            c->hres = g_na_hres;
+            c->hres_cap = sizeof(g_na_hres);
+            c->na_keys = g_na_keys;
+            c->na_out_dev = g_na_out_dev;
+            pthread_mutex_init(&c->mu, NULL);
+            g_na_ready = 1;
+        }
+    }
+    pthread_mutex_unlock(&g_mu);
+    if (rc != TC_OK) { return rc; }
+    pthread_mutex_lock(&g_na_ctx.mu);
+    *out = &g_na_ctx;
+    return TC_OK;
+}
 
 int32_t tc_engine_staging(tc_ectx* c, size_t in_bytes, size_t out_bytes, uint64_t* in, uint64_t* out)
 {
@@ -623,10 +695,22 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     tcg_batch_info info;
     memset(&info, 0, sizeof(info));
     info.max_cols = 1u;
-    work_key* wkeys = (work_key*)malloc(sizeof(work_key) * (nslices ? nslices : 1u));
-    if (wkeys == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "work list"); }
-    uint64_t* out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
-    if (out_dev == NULL) { free(wkeys); return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode plan"); }
+    work_key* wkeys;
+    uint64_t* out_dev;
+    if (c->noalloc) {
+        if (nslices > TC_NA_MAX_SLICES || n > TC_NA_MAX_FRAMES) {
+            return tc_fail(TC_ERR_LIMIT_EXCEEDED, "no-allocation decode: %u frames / %u slices exceed %u / %u", n,
+                           nslices, TC_NA_MAX_FRAMES, TC_NA_MAX_SLICES);
+        }
+        wkeys = (work_key*)c->na_keys;
+        out_dev = c->na_out_dev;
+        memset(out_dev, 0, ((size_t)n * 4u + 1u) * sizeof(uint64_t));
+    } else {
+        wkeys = (work_key*)malloc(sizeof(work_key) * (nslices ? nslices : 1u));
+        if (wkeys == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "work list"); }
+        out_dev = (uint64_t*)calloc((size_t)n * 4u + 1u, sizeof(uint64_t));
+        if (out_dev == NULL) { free(wkeys); return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode plan"); }
+    }
     for (uint32_t f = 0; f < n; ++f) {
         tc_dec_req* r = &reqs[f];
         const tc_view* v = r->view;
@@ -636,8 +720,8 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         if (pk == 0u) {
             pk = c->pkt.p + pkt_off;
             if (tcg_h2d(c->g, pk, r->data, r->size) != 0) {
-                free(wkeys);
-                free(out_dev);
+                plan_free(c, wkeys);
+                plan_free(c, out_dev);
                 return tc_fail(TC_ERR_OUT_OF_MEMORY, "packet upload: %s", tcg_last_error());
             }
             pkt_off += al256(r->size + 1024u);
@@ -777,11 +861,11 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         if (hp[i].deblock) { dbt = tcg_db_grid_max(dbt, tcg_db_grid(hp[i].vis_w, hp[i].vis_h, 8u)); }
     }
     (void)work_list(wkeys, info.ent_count, (tcg_work*)(c->hdesc + o_wk));
-    free(wkeys);
+    plan_free(c, wkeys);
     info.work = (tcg_work*)(uintptr_t)(c->desc.p + o_wk);
     entropy_shape(&info, 0, n, tile);
     if (tcg_h2d(c->g, c->desc.p, c->hdesc, desc_bytes) != 0) {
-        free(out_dev);
+        plan_free(c, out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "descriptor upload: %s", tcg_last_error());
     }
     if (tcg_decode(c->g, (const tcg_qctx*)(uintptr_t)(c->desc.p + o_q), nplanes,
@@ -789,17 +873,17 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                    (const tcg_dslice*)(uintptr_t)(c->desc.p + o_sl),
                    (const tcg_rowrec*)(uintptr_t)(c->desc.p + o_rs), maxrows, c->q.p, ntiles,
                    (int32_t*)(uintptr_t)c->res.p, dbt, &info) != 0) {
-        free(out_dev);
+        plan_free(c, out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode launch failed: %s", tcg_last_error());
     }
     if (ht != NULL && tcg_decode_sample(c->g, (const tcg_dplane*)(uintptr_t)(c->desc.p + o_pl),
                                         (const tcg_dplane*)(uintptr_t)(c->desc.p + o_tg), nplanes, max_px,
                                         tdbt) != 0) {
-        free(out_dev);
+        plan_free(c, out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "target-size sample launch failed: %s", tcg_last_error());
     }
     if (tcg_d2h(c->g, c->hres, c->res.p, (size_t)nslices * sizeof(int32_t)) != 0) {
-        free(out_dev);
+        plan_free(c, out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "status download: %s", tcg_last_error());
     }
     c->out_dev = out_dev;
@@ -813,7 +897,7 @@ int32_t tc_engine_decode_finish(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
     c->out_dev = NULL;
     if (out_dev == NULL) { return tc_fail(TC_ERR_STATE, "no decode in flight"); }
     if (tcg_sync(c->g) != 0) {
-        free(out_dev);
+        plan_free(c, out_dev);
         return tc_fail(TC_ERR_OUT_OF_MEMORY, "decode: %s", tcg_last_error());
     }
     record_timing(c, 0);
@@ -850,7 +934,7 @@ int32_t tc_engine_decode_finish(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
         }
         if (first == TC_OK && r->rc != TC_OK) { first = r->rc; }
     }
-    free(out_dev);
+    plan_free(c, out_dev);
     return first;
 }
 

@@ -1,4 +1,5 @@
 /* Device-resident batch entry points (include/topos_gpu.h). */
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -288,4 +289,85 @@ void tcx_graph_stats(uint64_t* replays, uint64_t* captures)
     tc_engine_graph_stats(&h, &b);
     if (replays) { *replays = h; }
     if (captures) { *captures = b; }
+}
+
+/* The no-allocation form of tcx_decode_batch_dev: the same launches, with
+ * every intermediate (coefficients, descriptors, slice status) in the
+ * library's static device scratch and every host array static, so a call
+ * makes no device or host allocation (the first call in the process creates
+ * its stream). Limits: TC_NA_MAX_FRAMES frames, 262,144 8x8 blocks (one
+ * 3840x2160 4:2:2 frame) and TC_NA_MAX_SLICES slices per call, packets and
+ * planes resident in HBM, full-resolution output; beyond them the call fails
+ * with TC_ERR_LIMIT_EXCEEDED before any work. Calls are serialised by one lock:
+ * the scratch holds the last call's contents. */
+static pthread_mutex_t g_na_mu = PTHREAD_MUTEX_INITIALIZER;
+static tc_view g_na_views[TC_NA_MAX_FRAMES];
+static tc_dec_req g_na_reqs[TC_NA_MAX_FRAMES], g_na_run[TC_NA_MAX_FRAMES];
+static uint32_t g_na_map[TC_NA_MAX_FRAMES];
+
+int32_t tcx_decode_batch_dev_noalloc(uint32_t n, const uint8_t* const* host_packets, const size_t* sizes,
+                                     const uint64_t* dev_packets, const uint64_t* dev_planes,
+                                     const size_t* dev_strides, topos_frame_output* infos, int32_t* rcs)
+{
+    if (n == 0u) { return TC_OK; }
+    if (host_packets == NULL || sizes == NULL || dev_packets == NULL || dev_planes == NULL || infos == NULL) {
+        return tc_fail(TC_ERR_INVALID_ARGUMENT, "batch decode args NULL");
+    }
+    if (n > TC_NA_MAX_FRAMES) {
+        return tc_fail(TC_ERR_LIMIT_EXCEEDED, "no-allocation decode: %u frames > %u", n, TC_NA_MAX_FRAMES);
+    }
+    pthread_mutex_lock(&g_na_mu);
+    uint64_t blocks = 0u;
+    uint32_t nrun = 0u, nslices = 0u;
+    int32_t rc = TC_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        tc_dec_req* r = &g_na_reqs[i];
+        memset(r, 0, sizeof(*r));
+        r->rc = tc_dec_prepare_check(host_packets[i], sizes[i], &infos[i], &g_na_views[i]);
+        r->data = host_packets[i];
+        r->size = sizes[i];
+        r->dev_data = dev_packets[i];
+        r->dev_planes = dev_planes + (size_t)i * 4u;
+        r->dev_strides = dev_strides ? dev_strides + (size_t)i * 4u : NULL;
+        r->info = &infos[i];
+        r->view = &g_na_views[i];
+        if (r->rc == TC_OK && r->dev_data == 0u) {
+            r->rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "no-allocation decode: packet %u is not in device memory", i);
+        }
+        for (uint32_t p = 0; r->rc == TC_OK && p < g_na_views[i].fh.plane_count; ++p) {
+            if (r->dev_planes[p] == 0u) {
+                r->rc = tc_fail(TC_ERR_INVALID_ARGUMENT, "no-allocation decode: frame %u plane %u is NULL", i, p);
+            }
+        }
+        if (r->rc == TC_OK) {
+            const tc_fhdr* fh = &g_na_views[i].fh;
+            for (uint32_t p = 0; p < fh->plane_count; ++p) { blocks += (uint64_t)fh->bcols[p] * fh->brows[p]; }
+            nslices += g_na_views[i].nslices;
+            g_na_map[nrun] = i;
+            g_na_run[nrun++] = *r;
+        }
+    }
+    if (blocks * 256u > TCG_NOALLOC_Q_BYTES || nslices > TC_NA_MAX_SLICES) {
+        rc = tc_fail(TC_ERR_LIMIT_EXCEEDED,
+                     "no-allocation decode: %llu blocks / %u slices exceed %zu / %u", (unsigned long long)blocks,
+                     nslices, TCG_NOALLOC_Q_BYTES / 256u, TC_NA_MAX_SLICES);
+        pthread_mutex_unlock(&g_na_mu);
+        return rc;
+    }
+    if (nrun > 0u) {
+        tc_ectx* c = NULL;
+        rc = tc_engine_acquire_noalloc(&c);
+        if (rc == TC_OK) {
+            rc = tc_engine_decode(c, g_na_run, nrun);
+            tc_engine_release(c);
+        }
+    }
+    for (uint32_t k = 0; k < nrun; ++k) { g_na_reqs[g_na_map[k]].rc = rc < TC_OK ? rc : g_na_run[k].rc; }
+    int32_t first = TC_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (rcs) { rcs[i] = g_na_reqs[i].rc; }
+        if (first == TC_OK && g_na_reqs[i].rc != TC_OK) { first = g_na_reqs[i].rc; }
+    }
+    pthread_mutex_unlock(&g_na_mu);
+    return first;
 }

@@ -110,6 +110,13 @@ int32_t tc_engine_acquire(tc_ectx** out);
 /* never blocks: 1 (and *out = NULL) when every context is in use */
 int32_t tc_engine_try_acquire(tc_ectx** out);
 void tc_engine_release(tc_ectx* c);
+/* the no-allocation context (locked until release): fixed device buffers in
+ * the static scratch, fixed host staging, eager launches; plans beyond
+ * TC_NA_MAX_FRAMES frames / TC_NA_MAX_SLICES slices or its buffers fail with
+ * TC_ERR_LIMIT_EXCEEDED */
+#define TC_NA_MAX_FRAMES 16u
+#define TC_NA_MAX_SLICES 4096u
+int32_t tc_engine_acquire_noalloc(tc_ectx** out);
 /* the context's input / output staging buffers, grown to at least the sizes
  * given (conversions of host buffers) */
 int32_t tc_engine_staging(tc_ectx* c, size_t in_bytes, size_t out_bytes, uint64_t* in, uint64_t* out);

@@ -93,6 +93,7 @@ struct tcg_ctx {
     uint64_t graph_hits, graph_builds;
     void* pin;      // pinned host staging for strided host copies (grow-only)
     size_t pin_cap;
+    int eager;      // 1: never capture graphs on this context (no graph allocation)
 };
 const TcgConsts* tcg_consts_dev();
 int32_t tcg_check(const char* what);
@@ -118,7 +119,7 @@ int32_t tcg_graph_store(tcg_ctx* c, const uint64_t* key, hipGraph_t g, hipGraphE
 template <typename F>
 int32_t tcg_run(tcg_ctx* c, const uint64_t* key, F&& launch)
 {
-    if (!tcg_use_graphs()) {
+    if (!tcg_use_graphs() || c->eager) {
         launch();
         return tcg_check("launch");
     }

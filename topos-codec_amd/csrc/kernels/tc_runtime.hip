@@ -3,6 +3,7 @@
 // stream each, with timing events), device memory, copies and per-kernel
 // timing with HIP events on the launch stream.
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 
 #include <stddef.h>
 #include <stdio.h>
@@ -38,6 +39,41 @@ int32_t tcg_check(const char* what)
 }
 
 int tcg_use_graphs() { return g_graphs && !g_timing; }
+
+// Device memory of the no-allocation decode entry point: coefficients,
+// descriptors and slice status of its launches live here instead of in
+// buffers the library allocates. Every GPU the library runs on holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives per device (the array's name is a host-side
+// stand-in, not a device address).
+static __device__ __align__(256) uint8_t scratch[TCG_NOALLOC_BYTES];
+static uint64_t g_scratch_ptr[TCG_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int32_t scratch_setup_current_device(uint64_t* out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { return fail("hipGetDevice", e); }
+    if (dev < 0 || dev >= TCG_MAX_DEVICES) {
+        snprintf(g_err, sizeof(g_err), "device %d beyond the %d the no-allocation scratch covers", dev,
+                 TCG_MAX_DEVICES);
+        return -8;
+    }
+    pthread_mutex_lock(&g_scratch_mu);
+    if (g_scratch_ptr[dev] == 0u) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            pthread_mutex_unlock(&g_scratch_mu);
+            return fail("hipGetSymbolAddress(scratch)", e);
+        }
+        g_scratch_ptr[dev] = (uint64_t)(uintptr_t)p;
+    }
+    *out = g_scratch_ptr[dev];
+    pthread_mutex_unlock(&g_scratch_mu);
+    return 0;
+}
 
 uint32_t tcg_cu_count() { return g_cus; }
 
@@ -115,9 +151,18 @@ int32_t tcg_init(void)
     }
     e = hipMalloc((void**)&g_consts, sizeof(TcgConsts));
     if (e != hipSuccess) { return fail("hipMalloc consts", e); }
+    uint64_t sp = 0u;
+    if (scratch_setup_current_device(&sp) != 0) { return -2; }
     g_ready = 1;
     return 0;
 }
+
+int32_t tcg_noalloc_scratch(uint64_t* ptr)
+{
+    return scratch_setup_current_device(ptr);
+}
+
+void tcg_ctx_set_eager(tcg_ctx* c, int eager) { c->eager = eager; }
 
 const char* tcg_last_error(void) { return g_err; }
 

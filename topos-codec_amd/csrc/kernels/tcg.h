@@ -190,6 +190,19 @@ typedef struct tcg_ctx tcg_ctx;
 int32_t tcg_init(void);
 const char* tcg_last_error(void);
 int32_t tcg_ctx_create(tcg_ctx** out);
+/* 1: this context never captures launch graphs (graph instantiation allocates) */
+void tcg_ctx_set_eager(tcg_ctx* c, int eager);
+
+/* The no-allocation decode's device memory: a file-scope __device__ array of
+ * TCG_NOALLOC_BYTES (64 MiB of coefficients = 262,144 8x8 blocks, then
+ * descriptors and slice status). Its address on the current device, looked up
+ * once per device (at tcg_init for the first device); -2 / -8 on failure. */
+#define TCG_NOALLOC_Q_BYTES ((size_t)64 << 20)
+#define TCG_NOALLOC_DESC_BYTES ((size_t)3 << 19)
+#define TCG_NOALLOC_RES_BYTES ((size_t)1 << 18)
+#define TCG_NOALLOC_BYTES (TCG_NOALLOC_Q_BYTES + TCG_NOALLOC_DESC_BYTES + TCG_NOALLOC_RES_BYTES)
+#define TCG_MAX_DEVICES 16
+int32_t tcg_noalloc_scratch(uint64_t* ptr);
 void tcg_ctx_destroy(tcg_ctx* c);
 int32_t tcg_malloc(uint64_t* dptr, size_t bytes);
 int32_t tcg_free(uint64_t dptr);
