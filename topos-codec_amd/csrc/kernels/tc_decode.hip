@@ -95,10 +95,9 @@ __device__ static inline void rr_init(RansRd& D, uint32_t x0, const uint8_t* pay
 
 typedef uint32_t tc_u32x2 __attribute__((ext_vector_type(2)));
 
-__device__ static inline void rr_norm(RansRd& D)
+// renormalisation by n8 = 8k bits: k bytes move from the reservoir into the state
+__device__ static inline void rr_shift(RansRd& D, uint32_t n8)
 {
-    // x > 0 here; k bytes with k = ceil((23 - floor(log2 x)) / 8) for x < 2^23
-    const uint32_t n8 = ((uint32_t)__builtin_clz(D.x) - 1u) & 0x18u;
     const tc_u32x2 pair = { (uint32_t)(D.R >> 32), D.x };  // (state : next 32 payload bits)
     const uint64_t t = __builtin_bit_cast(uint64_t, pair);
     D.x = (uint32_t)((t << n8) >> 32);
@@ -106,14 +105,39 @@ __device__ static inline void rr_norm(RansRd& D)
     D.rc -= n8;
 }
 
-// decodes one symbol of the row and returns its entry
-__device__ static inline uint32_t rr_sym(RansRd& D, uint32_t cumv, uint32_t entv)
+// bytes to take for a state x > 0: k = ceil((23 - floor(log2 x)) / 8) below
+// 2^23, as a bit count
+__device__ static inline uint32_t norm_bits(uint32_t x) { return ((uint32_t)__builtin_clz(x) - 1u) & 0x18u; }
+
+__device__ static inline void rr_norm(RansRd& D) { rr_shift(D, norm_bits(D.x)); }
+
+// Row vectors of one context model, symbol s in lane 63 - s:
+//   cum  cumulative frequency (0xFFFF for an unused symbol: never <= slot)
+//   f    frequency
+//   ent  what the decoder needs of the symbol: for a run row run + 1 (the
+//        scan-position step; 64 = end of block); for a DC or level row the
+//        magnitude category c as (nz << w) | w << 24 | pb << 29 with
+//        w = max(c - 1, 0) raw suffix bits, nz = (c != 0) (the implicit top
+//        bit, stored only when w <= 23) and pb the previous-category context
+//        bucket the symbol selects for the next one
+struct RowV { uint32_t cum, f, ent; };
+
+// Decodes one symbol: every lane forms its symbol's successor state
+// f (x >> 12) + slot - cum in one 24-bit multiply-add, and that state's
+// renormalisation bit count, while the wave compares the slot against the
+// cumulative frequencies; the lowest matching lane is the symbol, and
+// readlanes fetch its state, bit count and entry (the scalar unit does only the
+// slot, the search bit scan and the reservoir shift).
+__device__ static inline uint32_t dsym(RansRd& D, const RowV& r)
 {
     const uint32_t slot = D.x & 4095u;
-    const uint32_t L = (uint32_t)__builtin_ctzll(__ballot(cumv <= slot));
-    const uint32_t e = __builtin_amdgcn_readlane(entv, L);
-    D.x = (e >> 19) * (D.x >> 12) + slot - (e & 0xFFFu);
-    rr_norm(D);
+    const uint32_t cand = __umul24(r.f, D.x >> 12) + (slot - r.cum);
+    const uint32_t cand_n8 = norm_bits(cand);   // every successor is >= 2^11 > 0
+    const uint32_t L = (uint32_t)__builtin_ctzll(__ballot(r.cum <= slot));
+    D.x = __builtin_amdgcn_readlane(cand, L);
+    const uint32_t n8 = __builtin_amdgcn_readlane(cand_n8, L);
+    const uint32_t e = __builtin_amdgcn_readlane(r.ent, L);
+    rr_shift(D, n8);
     return e;
 }
 
@@ -125,54 +149,67 @@ __device__ static inline uint32_t rr_bits(RansRd& D, uint32_t w)
     return c;
 }
 
-// Row entries: f << 19 | field << 12 | cum. For run rows the field is the
-// run (6 bits); for DC and level rows it is w | nz << 5 with w = max(cat-1, 0)
-// raw suffix bits and nz = (cat != 0), so the mapped magnitude is
-// nz << w | (w raw bits) and cat = w + nz.
-__device__ static inline uint32_t ent_w(uint32_t e) { return (e >> 12) & 31u; }
-__device__ static inline uint32_t ent_nz(uint32_t e) { return (e >> 17) & 1u; }
-
-// mapped magnitude of a DC or level symbol; reservoir holds >= 24 bits
-__device__ static inline uint32_t rr_mag(RansRd& D, uint32_t e, uint32_t lane)
+// mapped magnitude of a DC or level entry: its implicit top bit and w raw
+// suffix bits (more than 23: the top w - 16 bits first, then the low 16); the
+// reservoir holds >= 24 bits
+__device__ static inline uint32_t dmag(RansRd& D, uint32_t e, uint32_t lane)
 {
-    const uint32_t w = ent_w(e);
-    if (w > 23u) {
+    const uint32_t w = (e >> 24) & 31u;
+    if (__builtin_expect(w > 23u, 0)) {
         const uint32_t lo = rr_bits(D, 16u);
         rr_need(D, 16u, lane);
         const uint32_t hi = rr_bits(D, w - 16u);
         return (1u << w) | (hi << 16) | lo;
     }
-    const uint32_t v = rr_bits(D, w);
-    return (ent_nz(e) << w) | v;
+    return (e & 0xFFFFFFu) | rr_bits(D, w);
 }
 
-// previous-category context bucket of a DC or level entry
-__device__ static inline uint32_t ctx_prev_of(uint32_t e) { return tc_ctx_prev_wnz(ent_w(e), ent_nz(e)); }
+__device__ static inline uint32_t ent_bucket(uint32_t e) { return e >> 29; }
+
+// the AC pairs of one block, level rows fixed for scan positions below LIM;
+// val holds mapped magnitudes by scan position (lane = position)
+template <uint32_t LIM>
+__device__ static inline void ac_phase(RansRd& D, const RowV& lvl, const RowV& run, uint32_t& idx, uint32_t& e,
+                                       uint32_t& val, uint32_t lane)
+{
+    while (idx < LIM) {
+        const uint32_t el = dsym(D, lvl);
+        rr_need(D, 24u, lane);
+        const uint32_t m = dmag(D, el, lane);
+        val = lane == idx ? m : val;
+        rr_need(D, 32u, lane);
+        e = dsym(D, run);
+        idx += e & 127u;
+    }
+}
+
+// row r of the per-lane row arrays (r is wave-uniform)
+#define ROW(r) RowV{ cumv[r], fv[r], entv[r] }
 
 template <uint32_t LM, uint32_t DM>
-__device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, const uint32_t* entv,
-                                            const tcg_dslice& S, uint32_t cols, int32_t* __restrict__ qbuf,
-                                            int32_t* sDcRow, uint32_t lane)
+__device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11], const uint32_t (&fv)[11],
+                                            const uint32_t (&entv)[11], const tcg_dslice& S, uint32_t cols,
+                                            int32_t* __restrict__ qbuf, int32_t* sDcRow, uint32_t lane)
 {
     constexpr uint32_t DC_N = DM ? 5u : 1u;
     constexpr uint32_t RUN_ROW = DC_N;
     constexpr uint32_t LVL0 = DC_N + 1u;
     const uint32_t dc_max = S.dc_max, lv_max = S.lv_max;
     const int64_t dlim = dc_max > 0x08000000u ? ((int64_t)1 << 27) : ((int64_t)1 << 25);
-    const uint32_t myscan = tc_zigzag_pos(lane);
-    const uint32_t run_cum = cumv[RUN_ROW], run_ent = entv[RUN_ROW];
-    int32_t* out = qbuf + (size_t)S.blk0 * 64u + lane;
+    const RowV run = ROW(RUN_ROW);
+    // lane = scan position; its coefficient's natural index in the block
+    int32_t* out = qbuf + (size_t)S.blk0 * 64u + kZigzag[lane];
     uint32_t pdc_e = 0u;
     int32_t left = 0;
     for (uint32_t by = 0; by < S.h; ++by) {
         for (uint32_t bx = 0; bx < cols; ++bx) {
             const int32_t top_v = sDcRow[bx];
             uint32_t drow = 0u;
-            if (DM) { drow = (by == 0u && bx == 0u) ? 0u : 1u + ctx_prev_of(pdc_e); }
+            if (DM) { drow = (by == 0u && bx == 0u) ? 0u : 1u + ent_bucket(pdc_e); }
             rr_need(D, 32u, lane);
-            const uint32_t de = rr_sym(D, cumv[drow], entv[drow]);
-            rr_need(D, 32u, lane);
-            const uint32_t dm = rr_mag(D, de, lane);
+            const uint32_t de = dsym(D, ROW(drow));
+            rr_need(D, 24u, lane);
+            const uint32_t dm = dmag(D, de, lane);
             const int32_t top = by ? (int32_t)wuni((uint32_t)top_v) : 0;
             const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
             const int64_t dv = (int64_t)pred + (int64_t)tc_unmap_signed(dm);
@@ -182,27 +219,35 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t* cumv, con
             left = dc;
             pdc_e = de;
             uint32_t val = 0u;
-            uint32_t pos = 1u;
-            uint32_t lrow = LM == 2u ? LVL0 + 4u : LVL0;
             rr_need(D, 32u, lane);
-            uint32_t e = rr_sym(D, run_cum, run_ent);
-            uint32_t idx = pos + ((e >> 12) & 63u);
-            // a run of 63 (end of block) or a position past 63 leaves the loop
-            while (idx <= 63u) {
-                if (LM == 1u) { lrow = LVL0 + tc_ctx_pos_nib(idx); }
-                const uint32_t el = rr_sym(D, cumv[lrow], entv[lrow]);
-                rr_need(D, 24u, lane);
-                const uint32_t m = rr_mag(D, el, lane);
-                val = myscan == idx ? m : val;
-                if (LM == 2u) { lrow = LVL0 + ctx_prev_of(el); }
-                pos = idx + 1u;
-                rr_need(D, 32u, lane);
-                e = rr_sym(D, run_cum, run_ent);
-                idx = pos + ((e >> 12) & 63u);
+            uint32_t e = dsym(D, run);
+            uint32_t idx = e & 127u;
+            if (LM == 1u) {
+                // position buckets (scan position < 8, < 16, < 32, else): the
+                // position only grows, so each bucket's row is fixed in its loop
+                ac_phase<8u>(D, ROW(LVL0 + 0u), run, idx, e, val, lane);
+                ac_phase<16u>(D, ROW(LVL0 + 1u), run, idx, e, val, lane);
+                ac_phase<32u>(D, ROW(LVL0 + 2u), run, idx, e, val, lane);
+                ac_phase<64u>(D, ROW(LVL0 + 3u), run, idx, e, val, lane);
+            } else if (LM == 0u) {
+                ac_phase<64u>(D, ROW(LVL0), run, idx, e, val, lane);
+            } else {
+                // previous-category buckets: the row follows the last level
+                uint32_t lrow = LVL0 + 4u;
+                while (idx < 64u) {
+                    const uint32_t el = dsym(D, ROW(lrow));
+                    lrow = LVL0 + ent_bucket(el);
+                    rr_need(D, 24u, lane);
+                    const uint32_t m = dmag(D, el, lane);
+                    val = lane == idx ? m : val;
+                    rr_need(D, 32u, lane);
+                    e = dsym(D, run);
+                    idx += e & 127u;
+                }
             }
-            // the loop ended on an overflow unless the run was 63; levels
-            // above the legal range anywhere in the block reject the slice
-            bad |= ((e >> 12) & 63u) != 63u ? 1u : 0u;
+            // the loop ends on a position past 63: legal only through the
+            // end-of-block run; levels above the legal range reject the slice
+            bad |= (e & 127u) != 64u ? 1u : 0u;
             bad |= __ballot(lane != 0u && val > lv_max) != 0ull ? 1u : 0u;
             if (bad) { return -9; }
             const int32_t v = lane == 0u ? dc : tc_unmap_signed(val);
@@ -251,10 +296,10 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
         prefix = 1u + dc_n * 29u + 64u + lvl_n * 28u;
         if (rc == 0 && n < prefix + 4u) { rc = -10; }
     }
-    const uint32_t rows = dc_n + 1u + lvl_n;
+    const uint32_t rows_n = dc_n + 1u + lvl_n;
     if (rc == 0) {
         // rows: [0, dc_n) dc, dc_n run, then lvl rows; lane r builds row r
-        if (lane < rows) {
+        if (lane < rows_n) {
             uint32_t off, nsym;
             if (lane < dc_n) { off = 1u + lane * 29u; nsym = 29u; }
             else if (lane == dc_n) { off = 1u + dc_n * 29u; nsym = 64u; }
@@ -268,17 +313,17 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
     }
     if (rc == 0) {
         // symbol s of row r sits in lane 63 - s
-        uint32_t cumv[11], entv[11];
+        uint32_t cumv[11], fv[11], entv[11];
         const uint32_t sym = 63u - lane;
-        const uint32_t magf = (sym > 1u ? sym - 1u : 0u) | (sym != 0u ? 32u : 0u);
+        const uint32_t w = sym > 1u ? sym - 1u : 0u, nz = sym != 0u ? 1u : 0u;
+        const uint32_t mag_ent = ((nz && w <= 23u) ? (1u << w) : 0u) | (w << 24) | (tc_ctx_prev_wnz(w, nz) << 29);
 #pragma unroll
         for (uint32_t r = 0; r < 11u; ++r) {
             const uint32_t nsym = r < dc_n ? 29u : (r == dc_n ? 64u : 28u);
-            const bool live = r < rows && sym < nsym;
-            const uint32_t c = live ? (uint32_t)sCum[r][sym] : 0xFFFFu;
-            const uint32_t f = live ? (uint32_t)sFreq[r][sym] : 0u;
-            cumv[r] = c;
-            entv[r] = (f << 19) | ((r == dc_n ? sym : magf) << 12) | (c & 0xFFFu);
+            const bool live = r < rows_n && sym < nsym;
+            cumv[r] = live ? (uint32_t)sCum[r][sym] : 0xFFFFu;
+            fv[r] = live ? (uint32_t)sFreq[r][sym] : 0u;
+            entv[r] = live ? (r == dc_n ? sym + 1u : mag_ent) : 0u;
         }
         const uint32_t x0 = wuni(((uint32_t)pl[prefix] << 24) | ((uint32_t)pl[prefix + 1u] << 16) |
                                  ((uint32_t)pl[prefix + 2u] << 8) | (uint32_t)pl[prefix + 3u]);
@@ -288,12 +333,12 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
             rr_init(D, x0, pl + prefix + 4u, n - prefix - 4u, lane);
             const uint32_t cols = P.bcols;
             switch (flags & 7u) {
-            case 0u: rc = slice_decode_body<0u, 0u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 1u: rc = slice_decode_body<1u, 0u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 2u: rc = slice_decode_body<2u, 0u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 4u: rc = slice_decode_body<0u, 1u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 5u: rc = slice_decode_body<1u, 1u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
-            default: rc = slice_decode_body<2u, 1u>(D, cumv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 0u: rc = slice_decode_body<0u, 0u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 1u: rc = slice_decode_body<1u, 0u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 2u: rc = slice_decode_body<2u, 0u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 4u: rc = slice_decode_body<0u, 1u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 5u: rc = slice_decode_body<1u, 1u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
+            default: rc = slice_decode_body<2u, 1u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
             }
             // the stream must be consumed exactly
             if (rc == 0 && D.loaded - D.rc != 8u * (n - prefix - 4u)) { rc = -9; }

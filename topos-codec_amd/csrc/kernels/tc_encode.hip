@@ -252,19 +252,50 @@ __device__ static inline void emit_byte(EncEmit& E, uint32_t b, uint32_t lane)
     }
 }
 
-// one symbol with the constants of tc_rans_enc_consts (built in LDS per slice)
-__device__ static inline uint32_t put_sym(uint32_t x, uint32_t A, uint32_t B, uint32_t X, EncEmit& E,
-                                          uint32_t lane)
+// Renormalisation of the encoder: bytes leave from the bottom of the state
+// until it is below the symbol's bound X (about one symbol in five emits)
+__device__ static inline uint32_t put_renorm(uint32_t x, uint32_t X, EncEmit& E, uint32_t lane)
 {
-    while (__builtin_expect(x >= X, 0)) { emit_byte(E, x, lane); x >>= 8; }
-    const uint32_t q = __umulhi(x << 1, A) >> (B >> 24);
-    return x + q * ((B >> 12) & 0xFFFu) + (B & 0xFFFu);
+    do { emit_byte(E, x, lane); x >>= 8; } while (x >= X);
+    return x;
 }
 
+// One symbol, with the constants of tc_rans_enc_consts unpacked: x' = x +
+// (umulhi(2x, A) >> l) (4096 - f) + cum, after renormalising below X = f 2^19
+struct SymK { uint32_t X, A, l, nf, cum; };
+
+__device__ static inline uint32_t put_k(uint32_t x, const SymK& k, EncEmit& E, uint32_t lane)
+{
+    if (__builtin_expect(x >= k.X, 0)) { x = put_renorm(x, k.X, E, lane); }
+    const uint32_t q = __umulhi(x << 1, k.A) >> k.l;
+    return x + q * k.nf + k.cum;
+}
+
+// the symbol's constants from a packed (A, B, X) table entry
+__device__ static inline SymK sym_k(uint32_t A, uint32_t B, uint32_t X)
+{
+    SymK k;
+    k.X = X; k.A = A; k.l = B >> 24; k.nf = (B >> 12) & 0xFFFu; k.cum = B & 0xFFFu;
+    return k;
+}
+
+// the same constants of lane p (a wave-uniform position)
+__device__ static inline SymK sym_k_at(const SymK& v, uint32_t p)
+{
+    SymK k;
+    k.X = __builtin_amdgcn_readlane(v.X, p);
+    k.A = __builtin_amdgcn_readlane(v.A, p);
+    k.l = __builtin_amdgcn_readlane(v.l, p);
+    k.nf = __builtin_amdgcn_readlane(v.nf, p);
+    k.cum = __builtin_amdgcn_readlane(v.cum, p);
+    return k;
+}
+
+// w raw bits c (w <= 23): x' = x 2^w + c after renormalising below 2^(31 - w)
 __device__ static inline uint32_t put_chunk(uint32_t x, uint32_t w, uint32_t c, EncEmit& E, uint32_t lane)
 {
     const uint32_t thr = 1u << (31u - w);
-    while (__builtin_expect(x >= thr, 0)) { emit_byte(E, x, lane); x >>= 8; }
+    if (__builtin_expect(x >= thr, 0)) { x = put_renorm(x, thr, E, lane); }
     return (x << w) | c;
 }
 
@@ -490,9 +521,8 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
     // v_readlane + scalar state arithmetic.
     const uint32_t run_slot = dc_n;
     const uint32_t lvl_slot0 = dc_n + 1u;
-    const uint32_t eobA = wave_u32(sTabA[run_slot * 64u + 63u]);
-    const uint32_t eobB = wave_u32(sTabB[run_slot * 64u + 63u]);
-    const uint32_t eobX = wave_u32(sTabX[run_slot * 64u + 63u]);
+    const SymK eob = sym_k(wave_u32(sTabA[run_slot * 64u + 63u]), wave_u32(sTabB[run_slot * 64u + 63u]),
+                           wave_u32(sTabX[run_slot * 64u + 63u]));
     EncEmit E;
     E.end = scratch + S.scratch + S.scratch_cap;
     E.n = 0u;
@@ -525,28 +555,27 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
         const uint32_t active = (lane != 0u && q != 0) ? 1u : 0u;
         const uint32_t li = active ? (lvl_slot0 + lc) * 64u + cat : 0u;
         const uint32_t ri = active ? run_slot * 64u + run : 0u;
-        const uint32_t lA = sTabA[li], lB = sTabB[li], lX = sTabX[li];
-        const uint32_t rA = sTabA[ri], rB = sTabB[ri], rX = sTabX[ri];
+        // this lane's pair, unpacked once per block: the serial chain below
+        // only reads lanes
+        const SymK lk = sym_k(sTabA[li], sTabB[li], sTabX[li]);
+        const SymK rk = sym_k(sTabA[ri], sTabB[ri], sTabX[ri]);
         // suffix: width w = cat - 1 (0 when cat <= 1), value = low w bits of lm
         const uint32_t w = cat > 1u ? cat - 1u : 0u;
-        const uint32_t sv = (w <= 23u ? (lm & ((1u << w) - 1u)) : 0u) | (w << 24);
+        const uint32_t sv = w <= 23u ? (lm & ((1u << w) - 1u)) : 0u;
 
-        x = put_sym(x, eobA, eobB, eobX, E, lane);
+        x = put_k(x, eob, E, lane);
         uint64_t m = nz;
         while (m) {
             const uint32_t p = 63u - (uint32_t)__clzll(m);
             m &= ~(1ull << p);
-            const uint32_t s = __builtin_amdgcn_readlane(sv, p);
-            const uint32_t sw = s >> 24;
+            const uint32_t sw = __builtin_amdgcn_readlane(w, p);
             if (__builtin_expect(sw <= 23u, 1)) {
-                x = put_chunk(x, sw, s & 0xFFFFFFu, E, lane);
+                x = put_chunk(x, sw, __builtin_amdgcn_readlane(sv, p), E, lane);
             } else {
                 x = put_suffix_wide(x, sw, __builtin_amdgcn_readlane(lm, p), E, lane);
             }
-            x = put_sym(x, __builtin_amdgcn_readlane(lA, p), __builtin_amdgcn_readlane(lB, p),
-                        __builtin_amdgcn_readlane(lX, p), E, lane);
-            x = put_sym(x, __builtin_amdgcn_readlane(rA, p), __builtin_amdgcn_readlane(rB, p),
-                        __builtin_amdgcn_readlane(rX, p), E, lane);
+            x = put_k(x, sym_k_at(lk, p), E, lane);
+            x = put_k(x, sym_k_at(rk, p), E, lane);
         }
         const uint32_t dsym = tc_bitlen(dcm);
         if (dsym > 1u) {
@@ -556,7 +585,7 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
         }
         const uint32_t dctx = dc_model ? tc_ctx_dc(tc_bitlen(dcm_prev), bb != 0u) : 0u;
         const uint32_t di = dctx * 64u + dsym;
-        x = put_sym(x, wave_u32(sTabA[di]), wave_u32(sTabB[di]), wave_u32(sTabX[di]), E, lane);
+        x = put_k(x, sym_k(wave_u32(sTabA[di]), wave_u32(sTabB[di]), wave_u32(sTabX[di])), E, lane);
         if (E.over) { break; }
     }
     // ---------------- flush ----------------
