@@ -39,10 +39,16 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 W, H, QP, NFRAMES = 3840, 2160, 62, 8
 HBM_PEAK_GBS = 8000.0
-# HBM bytes moved by one k_fdct_quant launch over the 8-frame batch, from two
-# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same
-# counters read for a plain 256 MiB device copy (fetch x2.0, write x1.0).
-FDCT_TRAFFIC_BYTES = 796_632_857
+COUNTER_BATCHES = (8, 64)          # isolated batch sizes of the counter passes
+CALIB_MIB = 256                    # plain device copy that scales the HBM byte counters
+# rocprofv3 counter passes over the counter workload (each within one pass's
+# hardware limits: <= 8 SQ, <= 4 TCC (FETCH_SIZE takes 3, WRITE_SIZE 2), <= 2 GRBM)
+PMC_PASSES = {
+    "issue": ["SQ_WAVES", "SQ_INSTS_SALU", "SQ_INSTS_VALU", "SQ_INSTS_SMEM", "SQ_INSTS_BRANCH", "SQ_INSTS_LDS",
+              "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE", "GRBM_COUNT"],
+    "fetch": ["FETCH_SIZE"],
+    "write": ["WRITE_SIZE"],
+}
 SIZED_TARGET = 6_000_000          # bytes per 4K 422 HQ frame for the target-size encode leg
 
 
@@ -569,6 +575,170 @@ def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
             "max_req_per_s": round(max_rate, 2), "loads": loads}
 
 
+def kernel_base(name):
+    """'void k_fdct_quant<1u>(...)' -> 'k_fdct_quant'."""
+    n = name.strip()
+    if n.startswith("void "):
+        n = n[5:]
+    for c in "(<":
+        n = n.split(c)[0]
+    return n.strip()
+
+
+def counter_workload(a):
+    """Child of the counter passes: a plain CALIB_MIB device copy, then
+    isolated encode and decode batches of the resident 4K 422 HQ frames, each
+    size in COUNTER_BATCHES twice (the second is the measured dispatch),
+    timing on. Prints one JSON line: per batch size the symbol pairs and
+    kernel times of the second sequence of each direction."""
+    pkg = load_pkg()
+    abi = pkg.abi
+    syn = pkg.load_synth()
+    codec = pkg.DeviceCodec()
+    cfg = hq_config(abi)
+    frames = [pkg.synth_frame422(syn, W, H, seed) for seed in range(NFRAMES)]
+    nb = CALIB_MIB << 20
+    src_b, dst_b = codec.alloc(nb), codec.alloc(nb)
+    codec.copy(dst_b, src_b, nb)
+    src_b.close(); dst_b.close()
+    dims = abi.plane_dims(W, H, 0, 0)
+    in_addr = []
+    keep = []
+    for fr in frames:
+        bufs = []
+        for p in fr:
+            b = codec.alloc(p.nbytes)
+            b.upload(p)
+            bufs.append(b)
+        keep.append(bufs)
+        in_addr.append([b.addr for b in bufs])
+    nmax = max(COUNTER_BATCHES)
+    pkt = [codec.alloc(64 << 20) for _ in range(nmax)]
+    outp = [[codec.alloc(w * h * 2) for (w, h) in dims] for _ in range(nmax)]
+    codec.set_timing(True)
+    res = {}
+    for n in COUNTER_BATCHES:
+        idx = [i % NFRAMES for i in range(n)]
+        for _ in range(2):
+            st, sizes, rcs, _ = codec.encode_batch_dev([cfg] * n, [in_addr[i] for i in idx], pkt[:n])
+            assert st == 0, codec.last_error()
+            te = codec.timing()
+            pe = codec.timing_pairs()["enc_pairs"]
+            host = [pkt[i].download(sizes[i]).tobytes() for i in range(n)]
+            st, rcs, _ = codec.decode_batch_dev(host, pkt[:n], [[b.addr for b in o] for o in outp[:n]])
+            assert st == 0, codec.last_error()
+            td = codec.timing()
+            pd = codec.timing_pairs()["dec_pairs"]
+        res[str(n)] = {"enc_pairs": pe, "dec_pairs": pd, "slice_enc_ms": te["slice_enc"],
+                       "slice_dec_ms": td["slice_dec"], "fdct_ms": te["fdct"], "idct_ms": td["idct"],
+                       "deblock_ms": td["deblock"]}
+    codec.set_timing(False)
+    cus, clk = codec.device_info()
+    res["device"] = {"cus": cus, "clock_khz": clk}
+    print(json.dumps({"counter_workload": res}), flush=True)
+
+
+def pmc_dispatches(csv_path):
+    """rocprofv3 counter CSV -> {dispatch id: {"kernel", "grid", "ns", counters...}}."""
+    import csv
+    out = {}
+    with open(csv_path, newline="") as f:
+        for row in csv.DictReader(f):
+            d = int(row["Dispatch_Id"])
+            rec = out.setdefault(d, {"kernel": kernel_base(row["Kernel_Name"]), "grid": int(row.get("Grid_Size", 0) or 0)})
+            if row.get("Start_Timestamp") and row.get("End_Timestamp"):
+                rec["ns"] = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+            rec[row["Counter_Name"]] = rec.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    return out
+
+
+def run_counter_passes(timeout_s=150):
+    """Runs the counter workload under rocprofv3 once per PMC_PASSES entry, each
+    pass its own process (cwd /tmp, TMPDIR=/tmp), killed after timeout_s.
+    Returns ({pass: {dispatch: record}}, child JSON) or (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    out, child = {}, None
+    for name, counters in PMC_PASSES.items():
+        d = tempfile.mkdtemp(prefix=f"tc_pmc_{name}_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(timeout_s), prof, "--pmc", *counters, "--output-format", "csv",
+               "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--counter-workload"]
+        print(f"# counter pass {name}: {' '.join(counters)}", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("{\"counter_workload\"")]
+        csvs = []
+        for root, _, files in os.walk(d):
+            csvs += [os.path.join(root, f) for f in files if f.endswith("counter_collection.csv")]
+        if not lines or not csvs:
+            return None, f"pass {name}: rc {r.returncode}, " + (r.stderr.strip().splitlines() or ["no output"])[-1][:300]
+        child = json.loads(lines[-1])["counter_workload"]
+        out[name] = pmc_dispatches(csvs[0])
+        shutil.rmtree(d, ignore_errors=True)
+    return out, child
+
+
+def last_dispatches(recs, kernel):
+    """Dispatch records of one kernel in launch order."""
+    return [recs[k] for k in sorted(recs) if recs[k]["kernel"] == kernel]
+
+
+def counter_report(passes, child):
+    """Entropy issue counters per symbol pair and HBM traffic of the transform
+    kernels from the counter passes (sizes run in COUNTER_BATCHES order, two
+    sequences each; the second is the measured dispatch)."""
+    rep = {"entropy": {}, "traffic": {}}
+    issue = passes["issue"]
+    cus, clk_khz = child["device"]["cus"], child["device"]["clock_khz"]
+    for kern, key, pk in (("k_slice_encode", "encode", "enc_pairs"), ("k_slice_decode", "decode", "dec_pairs")):
+        ds = last_dispatches(issue, kern)
+        per = len(ds) // len(COUNTER_BATCHES)
+        for j, n in enumerate(COUNTER_BATCHES):
+            d = ds[(j + 1) * per - 1]
+            pairs = child[str(n)][pk]
+            # GRBM_GUI_ACTIVE sums the active cycles of every XCD: divided by
+            # the dispatch's duration at the peak clock it gives the XCD count
+            gui, ns = d.get("GRBM_GUI_ACTIVE", 0.0), d.get("ns", 0)
+            xcds = max(1, round(gui / (ns * clk_khz * 1e-6))) if ns and clk_khz else None
+            cyc = gui / xcds if xcds else None
+            busy = {}
+            if cyc and cus:
+                # one scalar and one vector instruction per cycle per CU at most
+                # (the scalar unit is shared by the CU's four SIMDs; a SIMD
+                # issues one wave64 vector instruction per four cycles)
+                busy = {"salu_issue_busy": round(d["SQ_INSTS_SALU"] / (cus * cyc), 4),
+                        "valu_issue_busy": round(d["SQ_INSTS_VALU"] / (cus * cyc), 4)}
+            rep["entropy"].setdefault(key, {})[f"{n}_frames"] = {
+                "pairs": pairs, **busy, "kernel_cycles": round(cyc) if cyc else None, "xcds": xcds, "cus": cus,
+                "salu_per_pair": round(d["SQ_INSTS_SALU"] / pairs, 2),
+                "valu_per_pair": round(d["SQ_INSTS_VALU"] / pairs, 2),
+                "smem_per_pair": round(d["SQ_INSTS_SMEM"] / pairs, 3),
+                "branch_per_pair": round(d["SQ_INSTS_BRANCH"] / pairs, 2),
+                "lds_per_pair": round(d["SQ_INSTS_LDS"] / pairs, 3),
+                "raw": {k: v for k, v in d.items() if k.startswith(("SQ_", "GRBM_")) or k == "ns"},
+            }
+    # HBM bytes: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what
+    # they read for the plain copy of CALIB_MIB read + written
+    cal = float(CALIB_MIB << 20)
+    fc = last_dispatches(passes["fetch"], "k_copy")[0]["FETCH_SIZE"] * 1024.0
+    wc = last_dispatches(passes["write"], "k_copy")[0]["WRITE_SIZE"] * 1024.0
+    ff, wf = cal / fc, cal / wc
+    rep["traffic"]["calibration"] = {"copy_bytes_each_way": int(cal), "fetch_scale": round(ff, 5),
+                                     "write_scale": round(wf, 5)}
+    for kern, key in (("k_fdct_quant", "fdct"), ("k_dequant_idct", "idct"), ("k_deblock", "deblock")):
+        fd = last_dispatches(passes["fetch"], kern)
+        wd = last_dispatches(passes["write"], kern)
+        per = len(fd) // len(COUNTER_BATCHES)
+        f8 = fd[per - 1]["FETCH_SIZE"] * 1024.0 * ff
+        w8 = wd[per - 1]["WRITE_SIZE"] * 1024.0 * wf
+        rep["traffic"][key] = {"fetch_bytes": int(f8), "write_bytes": int(w8), "bytes": int(f8 + w8)}
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -587,11 +757,30 @@ def main():
                     help="skip the other batch configurations (4444 XQ + alpha, uneven batch, serving mix)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one plain device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (entropy issue counters, HBM traffic)")
+    ap.add_argument("--counter-workload", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.counter_workload:
+        counter_workload(a)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is design-only "
                                    "(frames are independent and would be split per GPU)"}))
         sys.exit(2)
+
+    # counter passes first, each in its own profiled process, before this
+    # process touches the GPU
+    counters, counter_child = None, None
+    if not a.no_counters:
+        t0 = time.perf_counter()
+        passes, counter_child = run_counter_passes()
+        if passes is None:
+            counters = {"error": counter_child}
+            counter_child = None
+        else:
+            counters = counter_report(passes, counter_child)
+        print(f"# counter passes {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
 
     pkg = load_pkg()
     abi = pkg.abi
@@ -757,13 +946,16 @@ def main():
     # isolated batched steps (encode, then decode) with per-kernel HIP event timing
     kt = {"fdct": [], "slice_enc": [], "scan": [], "emit": [], "slice_dec": [], "idct": [], "deblock": []}
     seq_times = []
+    pairs = {}
     codec.set_timing(True)
     for _ in range(max(3, min(a.steps, 5))):
         t0 = time.perf_counter()
         s = encode_batch(all_idx)
         te = codec.timing()
+        pairs["enc"] = codec.timing_pairs()["enc_pairs"]
         decode_batch(all_idx)
         td = codec.timing()
+        pairs["dec"] = codec.timing_pairs()["dec_pairs"]
         seq_times.append(time.perf_counter() - t0)
         for k in ("fdct", "slice_enc", "scan", "emit"):
             kt[k].append(te[k])
@@ -833,13 +1025,53 @@ def main():
     bitexact["served_decode0_equals_oracle"] = served_dec0 is not None and all(
         (x == y).all() for x, y in zip(oplanes, served_dec0))
 
-    # roofline of the dominant HBM-bound kernel: fused FDCT + quantisation,
-    # 2 B (u16 sample in) + 4 B (i32 coefficient out) per sample
+    # rooflines of the three HBM-bound kernels over the 8-frame batch:
+    # FDCT + quantisation reads 2 B (u16 sample) and writes 4 B (i32
+    # coefficient) per sample, dequantisation + IDCT the reverse, deblock
+    # reads and writes each u16 sample once; the record names the longest-
+    # running one, traffic from the counter passes (same batch, same kernels)
     samples = nf * (W * H + 2 * ((W + 1) // 2) * H)
-    fdct_ms = float(np.mean(kt["fdct"]))
-    achieved = samples * 6 / (fdct_ms * 1e-3) / 1e9
-    idct_ms = float(np.mean(kt["idct"]))
-    db_ms = float(np.mean(kt["deblock"]))
+    algo = {"fdct": ("k_fdct_quant", 6), "idct": ("k_dequant_idct", 6), "deblock": ("k_deblock", 4)}
+    traffic = (counters or {}).get("traffic", {}) if nf == NFRAMES else {}
+    kern_rl = {}
+    for key, (kname, bps) in algo.items():
+        ms = float(np.mean(kt[key]))
+        ach = samples * bps / (ms * 1e-3) / 1e9
+        tb = traffic.get(key, {}).get("bytes")
+        kern_rl[key] = {"kernel": kname, "ms": round(ms, 4), "bytes_per_launch": samples * bps,
+                        "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
+                        "traffic_ratio": round(tb / (samples * bps), 4) if tb else None}
+    top = max(kern_rl.values(), key=lambda r: r["ms"])
+
+    # entropy chain: symbol pairs (one per block plus one per nonzero AC
+    # coefficient) per second, per direction, with the issue counters
+    ent = {"pairs_per_frame": {"encode": pairs.get("enc", 0) / nf, "decode": pairs.get("dec", 0) / nf}}
+    se_ms, sd_ms = float(np.mean(kt["slice_enc"])), float(np.mean(kt["slice_dec"]))
+    ent["isolated_8_frames"] = {
+        "encode_pairs_per_s": round(pairs["enc"] / (se_ms * 1e-3)) if pairs.get("enc") else None,
+        "decode_pairs_per_s": round(pairs["dec"] / (sd_ms * 1e-3)) if pairs.get("dec") else None,
+        "slice_enc_ms": round(se_ms, 3), "slice_dec_ms": round(sd_ms, 3)}
+    if tune:
+        big = max(tune, key=lambda t: t["batch_frames"])
+        bf = big["batch_frames"]
+        ent[f"batch_{bf}_frames"] = {
+            "encode_pairs_per_s": round(pairs["enc"] / nf * bf / (big["encode"]["ms"] * 1e-3)),
+            "decode_pairs_per_s": round(pairs["dec"] / nf * bf / (big["decode"]["ms"] * 1e-3)),
+            "slice_enc_ms": big["encode"]["ms"], "slice_dec_ms": big["decode"]["ms"]}
+    ent["served_pairs_per_s_each_direction"] = round(fps_batched * pairs.get("enc", 0) / nf)
+    if counters and "entropy" in counters:
+        ent["counters"] = {}
+        for direction, per_batch in counters["entropy"].items():
+            ent["counters"][direction] = {}
+            for bk, c in per_batch.items():
+                raw = c["raw"]
+                rec = {k: c.get(k) for k in ("pairs", "salu_per_pair", "valu_per_pair", "smem_per_pair",
+                                             "branch_per_pair", "lds_per_pair", "salu_issue_busy",
+                                             "valu_issue_busy", "kernel_cycles", "xcds", "cus")}
+                rec["raw"] = raw
+                ent["counters"][direction][bk] = rec
+    elif counters:
+        ent["counters"] = counters
     ms_step = elapsed / a.steps * 1e3
     step_desc = ("one step = the 8 resident frames encoded and their 8 resident packets decoded, as 16 "
                  "requests to the serving loop (continuous batching, encode and decode batches in flight "
@@ -878,12 +1110,10 @@ def main():
                            "step_p50_ms": round(pct(times, 50) * 1e3, 3), "step_p99_ms": round(pct(times, 99) * 1e3, 3),
                            "what": "tcx_serve_step: 8 encodes on one stream while the previous 8 packets decode"},
         "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_fdct_quant", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": FDCT_TRAFFIC_BYTES if nf == NFRAMES else None,
-                     "bytes_per_launch": samples * 6,
-                     "idct_achieved": round(samples * 6 / (idct_ms * 1e-3) / 1e9, 1) if idct_ms > 0 else None,
-                     "deblock_achieved": round(samples * 4 / (db_ms * 1e-3) / 1e9, 1) if db_ms > 0 else None},
+        "roofline": {"bound": "hbm", "kernel": top["kernel"], "achieved": top["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": top["frac"], "traffic": top["traffic"],
+                     "bytes_per_launch": top["bytes_per_launch"], "kernels": kern_rl},
+        "entropy": ent,
         "cpu_baseline": cpu,
         "other_configs": configs,
         "autotune": tune,

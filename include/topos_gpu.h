@@ -278,6 +278,13 @@ void tcx_graph_stats(uint64_t* replays, uint64_t* captures);
  * emit, slice_dec, idct, deblock */
 void tcx_set_timing(int on);
 void tcx_get_timing(float out[7]);
+/* rANS2 symbol pairs coded by the same two sequences (encode, decode), counted
+ * from their coefficients while timing is on: one pair per 8x8 block (DC
+ * category, first run) plus one (level, run) pair per nonzero AC coefficient */
+void tcx_get_timing_pairs(uint64_t out[2]);
+/* compute units and peak shader clock (kHz) of the device the backend runs
+ * on (initialises the GPU; both 0 on failure) */
+int32_t tcx_device_info(uint32_t* cus, uint32_t* clock_khz);
 
 /* the host-built constant tables the kernels receive, for checking them:
  * tcx_ctx_log2 is the model-choice log2 series (f in 1..4096 gives the

@@ -59,6 +59,10 @@ def load_library(path=LIB_PATH):
     lib.tcx_serve_step.argtypes = [vp, vp]
     lib.tcx_set_timing.argtypes = [ctypes.c_int]
     lib.tcx_get_timing.argtypes = [vp]
+    lib.tcx_get_timing_pairs.argtypes = [vp]
+    lib.tcx_get_timing_pairs.restype = None
+    lib.tcx_device_info.argtypes = [vp, vp]
+    lib.tcx_device_info.restype = ctypes.c_int32
     lib.tcx_autotune_batch.argtypes = [vp, ctypes.c_uint32, vp, vp, vp, vp, ctypes.c_uint32, vp]
     lib.tcx_tune_force_tiles.argtypes = [ctypes.c_uint32]
     lib.tcx_tune_force_tiles.restype = None
@@ -756,6 +760,18 @@ class DeviceCodec(ToposCodec):
         t = (ctypes.c_float * 7)()
         self._lib.tcx_get_timing(t)
         return dict(zip(["fdct", "slice_enc", "scan", "emit", "slice_dec", "idct", "deblock"], list(t)))
+
+    def device_info(self):
+        """(compute units, peak shader clock in kHz) of the device."""
+        c, k = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        self._lib.tcx_device_info(ctypes.byref(c), ctypes.byref(k))
+        return c.value, k.value
+
+    def timing_pairs(self):
+        """rANS2 symbol pairs of the last timed encode and decode sequences."""
+        n = (ctypes.c_uint64 * 2)()
+        self._lib.tcx_get_timing_pairs(n)
+        return {"enc_pairs": int(n[0]), "dec_pairs": int(n[1])}
 
     @staticmethod
     def _enc_args(cfgs, plane_addrs, out_bufs, strides):

@@ -281,6 +281,24 @@ void tcx_get_timing(float out[7])
     out[4] = d.slice_dec_ms; out[5] = d.idct_ms; out[6] = d.deblock_ms;
 }
 
+int32_t tcx_device_info(uint32_t* cus, uint32_t* clock_khz)
+{
+    const int32_t rc = tcx_gpu_init();
+    uint32_t c = 0u, k = 0u;
+    if (rc == TC_OK) { tcg_device_info(&c, &k); }
+    if (cus) { *cus = c; }
+    if (clock_khz) { *clock_khz = k; }
+    return rc;
+}
+
+void tcx_get_timing_pairs(uint64_t out[2])
+{
+    tcg_timing e, d;
+    tc_engine_last_timing(&e, &d);
+    out[0] = e.pairs;
+    out[1] = d.pairs;
+}
+
 void tcx_set_graphs(int on) { tcg_set_graphs(on); }
 
 void tcx_graph_stats(uint64_t* replays, uint64_t* captures)

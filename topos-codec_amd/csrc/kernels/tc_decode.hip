@@ -751,6 +751,16 @@ __global__ __launch_bounds__(256) void k_sample(const tcg_dplane* __restrict__ s
     }
 }
 
+// symbol pairs of the batch's rANS2 colour slices (timed sequences only)
+__global__ __launch_bounds__(256) void k_count_pairs_dec(const tcg_dplane* __restrict__ planes,
+                                                         const tcg_dslice* __restrict__ slices,
+                                                         const int32_t* __restrict__ q, unsigned long long* counter)
+{
+    const tcg_dslice S = slices[blockIdx.x];
+    if (S.kind != 0u || S.mode != TCG_MODE_RANS2) { return; }
+    tc_count_pairs(q, S.blk0, S.h * planes[S.plane].bcols, counter);
+}
+
 // ---------------------------------------------------------------------------
 // host launch
 // ---------------------------------------------------------------------------
@@ -798,6 +808,11 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
                                dim3(DB_TX * DB_TY), 0, st, planes_dev);
         }
         tcg_mark(c, 3);
+        unsigned long long* pairs = (info->kinds & TCG_HAS_RANS2) ? tcg_pairs_counter(c) : nullptr;
+        if (pairs != nullptr) {
+            hipLaunchKernelGGL(k_count_pairs_dec, dim3(nslices), dim3(256), 0, st, planes_dev, slices_dev,
+                               (const int32_t*)qbuf, pairs);
+        }
     };
     const uint64_t key[TCG_GRAPH_KEY] = {
         2u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,

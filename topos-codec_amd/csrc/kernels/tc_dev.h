@@ -74,6 +74,27 @@ struct TcgConsts {
     tcg_vlc_books vlc;      // V2 canonical VLC books (tcg_set_vlc)
 };
 
+// rANS2 symbol pairs of the coefficient blocks [blk0, blk0 + nblk) (64 i32
+// each, position 0 the DC in scan and natural order alike), added to
+// *counter: one pair per block (DC category, first run) plus one per nonzero
+// AC coefficient. Every thread of the workgroup takes part.
+__device__ static inline void tc_count_pairs(const int32_t* __restrict__ q, uint32_t blk0, uint32_t nblk,
+                                             unsigned long long* counter)
+{
+    const int4* p = (const int4*)(q + (size_t)blk0 * 64u);
+    const uint32_t nchunk = nblk * 16u;
+    uint32_t n = 0u;
+    for (uint32_t c = threadIdx.x; c < nchunk; c += blockDim.x) {
+        const int4 v = p[c];
+        n += ((c & 15u) != 0u && v.x != 0 ? 1u : 0u) + (v.y != 0 ? 1u : 0u) + (v.z != 0 ? 1u : 0u) +
+             (v.w != 0 ? 1u : 0u);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) { n += (uint32_t)__shfl_xor((int)n, off); }
+    if ((threadIdx.x & 63u) == 0u) { atomicAdd(counter, (unsigned long long)n); }
+    if (threadIdx.x == 0u) { atomicAdd(counter, (unsigned long long)nblk); }
+}
+
 // host-side runtime services (tc_runtime.hip)
 #define TCG_GRAPH_KEY 20
 #define TCG_GRAPHS_PER_CTX 8
@@ -94,15 +115,22 @@ struct tcg_ctx {
     void* pin;      // pinned host staging for strided host copies (grow-only)
     size_t pin_cap;
     int eager;      // 1: never capture graphs on this context (no graph allocation)
+    unsigned long long* pairs_dev;  // symbol-pair counter of timed sequences (allocated on first use)
+    int pairs_valid;                // the last timed sequence counted its pairs
 };
 const TcgConsts* tcg_consts_dev();
 int32_t tcg_check(const char* what);
 // records event i of the launch sequence when timing is on
 void tcg_mark(tcg_ctx* c, int i);
+// the context's symbol-pair counter cleared on its stream, for a sequence
+// that recorded timing events; null (nothing to count) otherwise
+unsigned long long* tcg_pairs_counter(tcg_ctx* c);
 // 1 when launch sequences go through captured graphs (graphs on, timing off)
 int tcg_use_graphs();
 // compute units of the device
 uint32_t tcg_cu_count();
+// peak shader clock of the device (kHz, 0 if unknown)
+uint32_t tcg_clock_khz();
 uint32_t tcg_entropy_capacity_enc(uint32_t wpg, uint32_t max_cols);
 uint32_t tcg_entropy_capacity_dec(uint32_t wpg, uint32_t max_cols);
 // the cached executable graph for key, or null

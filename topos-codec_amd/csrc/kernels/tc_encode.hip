@@ -819,6 +819,16 @@ __global__ __launch_bounds__(256) void k_slice_emit(const TcgConsts* __restrict_
     }
 }
 
+// symbol pairs of the batch's rANS2 colour slices (timed sequences only)
+__global__ __launch_bounds__(256) void k_count_pairs_enc(const tcg_eplane* __restrict__ planes,
+                                                         const tcg_eslice* __restrict__ slices,
+                                                         const int32_t* __restrict__ q, unsigned long long* counter)
+{
+    const tcg_eslice S = slices[blockIdx.x];
+    if (S.mode != TCG_MODE_RANS2 || planes[S.plane].kind != 0u) { return; }
+    tc_count_pairs(q, S.blk0, S.nblk, counter);
+}
+
 // ---------------------------------------------------------------------------
 // host launch
 // ---------------------------------------------------------------------------
@@ -869,6 +879,11 @@ extern "C" int32_t tcg_encode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         hipLaunchKernelGGL(k_slice_emit, dim3(nslices), dim3(256), 0, st, K, frames_dev, slices_dev,
                            (const uint8_t*)scratch, results_dev, frame_out_dev);
         tcg_mark(c, 4);
+        unsigned long long* pairs = (info->kinds & TCG_HAS_RANS2) ? tcg_pairs_counter(c) : nullptr;
+        if (pairs != nullptr) {
+            hipLaunchKernelGGL(k_count_pairs_enc, dim3(nslices), dim3(256), 0, st, planes_dev, slices_dev,
+                               (const int32_t*)qbuf, pairs);
+        }
     };
     const uint64_t key[TCG_GRAPH_KEY] = {
         1u, (uint64_t)(uintptr_t)qctx_dev, nplanes, (uint64_t)(uintptr_t)planes_dev, nslices,

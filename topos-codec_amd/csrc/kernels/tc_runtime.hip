@@ -19,6 +19,7 @@ int g_ready = 0;
 TcgConsts* g_consts = nullptr;
 int g_timing = 0;
 uint32_t g_cus = 0;
+uint32_t g_clock_khz = 0;
 int g_graphs = 1;
 thread_local char g_err[256];
 
@@ -76,6 +77,7 @@ static int32_t scratch_setup_current_device(uint64_t* out)
 }
 
 uint32_t tcg_cu_count() { return g_cus; }
+uint32_t tcg_clock_khz() { return g_clock_khz; }
 
 hipGraphExec_t tcg_graph_find(tcg_ctx* c, const uint64_t* key)
 {
@@ -113,11 +115,32 @@ int32_t tcg_graph_store(tcg_ctx* c, const uint64_t* key, hipGraph_t g, hipGraphE
 
 void tcg_mark(tcg_ctx* c, int i)
 {
-    if (i == 0) { c->nmarks = 0; }
+    if (i == 0) { c->nmarks = 0; c->pairs_valid = 0; }
     if (g_timing && i < 8) {
         (void)hipEventRecord(c->ev[i], c->stream);
         c->nmarks = i + 1;
     }
+}
+
+unsigned long long* tcg_pairs_counter(tcg_ctx* c)
+{
+    // only sequences launched eagerly with timing events count (never under
+    // graph capture, never on the no-allocation context)
+    if (c->nmarks == 0 || c->eager) { return nullptr; }
+    if (c->pairs_dev == nullptr) {
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof(unsigned long long)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        c->pairs_dev = (unsigned long long*)p;
+    }
+    if (hipMemsetAsync(c->pairs_dev, 0, sizeof(unsigned long long), c->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    c->pairs_valid = 1;
+    return c->pairs_dev;
 }
 
 // plain 16-byte-vector device copy: the reference point for HBM byte counters
@@ -146,6 +169,7 @@ int32_t tcg_init(void)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.multiProcessorCount > 0) {
         g_cus = (uint32_t)prop.multiProcessorCount;
+        g_clock_khz = prop.clockRate > 0 ? (uint32_t)prop.clockRate : 0u;
     } else {
         g_cus = 1u;
     }
@@ -195,6 +219,7 @@ void tcg_ctx_destroy(tcg_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     for (int i = 0; i < c->ngraphs; ++i) { (void)hipGraphExecDestroy(c->graphs[i].exec); }
     if (c->pin != nullptr) { (void)hipHostFree(c->pin); }
+    if (c->pairs_dev != nullptr) { (void)hipFree(c->pairs_dev); }
     for (int i = 0; i < 8; ++i) { (void)hipEventDestroy(c->ev[i]); }
     (void)hipStreamDestroy(c->stream);
     free(c);
@@ -393,6 +418,12 @@ int32_t tcg_set_vlc(const tcg_vlc_books* books)
 
 void tcg_set_timing(int on) { g_timing = on; }
 
+void tcg_device_info(uint32_t* cus, uint32_t* clock_khz)
+{
+    *cus = g_ready ? g_cus : 0u;
+    *clock_khz = g_ready ? g_clock_khz : 0u;
+}
+
 void tcg_set_graphs(int on) { g_graphs = on; }
 
 void tcg_graph_stats(tcg_ctx* c, uint64_t* hits, uint64_t* builds)
@@ -418,6 +449,15 @@ void tcg_ctx_timing(tcg_ctx* c, tcg_timing* t)
         t->slice_dec_ms = ms[0]; t->idct_ms = ms[1]; t->deblock_ms = ms[2];
     } else if (c->kind == 3) {
         t->convert_ms = ms[0];
+    }
+    if (c->pairs_valid) {
+        unsigned long long n = 0;
+        if (hipMemcpyAsync(&n, c->pairs_dev, sizeof(n), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+            hipStreamSynchronize(c->stream) == hipSuccess) {
+            t->pairs = n;
+        } else {
+            (void)hipGetLastError();
+        }
     }
 }
 

@@ -315,8 +315,14 @@ typedef struct tcg_timing {
     float fdct_ms, slice_enc_ms, scan_ms, emit_ms;
     float slice_dec_ms, idct_ms, deblock_ms;
     float convert_ms;   /* conversion kernel */
+    /* rANS2 symbol pairs the sequence coded: one per block (DC category and
+     * first run) plus one (level, run) pair per nonzero AC coefficient,
+     * counted from the coefficients after the sequence (0 = not counted) */
+    uint64_t pairs;
 } tcg_timing;
 void tcg_set_timing(int on);
+/* compute units and peak shader clock (kHz) of the device, 0 before init */
+void tcg_device_info(uint32_t* cus, uint32_t* clock_khz);
 /* launch sequences through captured HIP graphs (default on; the environment
  * variable TOPOS_HIP_GRAPHS=0 turns them off at start-up) */
 void tcg_set_graphs(int on);
