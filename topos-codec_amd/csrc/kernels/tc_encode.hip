@@ -219,93 +219,90 @@ __global__ __launch_bounds__(FQ_THREADS) void k_fdct_quant(const tcg_qctx* __res
 // 11..15 dc-prev, 16 run
 #define NROWS 17
 
-// Backward-pass byte sink. Bytes are produced last-to-first; the most recent
-// byte sits at the lowest address. Complete dwords are placed into a vector
-// register (lane 63 - (k & 63) holds dword k of the current 256-byte chunk,
-// counting down from the end of the slice scratch region) by a lane select;
-// a full chunk leaves as one coalesced 256-byte store.
-struct EncEmit {
-    uint8_t* end;     // one past the last byte of the slice scratch region
-    uint32_t n;       // bytes emitted so far (uniform)
-    uint32_t acc;     // up to 3 pending bytes, most recent in the low byte (uniform)
-    uint32_t limit;   // emission cap in bytes (uniform)
-    uint32_t over;    // cap exceeded (uniform)
-    uint32_t obuf;    // lane i: dword of the current chunk
-};
-
-__device__ static inline void emit_byte(EncEmit& E, uint32_t b, uint32_t lane)
-{
-    E.acc = (E.acc << 8) | (b & 0xFFu);
-    E.n += 1u;
-    if (__builtin_expect((E.n & 3u) == 0u, 0)) {
-        const uint32_t k = (E.n >> 2) - 1u;  // dword index counted from the end
-        const uint32_t slot = 63u - (k & 63u);
-        E.obuf = lane == slot ? E.acc : E.obuf;
-        E.acc = 0u;
-        if (slot == 0u) {
-            if (E.n <= E.limit) {
-                *(uint32_t*)(E.end - E.n + 4u * lane) = E.obuf;
-            } else {
-                E.over = 1u;
-            }
-        }
-    }
-}
-
-// Renormalisation of the encoder: bytes leave from the bottom of the state
-// until it is below the symbol's bound X (about one symbol in five emits)
-__device__ static inline uint32_t put_renorm(uint32_t x, uint32_t X, EncEmit& E, uint32_t lane)
-{
-    do { emit_byte(E, x, lane); x >>= 8; } while (x >= X);
-    return x;
-}
-
-// One symbol, with the constants of tc_rans_enc_consts unpacked: x' = x +
-// (umulhi(2x, A) >> l) (4096 - f) + cum, after renormalising below X = f 2^19
-struct SymK { uint32_t X, A, l, nf, cum; };
-
-__device__ static inline uint32_t put_k(uint32_t x, const SymK& k, EncEmit& E, uint32_t lane)
-{
-    if (__builtin_expect(x >= k.X, 0)) { x = put_renorm(x, k.X, E, lane); }
-    const uint32_t q = __umulhi(x << 1, k.A) >> k.l;
-    return x + q * k.nf + k.cum;
-}
+// One symbol's encoder constants, with tc_rans_enc_consts unpacked:
+// x' = x + (umulhi(2x, A) >> l) (4096 - f) + cum after renormalising below
+// X = f 2^19; X8 = 256 X (saturated: x < 2^31 never reaches it when X >= 2^24)
+struct SymK { uint32_t X, X8, A, l, nf, cum; };
 
 // the symbol's constants from a packed (A, B, X) table entry
 __device__ static inline SymK sym_k(uint32_t A, uint32_t B, uint32_t X)
 {
     SymK k;
-    k.X = X; k.A = A; k.l = B >> 24; k.nf = (B >> 12) & 0xFFFu; k.cum = B & 0xFFFu;
+    k.X = X; k.X8 = X >= (1u << 24) ? 0xFFFFFFFFu : X << 8;
+    k.A = A; k.l = B >> 24; k.nf = (B >> 12) & 0xFFFu; k.cum = B & 0xFFFu;
     return k;
 }
 
-// the same constants of lane p (a wave-uniform position)
-__device__ static inline SymK sym_k_at(const SymK& v, uint32_t p)
+// The encoder state stays in [2^23, 2^31). Before a symbol the bytes below
+// the state leave until it is under X: 0, 1 or 2 bytes (x >> 16 < 2^15 <=
+// X), i.e. a shift of 0, 8 or 16 bits.
+__device__ static inline uint32_t enc_sym_shift(uint32_t x, const SymK& k)
 {
-    SymK k;
-    k.X = __builtin_amdgcn_readlane(v.X, p);
-    k.A = __builtin_amdgcn_readlane(v.A, p);
-    k.l = __builtin_amdgcn_readlane(v.l, p);
-    k.nf = __builtin_amdgcn_readlane(v.nf, p);
-    k.cum = __builtin_amdgcn_readlane(v.cum, p);
-    return k;
+    return x >= k.X8 ? 16u : (x >= k.X ? 8u : 0u);
 }
 
-// w raw bits c (w <= 23): x' = x 2^w + c after renormalising below 2^(31 - w)
-__device__ static inline uint32_t put_chunk(uint32_t x, uint32_t w, uint32_t c, EncEmit& E, uint32_t lane)
+__device__ static inline uint32_t enc_sym(uint32_t x, const SymK& k)
 {
-    const uint32_t thr = 1u << (31u - w);
-    if (__builtin_expect(x >= thr, 0)) { x = put_renorm(x, thr, E, lane); }
-    return (x << w) | c;
+    const uint32_t xr = x >> enc_sym_shift(x, k);
+    const uint32_t q = __umulhi(xr << 1, k.A) >> k.l;
+    return __umul24(q, k.nf) + xr + k.cum;
 }
 
-// suffix of a magnitude category: cat - 1 raw bits of m (> 23 bits: the top
-// bits first as one chunk, then the low 16)
-__device__ static inline uint32_t put_suffix_wide(uint32_t x, uint32_t b, uint32_t m, EncEmit& E,
-                                                  uint32_t lane)
+// Before w raw bits the bytes below the state leave until it is under
+// 2^(31 - w): with b = bit length of x (24..31) that is 8 ceil((b + w - 31) / 8)
+// bits = (w + 8 - clz(x)) & ~7 (b + w - 24 >= 0 always)
+__device__ static inline uint32_t enc_chunk_shift(uint32_t x, uint32_t wp8)
 {
-    x = put_chunk(x, b - 16u, (m >> 16) & ((1u << (b - 16u)) - 1u), E, lane);
-    return put_chunk(x, 16u, m & 0xFFFFu, E, lane);
+    return (wp8 - (uint32_t)__builtin_clz(x)) & 0x38u;
+}
+
+// w raw bits c: x' = (x >> shift) 2^w + c; wp8 = w + 8
+__device__ static inline uint32_t enc_chunk(uint32_t x, uint32_t wp8, uint32_t w, uint32_t c)
+{
+    return ((x >> enc_chunk_shift(x, wp8)) << w) | c;
+}
+
+// Raw-bit chunks of one position: the suffix of a magnitude category is
+// cat - 1 raw bits of the mapped magnitude m; more than 23 bits go as two
+// chunks (the top bits, then the low 16), fewer as one (the first chunk is
+// then empty: 0 bits, which moves nothing).
+struct Chunks { uint32_t wp8a, wa, ca, wp8b, wb, cb; };
+
+__device__ static inline Chunks make_chunks(uint32_t w, uint32_t m)
+{
+    Chunks c;
+    if (w > 23u) {
+        c.wa = w - 16u; c.ca = (m >> 16) & ((1u << (w - 16u)) - 1u);
+        c.wb = 16u; c.cb = m & 0xFFFFu;
+    } else {
+        c.wa = 0u; c.ca = 0u;
+        c.wb = w; c.cb = m & ((1u << w) - 1u);
+    }
+    c.wp8a = c.wa + 8u; c.wp8b = c.wb + 8u;
+    return c;
+}
+
+// The serial chain over one block's active positions (bits of m, highest
+// first). Every lane evaluates position p's operations with its own
+// constants; lane p's results are the real ones: it logs the states before
+// its chunks and its level symbol and the state after (for the byte pass) and
+// its final state is broadcast to the next position. WIDE: a first chunk of
+// some lane is not empty.
+template <bool WIDE>
+__device__ static inline uint32_t enc_positions(uint32_t x, uint64_t m, const SymK& rk, const Chunks& ch,
+                                                const SymK& bk, uint32_t* sLog, uint32_t lane)
+{
+    do {
+        const uint32_t p = 63u - (uint32_t)__clzll(m);
+        m &= ~(1ull << p);
+        const uint32_t xa = enc_sym(x, rk);
+        const uint32_t xa2 = WIDE ? enc_chunk(xa, ch.wp8a, ch.wa, ch.ca) : xa;
+        const uint32_t xb = enc_chunk(xa2, ch.wp8b, ch.wb, ch.cb);
+        const uint32_t xo = enc_sym(xb, bk);
+        if (lane == p) { *(uint4*)(sLog + 4u * p) = make_uint4(xa, xa2, xb, xo); }
+        x = __builtin_amdgcn_readlane(xo, p);
+    } while (m != 0ull);
+    return x;
 }
 
 __device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -314,16 +311,22 @@ __device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn
 // words of the model tables: row bytes, frequencies and cumulative frequencies
 #define ENC_MODEL_WORDS ((NROWS * 64 + NROWS * 64 * 2 + NROWS * 65 * 2 + 3) / 4)
 // five f64 costs (one spare) and the slice prefix bytes
-#define ENC_FIXED_WORDS (12 + (TCG_PREFIX_MAX + 3) / 4)
+#define ENC_FIXED_WORDS ((12 + (TCG_PREFIX_MAX + 3) / 4 + 3) & ~3)
+
+// encoder constants of the backward pass (three tables of 11 x 64 words) and
+// its per-position state log (64 x 4 words, 16-byte aligned)
+#define ENC_TAB_WORDS (3u * 11u * 64u)
+#define ENC_LOG_WORDS (4u * 64u)
 
 // LDS words of one encoding wave whose plane is `cols` blocks wide: the phase
-// space (histograms + DC row | model tables | encoder constants), then the
-// fixed part
+// space (histograms + DC row | model tables | encoder constants + state log),
+// then the fixed part; a multiple of four words, so every wave's space is
+// 16-byte aligned
 __host__ __device__ static inline uint32_t enc_phase_words(uint32_t cols)
 {
     const uint32_t a = H_TOTAL + (cols > ENC_MODEL_WORDS ? cols : ENC_MODEL_WORDS);
-    const uint32_t b = 3u * 11u * 64u;
-    return ((a > b ? a : b) + 1u) & ~1u;  // even: the costs that follow are f64
+    const uint32_t b = ENC_TAB_WORDS + ENC_LOG_WORDS;
+    return ((a > b ? a : b) + 3u) & ~3u;  // the costs that follow are f64
 }
 
 __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const tcg_eplane* __restrict__ planes,
@@ -339,7 +342,8 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
     //  model choice      histograms, then the rows' bytes, frequencies and
     //                    cumulative frequencies over the dead DC row
     //  backward pass     [0, 3 x 11 x 64) per-symbol encoder constants, stored
-    //                    from registers after the model tables' last read
+    //                    from registers after the model tables' last read,
+    //                    then the per-position state log (64 x 4 words)
     //  always            the five model costs and the slice prefix bytes, in
     //                    the ENC_FIXED_WORDS after the phase space
     uint32_t* const sH = sDyn;
@@ -516,21 +520,26 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
     wave_sync();
 
     // ---------------- backward pass ----------------
-    // Per block, lanes gather their pair's table entries (level and run) and
-    // suffix; the serial chain then walks the nonzero positions downwards with
-    // v_readlane + scalar state arithmetic.
+    // Symbols are pushed in reverse decode order. Per block, every position
+    // holding a nonzero level, and position 0 (the DC), pushes three things:
+    // the run symbol decoded after it (the step to the next such position
+    // above, or end of block), its suffix chunk(s), and its level symbol (the
+    // DC category symbol at position 0). Lane p holds position p's constants;
+    // the serial chain runs position p's operations in every lane at once from
+    // the broadcast state and keeps lane p's result (one readlane per
+    // position), so the chain is vector arithmetic with no branches. Bytes are
+    // not emitted on the chain: lane p logs the states before its operations,
+    // and after the block every lane recomputes its renormalisation shifts,
+    // finds where its bytes go from the byte counts of the positions above it
+    // and stores them.
     const uint32_t run_slot = dc_n;
     const uint32_t lvl_slot0 = dc_n + 1u;
-    const SymK eob = sym_k(wave_u32(sTabA[run_slot * 64u + 63u]), wave_u32(sTabB[run_slot * 64u + 63u]),
-                           wave_u32(sTabX[run_slot * 64u + 63u]));
-    EncEmit E;
-    E.end = scratch + S.scratch + S.scratch_cap;
-    E.n = 0u;
-    E.acc = 0u;
-    E.limit = S.scratch_cap - TCG_PREFIX_MAX - 8u;
-    E.over = 0u;
-    E.obuf = 0u;
+    uint32_t* const sLog = sDyn + ENC_TAB_WORDS;
+    uint8_t* const end = scratch + S.scratch + S.scratch_cap;
+    const uint32_t limit = S.scratch_cap - TCG_PREFIX_MAX - 8u;
+    uint32_t n = 0u, over = 0u;
     uint32_t x = 1u << 23;
+    const uint64_t above = lane == 63u ? 0ull : (~0ull << (lane + 1u));
     // one block ahead: its coefficients and the DC value of the block after it
     int32_t q_nx = nblk ? qs[(size_t)(nblk - 1u) * 64u] : 0;
     uint32_t d_cur = nblk ? dcmbuf[S.blk0 + nblk - 1u] : 0u;
@@ -543,66 +552,71 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
         d_cur = d_nx;
         d_nx = bb >= 2u ? dcmbuf[S.blk0 + bb - 2u] : 0u;
         const uint64_t nz = __ballot(q != 0) & ~1ull;
+        const uint64_t act = nz | 1ull;
         const uint32_t lm = min(tc_map_signed(q), 0x07FFFFFFu);
         const uint32_t cat = tc_bitlen(lm);
         const uint64_t bl = nz & below;
         const uint32_t pp = bl ? 63u - (uint32_t)__clzll(bl) : 0u;
         const uint32_t pcat = (uint32_t)__shfl((int)cat, (int)pp);
-        const uint32_t run = lane - pp - 1u;
         uint32_t lc = 0u;
         if (lvl_model == 1u) { lc = tc_ctx_pos(lane); }
         else if (lvl_model == 2u) { lc = tc_ctx_prev(pcat, pp != 0u); }
-        const uint32_t active = (lane != 0u && q != 0) ? 1u : 0u;
-        const uint32_t li = active ? (lvl_slot0 + lc) * 64u + cat : 0u;
-        const uint32_t ri = active ? run_slot * 64u + run : 0u;
-        // this lane's pair, unpacked once per block: the serial chain below
-        // only reads lanes
-        const SymK lk = sym_k(sTabA[li], sTabB[li], sTabX[li]);
-        const SymK rk = sym_k(sTabA[ri], sTabB[ri], sTabX[ri]);
-        // suffix: width w = cat - 1 (0 when cat <= 1), value = low w bits of lm
-        const uint32_t w = cat > 1u ? cat - 1u : 0u;
-        const uint32_t sv = w <= 23u ? (lm & ((1u << w) - 1u)) : 0u;
-
-        x = put_k(x, eob, E, lane);
-        uint64_t m = nz;
-        while (m) {
-            const uint32_t p = 63u - (uint32_t)__clzll(m);
-            m &= ~(1ull << p);
-            const uint32_t sw = __builtin_amdgcn_readlane(w, p);
-            if (__builtin_expect(sw <= 23u, 1)) {
-                x = put_chunk(x, sw, __builtin_amdgcn_readlane(sv, p), E, lane);
-            } else {
-                x = put_suffix_wide(x, sw, __builtin_amdgcn_readlane(lm, p), E, lane);
-            }
-            x = put_k(x, sym_k_at(lk, p), E, lane);
-            x = put_k(x, sym_k_at(rk, p), E, lane);
-        }
+        const bool is_act = ((act >> lane) & 1ull) != 0ull;
+        // the next active position above: the run decoded after this one
+        const uint64_t ab = act & above;
+        const uint32_t nx = ab ? (uint32_t)__builtin_ctzll(ab) : 64u;
+        const uint32_t rsym = nx == 64u ? 63u : nx - lane - 1u;
+        // position 0: the DC category in its context row, its suffix
         const uint32_t dsym = tc_bitlen(dcm);
-        if (dsym > 1u) {
-            const uint32_t dw = dsym - 1u;
-            if (dw <= 23u) { x = put_chunk(x, dw, dcm & ((1u << dw) - 1u), E, lane); }
-            else { x = put_suffix_wide(x, dw, dcm, E, lane); }
-        }
         const uint32_t dctx = dc_model ? tc_ctx_dc(tc_bitlen(dcm_prev), bb != 0u) : 0u;
-        const uint32_t di = dctx * 64u + dsym;
-        x = put_k(x, sym_k(wave_u32(sTabA[di]), wave_u32(sTabB[di]), wave_u32(sTabX[di])), E, lane);
-        if (E.over) { break; }
+        const uint32_t bi = lane == 0u ? dctx * 64u + dsym : (is_act ? (lvl_slot0 + lc) * 64u + cat : 0u);
+        const uint32_t ri = is_act ? run_slot * 64u + rsym : 0u;
+        const SymK rk = sym_k(sTabA[ri], sTabB[ri], sTabX[ri]);
+        const SymK bk = sym_k(sTabA[bi], sTabB[bi], sTabX[bi]);
+        const uint32_t ccat = lane == 0u ? dsym : cat;
+        const Chunks ch = make_chunks(ccat > 1u ? ccat - 1u : 0u, lane == 0u ? dcm : lm);
+        const uint32_t x_in = x;
+        if (__ballot(is_act && ch.wa != 0u) == 0ull) {
+            x = enc_positions<false>(x, act, rk, ch, bk, sLog, lane);
+        } else {
+            x = enc_positions<true>(x, act, rk, ch, bk, sLog, lane);
+        }
+        // this block's bytes, in push order: positions from the top, per
+        // position the run symbol's, the chunks' and the level symbol's
+        uint32_t nb = 0u, kR = 0u, kRA = 0u, kRA2 = 0u;
+        uint32_t xr = 0u, xa = 0u, xa2 = 0u, xb = 0u;
+        if (is_act) {
+            const uint4 e = *(const uint4*)(sLog + 4u * lane);
+            xa = e.x; xa2 = e.y; xb = e.z;
+            xr = nx == 64u ? x_in : sLog[4u * nx + 3u];
+            kR = enc_sym_shift(xr, rk) >> 3;
+            kRA = kR + (enc_chunk_shift(xa, ch.wp8a) >> 3);
+            kRA2 = kRA + (enc_chunk_shift(xa2, ch.wp8b) >> 3);
+            nb = kRA2 + (enc_sym_shift(xb, bk) >> 3);
+        }
+        // bytes before this lane's: those of the positions above (nb <= 10)
+        const uint64_t b0 = __ballot(nb & 1u), b1 = __ballot(nb & 2u), b2 = __ballot(nb & 4u),
+                       b3 = __ballot(nb & 8u);
+        const uint32_t tot = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2) + 8 * __popcll(b3));
+        if (n + tot > limit) { over = 1u; break; }
+        if (tot != 0u) {
+            const uint32_t off = (uint32_t)(__popcll(b0 & above) + 2 * __popcll(b1 & above) +
+                                            4 * __popcll(b2 & above) + 8 * __popcll(b3 & above));
+            // byte j of this lane is the (n + off + j + 1)-th byte pushed
+            uint8_t* const base = end - (size_t)(n + off) - 1;
+            for (uint32_t j = 0u; __ballot(j < nb) != 0ull; ++j) {
+                const uint32_t src = j < kR ? xr : (j < kRA ? xa : (j < kRA2 ? xa2 : xb));
+                const uint32_t sft = j - (j < kR ? 0u : (j < kRA ? kR : (j < kRA2 ? kRA : kRA2)));
+                if (j < nb) { *(base - j) = (uint8_t)(src >> (8u * sft)); }
+            }
+            n += tot;
+        }
     }
     // ---------------- flush ----------------
-    // emitted bytes occupy [end - n, end): whole dwords of the partial chunk
-    // from the vector register, then the (n & 3) pending bytes, the state and
-    // the prefix in front.
-    const uint32_t n = E.n;
-    const uint32_t ok_len = (E.over == 0u && n <= E.limit) ? 1u : 0u;
+    // the pushed bytes occupy [end - n, end); the final state (big-endian)
+    // and the prefix go in front of them
+    const uint32_t ok_len = (over == 0u && n <= limit) ? 1u : 0u;
     if (ok_len) {
-        const uint32_t nd = n >> 2;           // whole dwords
-        const uint32_t part = nd & 63u;       // dwords in the unfinished chunk
-        uint8_t* end = E.end;
-        if (part != 0u && lane >= 64u - part) {
-            *(uint32_t*)(end - 256u * ((nd >> 6) + 1u) + 4u * lane) = E.obuf;
-        }
-        const uint32_t r = n & 3u;
-        if (lane < r) { end[-(int64_t)n + (int64_t)lane] = (uint8_t)(E.acc >> (8u * lane)); }
         uint8_t* st = end - n - 4;
         if (lane < 4u) { st[lane] = (uint8_t)(x >> (24u - 8u * lane)); }
         uint8_t* pre = st - prefix_len;
