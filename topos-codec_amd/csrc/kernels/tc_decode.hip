@@ -111,24 +111,22 @@ __device__ static inline uint32_t norm_bits(uint32_t x) { return ((uint32_t)__bu
 
 __device__ static inline void rr_norm(RansRd& D) { rr_shift(D, norm_bits(D.x)); }
 
-// Row vectors of one context model, symbol s in lane 63 - s:
-//   cum  cumulative frequency (0xFFFF for an unused symbol: never <= slot)
-//   f    frequency
-//   ent  what the decoder needs of the symbol: for a run row run + 1 (the
-//        scan-position step; 64 = end of block); for a DC or level row the
-//        magnitude category c as (nz << w) | w << 24 | pb << 29 with
-//        w = max(c - 1, 0) raw suffix bits, nz = (c != 0) (the implicit top
-//        bit, stored only when w <= 23) and pb the previous-category context
-//        bucket the symbol selects for the next one
-struct RowV { uint32_t cum, f, ent; };
+// Row vectors of one context model, symbol s in lane 63 - s: the cumulative
+// frequency (0xFFFF for an unused symbol: never <= slot) and the frequency.
+// Per lane, for the DC and level rows (symbol = magnitude category c = 63 -
+// lane): w = max(c - 1, 0) raw suffix bits, the implicit top bit (c != 0) << w
+// when w <= 23, and the previous-category bucket pb the symbol selects; for
+// the run row the scan-position step run + 1 (64 = end of block).
+struct RowV { uint32_t cum, f; };
+struct LaneSym { uint32_t w, topb, pb, step; };
 
-// Decodes one symbol: every lane forms its symbol's successor state
+// Decodes one run symbol: every lane forms its symbol's successor state
 // f (x >> 12) + slot - cum in one 24-bit multiply-add, and that state's
 // renormalisation bit count, while the wave compares the slot against the
 // cumulative frequencies; the lowest matching lane is the symbol, and
-// readlanes fetch its state, bit count and entry (the scalar unit does only the
+// readlanes fetch its state, bit count and step (the scalar unit does only the
 // slot, the search bit scan and the reservoir shift).
-__device__ static inline uint32_t dsym(RansRd& D, const RowV& r)
+__device__ static inline uint32_t drun(RansRd& D, const RowV& r, uint32_t step)
 {
     const uint32_t slot = D.x & 4095u;
     const uint32_t cand = __umul24(r.f, D.x >> 12) + (slot - r.cum);
@@ -136,7 +134,7 @@ __device__ static inline uint32_t dsym(RansRd& D, const RowV& r)
     const uint32_t L = (uint32_t)__builtin_ctzll(__ballot(r.cum <= slot));
     D.x = __builtin_amdgcn_readlane(cand, L);
     const uint32_t n8 = __builtin_amdgcn_readlane(cand_n8, L);
-    const uint32_t e = __builtin_amdgcn_readlane(r.ent, L);
+    const uint32_t e = __builtin_amdgcn_readlane(step, L);
     rr_shift(D, n8);
     return e;
 }
@@ -149,46 +147,62 @@ __device__ static inline uint32_t rr_bits(RansRd& D, uint32_t w)
     return c;
 }
 
-// mapped magnitude of a DC or level entry: its implicit top bit and w raw
-// suffix bits (more than 23: the top w - 16 bits first, then the low 16); the
-// reservoir holds >= 24 bits
-__device__ static inline uint32_t dmag(RansRd& D, uint32_t e, uint32_t lane)
+// Decodes one DC or level symbol and its raw suffix; returns the mapped
+// magnitude (implicit top bit and suffix) and the symbol's lane. Besides the
+// successor state every lane renormalises it from the top reservoir word
+// (<= 16 bits) and splits off its own suffix bits, so after the search the
+// scalar unit only renormalises once more after the suffix. A suffix over 23
+// bits (category above 24: lanes below 39) is taken in two steps, the low 16
+// bits first. The reservoir holds >= 16 bits on entry.
+__device__ static inline uint32_t dlevel(RansRd& D, const RowV& r, const LaneSym& ls, uint32_t lane, uint32_t& L)
 {
-    const uint32_t w = (e >> 24) & 31u;
-    if (__builtin_expect(w > 23u, 0)) {
+    const uint32_t slot = D.x & 4095u;
+    const uint32_t cand = __umul24(r.f, D.x >> 12) + (slot - r.cum);
+    const uint32_t n8 = norm_bits(cand);
+    L = (uint32_t)__builtin_ctzll(__ballot(r.cum <= slot));
+    const tc_u32x2 pr = { (uint32_t)(D.R >> 32), cand };  // (state : next 32 payload bits)
+    const uint32_t x1 = (uint32_t)((__builtin_bit_cast(uint64_t, pr) << n8) >> 32);
+    const uint32_t m = ls.topb | __builtin_amdgcn_ubfe(x1, 0u, ls.w);
+    const uint32_t xs = x1 >> ls.w;
+    const uint32_t n8s = __builtin_amdgcn_readlane(n8, L);
+    D.R <<= n8s;
+    D.rc -= n8s;
+    rr_need(D, 24u, lane);
+    if (__builtin_expect(L < 39u, 0)) {
+        const uint32_t w = 62u - L;
+        D.x = __builtin_amdgcn_readlane(x1, L);
         const uint32_t lo = rr_bits(D, 16u);
         rr_need(D, 16u, lane);
         const uint32_t hi = rr_bits(D, w - 16u);
         return (1u << w) | (hi << 16) | lo;
     }
-    return (e & 0xFFFFFFu) | rr_bits(D, w);
+    D.x = __builtin_amdgcn_readlane(xs, L);
+    rr_norm(D);
+    return __builtin_amdgcn_readlane(m, L);
 }
-
-__device__ static inline uint32_t ent_bucket(uint32_t e) { return e >> 29; }
 
 // the AC pairs of one block, level rows fixed for scan positions below LIM;
 // val holds mapped magnitudes by scan position (lane = position)
 template <uint32_t LIM>
-__device__ static inline void ac_phase(RansRd& D, const RowV& lvl, const RowV& run, uint32_t& idx, uint32_t& e,
-                                       uint32_t& val, uint32_t lane)
+__device__ static inline void ac_phase(RansRd& D, const RowV& lvl, const RowV& run, const LaneSym& ls,
+                                       uint32_t& idx, uint32_t& e, uint32_t& val, uint32_t lane)
 {
     while (idx < LIM) {
-        const uint32_t el = dsym(D, lvl);
-        rr_need(D, 24u, lane);
-        const uint32_t m = dmag(D, el, lane);
+        uint32_t L;
+        const uint32_t m = dlevel(D, lvl, ls, lane, L);
         val = lane == idx ? m : val;
         rr_need(D, 32u, lane);
-        e = dsym(D, run);
-        idx += e & 127u;
+        e = drun(D, run, ls.step);
+        idx += e;
     }
 }
 
 // row r of the per-lane row arrays (r is wave-uniform)
-#define ROW(r) RowV{ cumv[r], fv[r], entv[r] }
+#define ROW(r) RowV{ cumv[r], fv[r] }
 
 template <uint32_t LM, uint32_t DM>
 __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11], const uint32_t (&fv)[11],
-                                            const uint32_t (&entv)[11], const tcg_dslice& S, uint32_t cols,
+                                            const LaneSym& ls, const tcg_dslice& S, uint32_t cols,
                                             int32_t* __restrict__ qbuf, int32_t* sDcRow, uint32_t lane)
 {
     constexpr uint32_t DC_N = DM ? 5u : 1u;
@@ -199,17 +213,16 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
     const RowV run = ROW(RUN_ROW);
     // lane = scan position; its coefficient's natural index in the block
     int32_t* out = qbuf + (size_t)S.blk0 * 64u + kZigzag[lane];
-    uint32_t pdc_e = 0u;
+    uint32_t pdc_b = 0u;
     int32_t left = 0;
     for (uint32_t by = 0; by < S.h; ++by) {
         for (uint32_t bx = 0; bx < cols; ++bx) {
             const int32_t top_v = sDcRow[bx];
             uint32_t drow = 0u;
-            if (DM) { drow = (by == 0u && bx == 0u) ? 0u : 1u + ent_bucket(pdc_e); }
+            if (DM) { drow = (by == 0u && bx == 0u) ? 0u : 1u + pdc_b; }
             rr_need(D, 32u, lane);
-            const uint32_t de = dsym(D, ROW(drow));
-            rr_need(D, 24u, lane);
-            const uint32_t dm = dmag(D, de, lane);
+            uint32_t Ld;
+            const uint32_t dm = dlevel(D, ROW(drow), ls, lane, Ld);
             const int32_t top = by ? (int32_t)wuni((uint32_t)top_v) : 0;
             const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
             const int64_t dv = (int64_t)pred + (int64_t)tc_unmap_signed(dm);
@@ -217,37 +230,36 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
             const int32_t dc = (int32_t)dv;
             if (lane == 0u) { sDcRow[bx] = dc; }
             left = dc;
-            pdc_e = de;
+            if (DM) { pdc_b = __builtin_amdgcn_readlane(ls.pb, Ld); }
             uint32_t val = 0u;
             rr_need(D, 32u, lane);
-            uint32_t e = dsym(D, run);
-            uint32_t idx = e & 127u;
+            uint32_t e = drun(D, run, ls.step);
+            uint32_t idx = e;
             if (LM == 1u) {
                 // position buckets (scan position < 8, < 16, < 32, else): the
                 // position only grows, so each bucket's row is fixed in its loop
-                ac_phase<8u>(D, ROW(LVL0 + 0u), run, idx, e, val, lane);
-                ac_phase<16u>(D, ROW(LVL0 + 1u), run, idx, e, val, lane);
-                ac_phase<32u>(D, ROW(LVL0 + 2u), run, idx, e, val, lane);
-                ac_phase<64u>(D, ROW(LVL0 + 3u), run, idx, e, val, lane);
+                ac_phase<8u>(D, ROW(LVL0 + 0u), run, ls, idx, e, val, lane);
+                ac_phase<16u>(D, ROW(LVL0 + 1u), run, ls, idx, e, val, lane);
+                ac_phase<32u>(D, ROW(LVL0 + 2u), run, ls, idx, e, val, lane);
+                ac_phase<64u>(D, ROW(LVL0 + 3u), run, ls, idx, e, val, lane);
             } else if (LM == 0u) {
-                ac_phase<64u>(D, ROW(LVL0), run, idx, e, val, lane);
+                ac_phase<64u>(D, ROW(LVL0), run, ls, idx, e, val, lane);
             } else {
                 // previous-category buckets: the row follows the last level
                 uint32_t lrow = LVL0 + 4u;
                 while (idx < 64u) {
-                    const uint32_t el = dsym(D, ROW(lrow));
-                    lrow = LVL0 + ent_bucket(el);
-                    rr_need(D, 24u, lane);
-                    const uint32_t m = dmag(D, el, lane);
+                    uint32_t L;
+                    const uint32_t m = dlevel(D, ROW(lrow), ls, lane, L);
+                    lrow = LVL0 + __builtin_amdgcn_readlane(ls.pb, L);
                     val = lane == idx ? m : val;
                     rr_need(D, 32u, lane);
-                    e = dsym(D, run);
-                    idx += e & 127u;
+                    e = drun(D, run, ls.step);
+                    idx += e;
                 }
             }
             // the loop ends on a position past 63: legal only through the
             // end-of-block run; levels above the legal range reject the slice
-            bad |= (e & 127u) != 64u ? 1u : 0u;
+            bad |= e != 64u ? 1u : 0u;
             bad |= __ballot(lane != 0u && val > lv_max) != 0ull ? 1u : 0u;
             if (bad) { return -9; }
             const int32_t v = lane == 0u ? dc : tc_unmap_signed(val);
@@ -313,17 +325,20 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
     }
     if (rc == 0) {
         // symbol s of row r sits in lane 63 - s
-        uint32_t cumv[11], fv[11], entv[11];
+        uint32_t cumv[11], fv[11];
         const uint32_t sym = 63u - lane;
         const uint32_t w = sym > 1u ? sym - 1u : 0u, nz = sym != 0u ? 1u : 0u;
-        const uint32_t mag_ent = ((nz && w <= 23u) ? (1u << w) : 0u) | (w << 24) | (tc_ctx_prev_wnz(w, nz) << 29);
+        LaneSym ls;
+        ls.w = w;
+        ls.topb = (nz && w <= 23u) ? (1u << w) : 0u;
+        ls.pb = tc_ctx_prev_wnz(w, nz);
+        ls.step = sym + 1u;
 #pragma unroll
         for (uint32_t r = 0; r < 11u; ++r) {
             const uint32_t nsym = r < dc_n ? 29u : (r == dc_n ? 64u : 28u);
             const bool live = r < rows_n && sym < nsym;
             cumv[r] = live ? (uint32_t)sCum[r][sym] : 0xFFFFu;
             fv[r] = live ? (uint32_t)sFreq[r][sym] : 0u;
-            entv[r] = live ? (r == dc_n ? sym + 1u : mag_ent) : 0u;
         }
         const uint32_t x0 = wuni(((uint32_t)pl[prefix] << 24) | ((uint32_t)pl[prefix + 1u] << 16) |
                                  ((uint32_t)pl[prefix + 2u] << 8) | (uint32_t)pl[prefix + 3u]);
@@ -333,12 +348,12 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
             rr_init(D, x0, pl + prefix + 4u, n - prefix - 4u, lane);
             const uint32_t cols = P.bcols;
             switch (flags & 7u) {
-            case 0u: rc = slice_decode_body<0u, 0u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 1u: rc = slice_decode_body<1u, 0u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 2u: rc = slice_decode_body<2u, 0u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 4u: rc = slice_decode_body<0u, 1u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
-            case 5u: rc = slice_decode_body<1u, 1u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
-            default: rc = slice_decode_body<2u, 1u>(D, cumv, fv, entv, S, cols, qbuf, sDcRow, lane); break;
+            case 0u: rc = slice_decode_body<0u, 0u>(D, cumv, fv, ls, S, cols, qbuf, sDcRow, lane); break;
+            case 1u: rc = slice_decode_body<1u, 0u>(D, cumv, fv, ls, S, cols, qbuf, sDcRow, lane); break;
+            case 2u: rc = slice_decode_body<2u, 0u>(D, cumv, fv, ls, S, cols, qbuf, sDcRow, lane); break;
+            case 4u: rc = slice_decode_body<0u, 1u>(D, cumv, fv, ls, S, cols, qbuf, sDcRow, lane); break;
+            case 5u: rc = slice_decode_body<1u, 1u>(D, cumv, fv, ls, S, cols, qbuf, sDcRow, lane); break;
+            default: rc = slice_decode_body<2u, 1u>(D, cumv, fv, ls, S, cols, qbuf, sDcRow, lane); break;
             }
             // the stream must be consumed exactly
             if (rc == 0 && D.loaded - D.rc != 8u * (n - prefix - 4u)) { rc = -9; }
