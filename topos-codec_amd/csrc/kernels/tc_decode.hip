@@ -270,9 +270,9 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
 }
 
 // LDS words of one decoding wave: model frequencies (11 x 64 u16) and
-// cumulative frequencies (11 x 65 u16), a failure flag, then the DC values
-// of the block row above (one per block column of the batch's widest plane)
-#define DEC_FIXED_WORDS 712u
+// cumulative frequencies (11 x 65 u16), then the DC values of the block row
+// above (one per block column of the batch's widest plane)
+#define DEC_FIXED_WORDS 710u
 
 __host__ __device__ static inline uint32_t dec_wave_words(uint32_t cols) { return (DEC_FIXED_WORDS + cols + 1u) & ~1u; }
 
@@ -283,13 +283,11 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
 {
     uint16_t (*const sFreq)[64] = (uint16_t (*)[64])sW;
     uint16_t (*const sCum)[65] = (uint16_t (*)[65])(sW + 352);
-    int32_t* const sBad = (int32_t*)(sW + 710);
     int32_t* const sDcRow = (int32_t*)(sW + DEC_FIXED_WORDS);
 
     const tcg_dslice S = slices[si];
     if (S.kind != 0u || S.mode != TCG_MODE_RANS2) { return; }
     const tcg_dplane P = planes[S.plane];
-    if (lane == 0u) { sBad[0] = 0; }
     wave_sync();
 
     const uint8_t* pl = (const uint8_t*)S.payload;
@@ -310,18 +308,20 @@ __device__ static void slice_decode_one(const TcgConsts* __restrict__ K, const t
     }
     const uint32_t rows_n = dc_n + 1u + lvl_n;
     if (rc == 0) {
-        // rows: [0, dc_n) dc, dc_n run, then lvl rows; lane r builds row r
-        if (lane < rows_n) {
+        // rows: [0, dc_n) dc, dc_n run, then lvl rows; one row at a time, one
+        // symbol per lane
+        for (uint32_t r = 0; r < rows_n; ++r) {
             uint32_t off, nsym;
-            if (lane < dc_n) { off = 1u + lane * 29u; nsym = 29u; }
-            else if (lane == dc_n) { off = 1u + dc_n * 29u; nsym = 64u; }
-            else { off = 1u + dc_n * 29u + 64u + (lane - dc_n - 1u) * 28u; nsym = 28u; }
-            uint8_t rb[64];
-            for (uint32_t s = 0; s < nsym; ++s) { rb[s] = pl[off + s]; }
-            if (tc_model_from_bytes(rb, nsym, sFreq[lane], sCum[lane]) != 0) { atomicOr(&sBad[0], 1); }
+            if (r < dc_n) { off = 1u + r * 29u; nsym = 29u; }
+            else if (r == dc_n) { off = 1u + dc_n * 29u; nsym = 64u; }
+            else { off = 1u + dc_n * 29u + 64u + (r - dc_n - 1u) * 28u; nsym = 28u; }
+            const uint32_t b = lane < nsym ? (uint32_t)pl[off + lane] : 0u;
+            uint32_t f = 0u, c = 0u;
+            if (wave_model_from_bytes(b, lane, f, c) != 0) { rc = -9; break; }
+            if (lane < nsym) { sFreq[r][lane] = (uint16_t)f; sCum[r][lane] = (uint16_t)c; }
+            if (lane == 0u) { sCum[r][nsym] = 4096u; }
         }
         wave_sync();
-        if (wuni((uint32_t)sBad[0]) != 0u) { rc = -9; }
     }
     if (rc == 0) {
         // symbol s of row r sits in lane 63 - s

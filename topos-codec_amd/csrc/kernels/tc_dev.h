@@ -179,6 +179,86 @@ __device__ static inline void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// wave-wide sum and maximum (every lane's value, result wave-uniform) and
+// inclusive prefix sum over lanes 0..lane
+__device__ static inline uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) { v += (uint32_t)__shfl_xor((int)v, off); }
+    return wuni(v);
+}
+
+__device__ static inline uint32_t wave_max(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t u = (uint32_t)__shfl_xor((int)v, off);
+        v = u > v ? u : v;
+    }
+    return wuni(v);
+}
+
+__device__ static inline uint32_t wave_incl_scan(uint32_t v, uint32_t lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)v, (unsigned)off);
+        if (lane >= (uint32_t)off) { v += u; }
+    }
+    return v;
+}
+
+// tc_row_bytes with one symbol per lane: lane s holds symbol s's count (0 at
+// and beyond the row's symbol count; the counts of a slice sum below 2^32).
+// Byte of a used symbol round(count 255 / total) clamped to [1, 255]; an
+// all-zero row becomes byte 1 for symbol 0.
+__device__ static inline uint32_t wave_row_byte(uint32_t count, uint32_t lane)
+{
+    const uint32_t total = wave_sum(count);
+    if (total == 0u) { return lane == 0u ? 1u : 0u; }
+    if (count == 0u) { return 0u; }
+    const uint64_t num = (uint64_t)count * 255u + total / 2u;
+    const uint64_t v = (num >> 32) == 0u ? (uint64_t)((uint32_t)num / total) : num / total;
+    return v == 0u ? 1u : (v > 255u ? 255u : (uint32_t)v);
+}
+
+// tc_model_from_bytes with one symbol per lane: lane s holds symbol s's byte
+// (0 at and beyond the row's symbol count). Floor share b 4096 / total (at
+// least 1 for a used symbol); while the sum is over 4096, one off the largest
+// frequency above 1; while under, one onto the largest remainder of a used
+// symbol (ties: the lowest symbol; an adjusted symbol's remainder becomes 0).
+// Returns 0 with the lane's frequency and cumulative frequency, or -1 (an
+// all-zero row, or no fit); the result is wave-uniform.
+__device__ static inline int wave_model_from_bytes(uint32_t b, uint32_t lane, uint32_t& fo, uint32_t& co)
+{
+    const uint32_t total = wave_sum(b);
+    if (total == 0u) { return -1; }
+    const uint32_t sc = b * 4096u;
+    uint32_t f = b == 0u ? 0u : (sc / total > 1u ? sc / total : 1u);
+    uint32_t rem = sc % total;
+    uint32_t sum = wave_sum(f);
+    while (sum > 4096u) {
+        const uint32_t key = f > 1u ? f : 0u;
+        const uint32_t mx = wave_max(key);
+        if (mx == 0u) { break; }
+        const uint32_t big = (uint32_t)__builtin_ctzll(__ballot(key == mx));
+        if (lane == big) { f -= 1u; rem = 0u; }
+        sum -= 1u;
+    }
+    while (sum < 4096u) {
+        const uint32_t key = b != 0u ? rem + 1u : 0u;
+        const uint32_t mx = wave_max(key);
+        if (mx == 0u) { break; }
+        const uint32_t best = (uint32_t)__builtin_ctzll(__ballot(key == mx));
+        if (lane == best) { f += 1u; rem = 0u; }
+        sum += 1u;
+    }
+    if (sum != 4096u) { return -1; }
+    fo = f;
+    co = wave_incl_scan(f, lane) - f;
+    return 0;
+}
+
 // This wave's first entry and stride in a persistent launch's work list:
 // wave k of the grid takes entries k, k + W, k + 2W, ... (W = waves in the
 // grid). The list is sorted longest first, so every wave gets a similar share

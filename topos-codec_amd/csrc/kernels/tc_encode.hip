@@ -430,19 +430,26 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
     const uint32_t any_viol = __ballot(viol != 0u) != 0ull;
 
     // ---------------- model choice ----------------
-    if (lane < NROWS) {
+    // candidate rows one at a time, one symbol per lane (row bytes from the
+    // counts, then the model; rows from nonzero bytes always fit)
+    for (uint32_t r = 0; r < NROWS; ++r) {
         const uint32_t* cnt;
         uint32_t nsym;
-        if (lane == 0u) { cnt = sH + H_LVL; nsym = 28u; }
-        else if (lane <= 4u) { cnt = sH + H_LPOS + (lane - 1u) * 28u; nsym = 28u; }
-        else if (lane <= 9u) { cnt = sH + H_LPREV + (lane - 5u) * 28u; nsym = 28u; }
-        else if (lane == 10u) { cnt = sH + H_DC; nsym = 29u; }
-        else if (lane <= 15u) { cnt = sH + H_DPREV + (lane - 11u) * 29u; nsym = 29u; }
+        if (r == 0u) { cnt = sH + H_LVL; nsym = 28u; }
+        else if (r <= 4u) { cnt = sH + H_LPOS + (r - 1u) * 28u; nsym = 28u; }
+        else if (r <= 9u) { cnt = sH + H_LPREV + (r - 5u) * 28u; nsym = 28u; }
+        else if (r == 10u) { cnt = sH + H_DC; nsym = 29u; }
+        else if (r <= 15u) { cnt = sH + H_DPREV + (r - 11u) * 29u; nsym = 29u; }
         else { cnt = sH + H_RUN; nsym = 64u; }
-        uint32_t c[64];
-        for (uint32_t s = 0; s < nsym; ++s) { c[s] = cnt[s]; }
-        tc_row_bytes(sRowB[lane], nsym, c);
-        tc_model_from_bytes(sRowB[lane], nsym, sFreq[lane], sCum[lane]);
+        const uint32_t b = wave_row_byte(lane < nsym ? cnt[lane] : 0u, lane);
+        uint32_t f = 0u, c = 0u;
+        (void)wave_model_from_bytes(b, lane, f, c);
+        if (lane < nsym) {
+            sRowB[r][lane] = (uint8_t)b;
+            sFreq[r][lane] = (uint16_t)f;
+            sCum[r][lane] = (uint16_t)c;
+        }
+        if (lane == 0u) { sCum[r][nsym] = 4096u; }
     }
     wave_sync();
     // exact cost sums, one candidate per lane, in the reference's order:
