@@ -160,6 +160,9 @@ __device__ static inline uint32_t dlevel(RansRd& D, const RowV& r, const LaneSym
     const uint32_t cand = __umul24(r.f, D.x >> 12) + (slot - r.cum);
     const uint32_t n8 = norm_bits(cand);
     L = (uint32_t)__builtin_ctzll(__ballot(r.cum <= slot));
+    // a 32-bit scalar from here on (the compiler would otherwise test the
+    // 64-bit bit count with a vector compare)
+    asm("" : "+s"(L));
     const tc_u32x2 pr = { (uint32_t)(D.R >> 32), cand };  // (state : next 32 payload bits)
     const uint32_t x1 = (uint32_t)((__builtin_bit_cast(uint64_t, pr) << n8) >> 32);
     const uint32_t m = ls.topb | __builtin_amdgcn_ubfe(x1, 0u, ls.w);
@@ -209,9 +212,10 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
     constexpr uint32_t RUN_ROW = DC_N;
     constexpr uint32_t LVL0 = DC_N + 1u;
     const uint32_t dc_max = S.dc_max, lv_max = S.lv_max;
-    const int64_t dlim = dc_max > 0x08000000u ? ((int64_t)1 << 27) : ((int64_t)1 << 25);
+    const int32_t dlim = dc_max > 0x08000000u ? (1 << 27) : (1 << 25);
     const RowV run = ROW(RUN_ROW);
     // lane = scan position; its coefficient's natural index in the block
+    // (blocks of the band are consecutive)
     int32_t* out = qbuf + (size_t)S.blk0 * 64u + kZigzag[lane];
     uint32_t pdc_b = 0u;
     int32_t left = 0;
@@ -224,10 +228,15 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
             uint32_t Ld;
             const uint32_t dm = dlevel(D, ROW(drow), ls, lane, Ld);
             const int32_t top = by ? (int32_t)wuni((uint32_t)top_v) : 0;
-            const int32_t pred = tc_dc_predict(bx != 0u, left, by != 0u, top);
-            const int64_t dv = (int64_t)pred + (int64_t)tc_unmap_signed(dm);
+            // tc_dc_predict in 32 bits: accepted DC values lie within +-dlim
+            // <= 2^27, so left + top + 1 cannot overflow, and the rounded
+            // mean (s + 1) >> 1 (arithmetic) equals its s >= 0 / s < 0 split;
+            // with dm <= dc_max < 2^28 the residual is within +-2^27, so dv is
+            // exact whenever the block is accepted
+            const int32_t pred = bx != 0u ? (by != 0u ? (left + top + 1) >> 1 : left) : (by != 0u ? top : 0);
+            const int32_t dv = pred + (int32_t)((dm >> 1) ^ (0u - (dm & 1u)));   // tc_unmap_signed
             uint32_t bad = (dm > dc_max || dv > dlim || dv < -dlim) ? 1u : 0u;
-            const int32_t dc = (int32_t)dv;
+            const int32_t dc = dv;
             if (lane == 0u) { sDcRow[bx] = dc; }
             left = dc;
             if (DM) { pdc_b = __builtin_amdgcn_readlane(ls.pb, Ld); }
@@ -262,8 +271,9 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
             bad |= e != 64u ? 1u : 0u;
             bad |= __ballot(lane != 0u && val > lv_max) != 0ull ? 1u : 0u;
             if (bad) { return -9; }
-            const int32_t v = lane == 0u ? dc : tc_unmap_signed(val);
-            out[((size_t)by * cols + bx) * 64u] = v;
+            const int32_t v = lane == 0u ? dc : (int32_t)((val >> 1) ^ (0u - (val & 1u)));  // tc_unmap_signed
+            *out = v;
+            out += 64;
         }
     }
     return 0;
