@@ -666,46 +666,88 @@ __device__ static inline void db_edge(int32_t& p1, int32_t& p0, int32_t& q0, int
 #define DB_TX TCG_DB_TX   // tiles per tile row of a workgroup (one wave)
 #define DB_TY TCG_DB_TY   // tile rows per workgroup
 
-// grid: x = group of DB_TX tiles along a tile row, y = group of DB_TY tile
-// rows, z = plane of the batch
-template <uint32_t G>
-__global__ __launch_bounds__(DB_TX * DB_TY) void k_deblock(const tcg_dplane* __restrict__ planes)
+typedef __attribute__((address_space(1))) uint8_t gbyte;
+
+// half-row of a tile (g/2 samples) as packed sample pairs, at a byte offset
+// of the plane (offsets are 32-bit, so the address is the plane base in
+// scalar registers plus a 32-bit vector offset)
+template <uint32_t G, bool VEC>
+__device__ static inline uint2 db_load_half(const gbyte* base, uint32_t off)
 {
-    const tcg_dplane P = planes[blockIdx.z];
-    if (!P.deblock || (P.db_unit ? P.db_unit : 8u) != G) { return; }
-    const int32_t w = (int32_t)P.vis_w, h = (int32_t)P.vis_h;
-    const int32_t k = (int32_t)(blockIdx.x * DB_TX + (threadIdx.x % DB_TX));
-    const int32_t r = (int32_t)(blockIdx.y * DB_TY + (threadIdx.x / DB_TX));
+    const __attribute__((address_space(1))) uint16_t* p = (const __attribute__((address_space(1))) uint16_t*)(base + off);
+    if (G == 8u) {
+        if (VEC) {
+            const uint64_t q = *(const __attribute__((address_space(1))) uint64_t*)(base + off);
+            return make_uint2((uint32_t)q, (uint32_t)(q >> 32));
+        }
+        return make_uint2(p[0] | (uint32_t)p[1] << 16, p[2] | (uint32_t)p[3] << 16);
+    }
+    if (VEC) { return make_uint2(*(const __attribute__((address_space(1))) uint32_t*)(base + off), 0u); }
+    return make_uint2(p[0] | (uint32_t)p[1] << 16, 0u);
+}
+
+template <uint32_t G, bool VEC>
+__device__ static inline void db_store_half(gbyte* base, uint32_t off, const int32_t* in)
+{
+    if (G == 8u) {
+        const uint2 d = make_uint2((uint32_t)in[0] | (uint32_t)in[1] << 16, (uint32_t)in[2] | (uint32_t)in[3] << 16);
+        if (VEC) { *(__attribute__((address_space(1))) uint64_t*)(base + off) = (uint64_t)d.y << 32 | d.x; }
+        else {
+            __attribute__((address_space(1))) uint16_t* p = (__attribute__((address_space(1))) uint16_t*)(base + off);
+            p[0] = (uint16_t)d.x; p[1] = (uint16_t)(d.x >> 16); p[2] = (uint16_t)d.y; p[3] = (uint16_t)(d.y >> 16);
+        }
+    } else {
+        const uint32_t d = (uint32_t)in[0] | (uint32_t)in[1] << 16;
+        if (VEC) { *(__attribute__((address_space(1))) uint32_t*)(base + off) = d; }
+        else {
+            __attribute__((address_space(1))) uint16_t* p = (__attribute__((address_space(1))) uint16_t*)(base + off);
+            p[0] = (uint16_t)d; p[1] = (uint16_t)(d >> 16);
+        }
+    }
+}
+
+// One corner-centred tile (k, r): half-rows outside the plane read as zeros
+// from a clamped in-plane address (no branch around the loads) and are not
+// written back. Byte offsets inside a plane fit 32 bits (planes are below
+// 2^31 samples).
+template <uint32_t G, bool VEC>
+__device__ static inline void deblock_tile(const tcg_dplane& P, int32_t k, int32_t r)
+{
     constexpr int32_t HG = (int32_t)G / 2;
-    if (k > w / (int32_t)G || r > h / (int32_t)G) { return; }
+    const int32_t w = (int32_t)P.vis_w, h = (int32_t)P.vis_h;
     const int32_t x0 = k * (int32_t)G - HG, y0 = r * (int32_t)G - HG;  // tile origin
-    uint16_t* base = (uint16_t*)P.dst;
-    const size_t stride = P.stride;
-    // vector access needs the tile's half-rows (HG samples) aligned: the
-    // plane's base and pitch at HG * 2 bytes (w and h are multiples of g)
-    const bool vec = ((((uintptr_t)base) | (stride * 2u)) & (uintptr_t)(HG * 2 - 1)) == 0u;
-    int32_t v[G][G];
+    gbyte* const base = (gbyte*)(uintptr_t)P.dst;
+    const uint32_t pitch = P.stride * 2u;
+    bool hin[2];
+    uint32_t xb[2];
+#pragma unroll
+    for (int32_t half = 0; half < 2; ++half) {
+        const int32_t xs = x0 + half * HG;
+        hin[half] = xs >= 0 && xs < w;   // a half-row is all in or all out
+        xb[half] = hin[half] ? (uint32_t)xs * 2u : 0u;
+    }
+    // every load first (unconditional, clamped addresses), then the masking
+    uint2 raw[G][2];
 #pragma unroll
     for (int32_t i = 0; i < (int32_t)G; ++i) {
         const int32_t y = y0 + i;
-        const bool row_in = y >= 0 && y < h;
-        const uint16_t* row = base + (size_t)(row_in ? y : 0) * stride;
+        const uint32_t ro = (y >= 0 && y < h) ? (uint32_t)y * pitch : 0u;
+#pragma unroll
+        for (int32_t half = 0; half < 2; ++half) { raw[i][half] = db_load_half<G, VEC>(base, ro + xb[half]); }
+    }
+    int32_t v[G][G];
+#pragma unroll
+    for (int32_t i = 0; i < (int32_t)G; ++i) {
+        const bool rin = y0 + i >= 0 && y0 + i < h;
 #pragma unroll
         for (int32_t half = 0; half < 2; ++half) {
-            const int32_t xs = x0 + half * HG;
-            const bool in = row_in && xs >= 0 && xs < w;   // a half-row is all in or all out
+            const bool in = rin && hin[half];
+            const uint32_t a = in ? raw[i][half].x : 0u, b = in ? raw[i][half].y : 0u;
+            v[i][half * HG + 0] = (int32_t)(a & 0xFFFFu);
+            v[i][half * HG + 1] = (int32_t)(a >> 16);
             if (G == 8u) {
-                uint2 d = make_uint2(0u, 0u);
-                if (in) {
-                    if (vec) { d = *(const uint2*)(row + xs); }
-                    else { d = make_uint2(row[xs] | (uint32_t)row[xs + 1] << 16, row[xs + 2] | (uint32_t)row[xs + 3] << 16); }
-                }
-                v[i][half * 4 + 0] = (int32_t)(d.x & 0xFFFFu); v[i][half * 4 + 1] = (int32_t)(d.x >> 16);
-                v[i][half * 4 + 2] = (int32_t)(d.y & 0xFFFFu); v[i][half * 4 + 3] = (int32_t)(d.y >> 16);
-            } else {
-                uint32_t d = 0u;
-                if (in) { d = vec ? *(const uint32_t*)(row + xs) : (row[xs] | (uint32_t)row[xs + 1] << 16); }
-                v[i][half * 2 + 0] = (int32_t)(d & 0xFFFFu); v[i][half * 2 + 1] = (int32_t)(d >> 16);
+                v[i][half * HG + 2] = (int32_t)(b & 0xFFFFu);
+                v[i][half * HG + 3] = (int32_t)(b >> 16);
             }
         }
     }
@@ -733,26 +775,29 @@ __global__ __launch_bounds__(DB_TX * DB_TY) void k_deblock(const tcg_dplane* __r
     for (int32_t i = 0; i < (int32_t)G; ++i) {
         const int32_t y = y0 + i;
         if (y < 0 || y >= h) { continue; }
-        uint16_t* row = base + (size_t)y * stride;
+        const uint32_t ro = (uint32_t)y * pitch;
 #pragma unroll
         for (int32_t half = 0; half < 2; ++half) {
-            const int32_t xs = x0 + half * HG;
-            if (xs < 0 || xs >= w) { continue; }
-            if (G == 8u) {
-                const uint2 d = make_uint2((uint32_t)v[i][half * 4 + 0] | (uint32_t)v[i][half * 4 + 1] << 16,
-                                           (uint32_t)v[i][half * 4 + 2] | (uint32_t)v[i][half * 4 + 3] << 16);
-                if (vec) { *(uint2*)(row + xs) = d; }
-                else {
-                    row[xs] = (uint16_t)d.x; row[xs + 1] = (uint16_t)(d.x >> 16);
-                    row[xs + 2] = (uint16_t)d.y; row[xs + 3] = (uint16_t)(d.y >> 16);
-                }
-            } else {
-                const uint32_t d = (uint32_t)v[i][half * 2 + 0] | (uint32_t)v[i][half * 2 + 1] << 16;
-                if (vec) { *(uint32_t*)(row + xs) = d; }
-                else { row[xs] = (uint16_t)d; row[xs + 1] = (uint16_t)(d >> 16); }
-            }
+            if (hin[half]) { db_store_half<G, VEC>(base, ro + xb[half], &v[i][half * HG]); }
         }
     }
+}
+
+// grid: x = group of DB_TX tiles along a tile row, y = group of DB_TY tile
+// rows, z = plane of the batch
+template <uint32_t G>
+__global__ __launch_bounds__(DB_TX * DB_TY) void k_deblock(const tcg_dplane* __restrict__ planes)
+{
+    const tcg_dplane P = planes[blockIdx.z];
+    if (!P.deblock || (P.db_unit ? P.db_unit : 8u) != G) { return; }
+    const int32_t k = (int32_t)(blockIdx.x * DB_TX + (threadIdx.x % DB_TX));
+    const int32_t r = (int32_t)(blockIdx.y * DB_TY + (threadIdx.x / DB_TX));
+    if (k > (int32_t)P.vis_w / (int32_t)G || r > (int32_t)P.vis_h / (int32_t)G) { return; }
+    // vector access needs the tile's half-rows (g/2 samples) aligned: the
+    // plane's base and pitch at g bytes (w and h are multiples of g)
+    const bool vec = ((P.dst | ((uint64_t)P.stride * 2u)) & (uint64_t)(G - 1u)) == 0u;
+    if (vec) { deblock_tile<G, true>(P, k, r); }
+    else { deblock_tile<G, false>(P, k, r); }
 }
 
 // Target-size decode: target sample (tx, ty) of a plane takes the source
