@@ -428,13 +428,13 @@ __device__ static inline void iq_load(const tcg_dplane& P, uint32_t brow, uint32
 }
 
 // Exact rounding of the f64 transform output v (an integer below 2^53) by
-// the reference's sign-split rule r = sign(v) floor((|v| + 2^31) / 2^32): |v|
-// 2^-32 is exact (a power-of-two scale) and adding 0.5 is exact below 2^52,
-// so one FMA forms (|v| + 2^31) / 2^32 and the truncating conversion floors it.
+// the reference's sign-split rule r = sign(v) floor((|v| + 2^31) / 2^32):
+// t = v 2^-32 is exact (a power-of-two scale), |t| < 2^21 with at most 32
+// fraction bits, so t + copysign(1/2, t) is exact as well (one FMA), and its
+// truncation toward zero is sign(t) floor(|t| + 1/2).
 __device__ static inline int32_t idct_round(double v)
 {
-    const int32_t m = (int32_t)__builtin_fma(__builtin_fabs(v), 0x1p-32, 0.5);
-    return v < 0.0 ? -m : m;
+    return (int32_t)__builtin_fma(v, 0x1p-32, __builtin_copysign(0.5, v));
 }
 
 __device__ static inline int32_t idct_round(int64_t v)
@@ -497,12 +497,14 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
             }
         }
         if (j < nb) {
-            bool big = false;
+            // lanes holding a coefficient above its clamp limit:
+            // |q| > lim  <=>  q + lim (unsigned) > 2 lim
+            uint64_t big = 0ull;
             if (!WIDE) {
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
-                    const int32_t lim = sQlim[u * 8 + v];
-                    big |= qv[v] > lim || qv[v] < -lim;
+                    const uint32_t lim = (uint32_t)sQlim[u * 8 + v];
+                    big |= __ballot((uint32_t)qv[v] + lim > 2u * lim);
                 }
             }
             acc_t G[8];
@@ -514,7 +516,7 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
                     d = d > (1 << 30) ? (1 << 30) : (d < -(1 << 30) ? -(1 << 30) : d);
                     G[v] = (acc_t)d * (acc_t)kTW[u * 8 + v];
                 }
-            } else if (__builtin_expect(__ballot(big) == 0ull, 1)) {
+            } else if (__builtin_expect(big == 0ull, 1)) {
 #pragma unroll
                 for (int v = 0; v < 8; ++v) { G[v] = (acc_t)((double)qv[v] * sQW[u * 8 + v]); }
             } else {
@@ -547,7 +549,7 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
 #pragma unroll
         for (int y = 0; y < 8; ++y) {
             const int32_t sv = idct_round(Xv[y]) + mid;
-            col[y] = sv < 0 ? 0 : (sv > maxv ? maxv : sv);
+            col[y] = min(max(sv, 0), maxv);
         }
     }
     __syncthreads();   // every thread is done reading sB
