@@ -721,6 +721,25 @@ def counter_report(passes, child):
                 "lds_per_pair": round(d["SQ_INSTS_LDS"] / pairs, 3),
                 "raw": {k: v for k, v in d.items() if k.startswith(("SQ_", "GRBM_")) or k == "ns"},
             }
+    # issue counters of the transform kernels (8-frame dispatch): vector and
+    # scalar instructions per sample and the vector issue busy fraction
+    samples8 = NFRAMES * (W * H + 2 * ((W + 1) // 2) * H)
+    rep["transform_issue"] = {}
+    for kern, key in (("k_fdct_quant", "fdct"), ("k_dequant_idct", "idct"), ("k_deblock", "deblock")):
+        ds = last_dispatches(issue, kern)
+        per = len(ds) // len(COUNTER_BATCHES)
+        if per == 0:
+            continue
+        d = ds[per - 1]
+        gui, ns = d.get("GRBM_GUI_ACTIVE", 0.0), d.get("ns", 0)
+        xcds = max(1, round(gui / (ns * clk_khz * 1e-6))) if ns and clk_khz else None
+        cyc = gui / xcds if xcds else None
+        rep["transform_issue"][key] = {
+            "valu_per_sample": round(d["SQ_INSTS_VALU"] * 64 / samples8, 3),
+            "salu_per_sample": round(d["SQ_INSTS_SALU"] * 64 / samples8, 3),
+            "valu_issue_busy": round(d["SQ_INSTS_VALU"] / (cus * cyc), 4) if cyc and cus else None,
+            "waves": d.get("SQ_WAVES"), "ns": ns}
+
     # HBM bytes: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by what
     # they read for the plain copy of CALIB_MIB read + written
     cal = float(CALIB_MIB << 20)
@@ -1041,6 +1060,8 @@ def main():
         kern_rl[key] = {"kernel": kname, "ms": round(ms, 4), "bytes_per_launch": samples * bps,
                         "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
                         "traffic_ratio": round(tb / (samples * bps), 4) if tb else None}
+    for key, iss in ((counters or {}).get("transform_issue") or {}).items():
+        kern_rl[key]["issue"] = iss
     top = max(kern_rl.values(), key=lambda r: r["ms"])
 
     # entropy chain: symbol pairs (one per block plus one per nonzero AC

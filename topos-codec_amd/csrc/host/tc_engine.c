@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include "tc_host.h"
+#include "../kernels/tc_math.h"
 
 typedef struct dbuf { uint64_t p; size_t cap; } dbuf;
 
@@ -467,6 +468,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
             E->blk0 = blk;
             E->coded_w = fh->pcw[p];
             E->tile0 = tile;
+            tc_qdiv32_consts((fh->bcols[p] + TCG_FQ_BLOCKS - 1u) / TCG_FQ_BLOCKS, &E->tile_m, &E->tile_l);
             /* quantiser context i belongs to plane i of the batch */
             E->qctx = iq;
             if (tc_is_alpha(fh, p)) {
@@ -761,6 +763,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             D->row_slice = row;
             D->coded_w = fh->pcw[p];
             D->tile0 = tile;
+            tc_qdiv32_consts((fh->bcols[p] + TCG_IQ_BLOCKS - 1u) / TCG_IQ_BLOCKS, &D->tile_m, &D->tile_l);
             D->limit = scaled ? r->limit : 63u;
             const int colour = !tc_is_alpha(fh, p);
             if (!colour) {

@@ -695,6 +695,7 @@ void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bd, ui
     q->maxv = (int32_t)((1u << bd) - 1u);
     q->fclamp = bd <= 12u ? (1 << 25)
                           : (int32_t)((((int64_t)1 << 25) * (int64_t)((1u << (bd - 1u)) - 1u)) / 2047);
+    for (int i = 0; i < 64; ++i) { q->qlim[i] = (uint32_t)q->fclamp / q->Q[i]; }
 }
 
 /* alpha = clamp(2 * max(1, round(scale(qp) * 3 / 173056)), 3, 48)

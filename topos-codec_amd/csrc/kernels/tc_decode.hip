@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void k_slice_decode(const TcgConsts* __restric
 // column x of the block. For bit depth <= 12 every intermediate is an integer
 // below 2^53 in magnitude, so f64 FMA is exact; deeper samples use int64.
 // ---------------------------------------------------------------------------
-#define IQ_BLOCKS 32
+#define IQ_BLOCKS TCG_IQ_BLOCKS
 
 // 8-point inverse transform (exact even/odd form of out[x] = sum_v in[v] M[v][x])
 __device__ static inline void idct8(const double* in, double* out) { tc_idct8_f64(in, out); }
@@ -470,7 +470,7 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
             // Q W < 2^40 is exact; |q| <= floor(fclamp / Q) keeps |q Q| within
             // the clamp, so such a coefficient dequantises as q (Q W) exactly
             sQW[t] = (double)Q * sW[t];
-            sQlim[t] = (int32_t)((uint64_t)fclamp / Q);
+            sQlim[t] = (int32_t)qc->qlim[t];
         }
     }
     uint16_t* out = (uint16_t*)P.dst;   // deblocking, if any, runs in place afterwards
@@ -602,8 +602,8 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
     for (uint32_t k = 0; k < TPW; ++k) {
         const uint32_t local = blockIdx.x * TPW + k;
         if (local < ntiles) {
-            const uint32_t brow = local / tiles_per_row;
-            const uint32_t bx0 = (local % tiles_per_row) * IQ_BLOCKS;
+            const uint32_t brow = tc_qdiv32(local, P.tile_m, P.tile_l);
+            const uint32_t bx0 = (local - brow * tiles_per_row) * IQ_BLOCKS;
             iq_load(P, brow, bx0, min((uint32_t)IQ_BLOCKS, P.bcols - bx0), qbuf, pre[k]);
         }
     }
@@ -611,8 +611,8 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
     for (uint32_t k = 0; k < TPW; ++k) {
         const uint32_t local = blockIdx.x * TPW + k;
         if (local >= ntiles) { break; }
-        const uint32_t brow = local / tiles_per_row;
-        const uint32_t bx0 = (local % tiles_per_row) * IQ_BLOCKS;
+        const uint32_t brow = tc_qdiv32(local, P.tile_m, P.tile_l);
+        const uint32_t bx0 = (local - brow * tiles_per_row) * IQ_BLOCKS;
         const uint32_t nb = min((uint32_t)IQ_BLOCKS, P.bcols - bx0);
         const tcg_rowrec rr = rowrec[(size_t)pi * max_rows + brow];
         const tcg_qctx* qc = &qctxs[rr.qctx];

@@ -42,8 +42,9 @@ __device__ static inline FqTile fq_tile(const tcg_eplane& P, uint32_t local)
     FqTile T;
     const uint32_t tiles_per_row = (P.bcols + FQ_BLOCKS - 1u) / FQ_BLOCKS;
     T.live = local < P.brows * tiles_per_row;
-    T.brow = T.live ? local / tiles_per_row : 0u;
-    T.bx0 = T.live ? (local % tiles_per_row) * FQ_BLOCKS : 0u;
+    const uint32_t brow = tc_qdiv32(local, P.tile_m, P.tile_l);   // local / tiles_per_row
+    T.brow = T.live ? brow : 0u;
+    T.bx0 = T.live ? (local - brow * tiles_per_row) * FQ_BLOCKS : 0u;
     T.nb = T.live ? min((uint32_t)FQ_BLOCKS, P.bcols - T.bx0) : 0u;
     return T;
 }

@@ -51,6 +51,8 @@ typedef struct tcg_vlc_books {
 #ifndef TCG_FQ_BLOCKS
 #define TCG_FQ_BLOCKS 32u
 #endif
+/* decode transform tile: blocks of one block row per IDCT tile */
+#define TCG_IQ_BLOCKS 32u
 
 typedef struct tcg_qctx {
     uint32_t qmul[64];   /* the quantiser divide floor(n / Q) = tc_qdiv32(n, qmul, qsh) */
@@ -58,6 +60,7 @@ typedef struct tcg_qctx {
     uint32_t Q[64];
     uint32_t half[64];
     uint32_t dz[64];
+    uint32_t qlim[64];   /* floor(fclamp / Q): larger |q| dequantise to the clamp */
     int32_t fclamp;
     uint32_t bit_depth;
     int32_t mid;
@@ -75,6 +78,8 @@ typedef struct tcg_eplane {
     uint32_t alpha_shift;/* alpha mode 2: pre-quantisation shift (0 = off) */
     uint32_t coded_w;    /* coded width in samples */
     uint32_t tile0;      /* first transform tile of this plane (ascending) */
+    uint32_t tile_m, tile_l; /* transform tile t of the plane sits in block row
+                                tc_qdiv32(t, tile_m, tile_l) (= t / tiles per row) */
 } tcg_eplane;
 
 typedef struct tcg_eslice {
@@ -151,6 +156,8 @@ typedef struct tcg_dplane {
     uint32_t limit;      /* highest kept scan position; 0 or >= 63 keep all (reduced decode) */
     uint32_t db_unit;    /* deblock grid in samples (4 or 8; 0 = 8) */
     uint32_t sample;     /* target descriptor of a target-size decode: sample this plane */
+    uint32_t tile_m, tile_l; /* transform tile t of the plane sits in block row
+                                tc_qdiv32(t, tile_m, tile_l) (= t / tiles per row) */
 } tcg_dplane;
 
 /* per (plane of the batch, block row): the quantiser context and the slice
