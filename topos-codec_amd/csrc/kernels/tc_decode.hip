@@ -427,14 +427,24 @@ __device__ static inline void iq_load(const tcg_dplane& P, uint32_t brow, uint32
     }
 }
 
-// Exact rounding of the f64 transform output v (an integer below 2^53) by
-// the reference's sign-split rule r = sign(v) floor((|v| + 2^31) / 2^32):
-// t = v 2^-32 is exact (a power-of-two scale), |t| < 2^21 with at most 32
-// fraction bits, so t + copysign(1/2, t) is exact as well (one FMA), and its
-// truncation toward zero is sign(t) floor(|t| + 1/2).
-__device__ static inline int32_t idct_round(double v)
+// Exact rounding of the f64 transform output by the reference's sign-split
+// rule r = sign(v) floor((|v| + 2^31) / 2^32) for the integer v < 2^53 of the
+// int64 transform: the f64 path carries t = v 2^-32 throughout (the weights
+// are pre-scaled by 2^-32, a power of two, so every sum stays exact: the
+// integers below 2^53 become multiples of 2^-32 below 2^21), t +
+// copysign(1/2, t) is exact as well, and its truncation toward zero is
+// sign(t) floor(|t| + 1/2).
+__device__ static inline int32_t idct_round(double t)
 {
-    return (int32_t)__builtin_fma(v, 0x1p-32, __builtin_copysign(0.5, v));
+    return (int32_t)(t + __builtin_copysign(0.5, t));
+}
+
+// x + mid clamped to [0, maxv] in one vector median
+__device__ static inline int32_t idct_clamp(int32_t v, int32_t mid, int32_t maxv)
+{
+    int32_t r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(v + mid), "s"(maxv));
+    return r;
 }
 
 __device__ static inline int32_t idct_round(int64_t v)
@@ -467,8 +477,9 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
         const uint32_t Q = qc->Q[t];
         sQ[t] = Q;
         if (!WIDE) {
-            // Q W < 2^40 is exact; |q| <= floor(fclamp / Q) keeps |q Q| within
-            // the clamp, so such a coefficient dequantises as q (Q W) exactly
+            // Q W 2^-32 is exact (Q W < 2^40); |q| <= floor(fclamp / Q) keeps
+            // |q Q| within the clamp, so such a coefficient dequantises as
+            // q (Q W 2^-32) exactly
             sQW[t] = (double)Q * sW[t];
             sQlim[t] = (int32_t)qc->qlim[t];
         }
@@ -522,7 +533,8 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
             } else {
                 // |q| < 2^27 and Q < 2^26, so q Q is an integer below 2^53 and
                 // exact in f64; the clamp bounds it by 2^25 and the weight by
-                // 2^14, so the product with the weight is exact as well
+                // 2^14, so the product with the (2^-32-scaled) weight is exact
+                // as well
                 const double fc = (double)fclamp;
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
@@ -547,10 +559,7 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
         acc_t Xv[8];
         idct8(B, Xv);
 #pragma unroll
-        for (int y = 0; y < 8; ++y) {
-            const int32_t sv = idct_round(Xv[y]) + mid;
-            col[y] = min(max(sv, 0), maxv);
-        }
+        for (int y = 0; y < 8; ++y) { col[y] = idct_clamp(idct_round(Xv[y]), mid, maxv); }
     }
     __syncthreads();   // every thread is done reading sB
     if (j < nb) {
@@ -594,7 +603,8 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
     const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (P.kind != 0u) { return; }
-    if (threadIdx.x < 64u) { sW[threadIdx.x] = (double)kTW[threadIdx.x]; }  // read after each tile's barrier
+    // weights scaled by 2^-32 (exact): the f64 path then ends in the value to round
+    if (threadIdx.x < 64u) { sW[threadIdx.x] = (double)kTW[threadIdx.x] * 0x1p-32; }  // read after each tile's barrier
     const uint32_t tiles_per_row = (P.bcols + IQ_BLOCKS - 1u) / IQ_BLOCKS;
     const uint32_t ntiles = P.brows * tiles_per_row;
     IqRow pre[TPW];
