@@ -453,13 +453,9 @@ __device__ static inline int32_t idct_round(int64_t v)
 }
 
 // LDS of one IDCT tile: the first pass's output (f64 or int64, 65-word pitch
-// per block), reused after the second pass as the tile's output samples
-// (8 rows of 32 blocks x 8 u16) for the row stores
+// per block)
 struct IqLds {
-    union {
-        uint64_t B[IQ_BLOCKS * 65];
-        uint16_t out[8][IQ_BLOCKS * 8 + 8];
-    };
+    uint64_t B[IQ_BLOCKS * 65];
 };
 
 template <bool WIDE>
@@ -560,25 +556,15 @@ __device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __
         idct8(B, Xv);
 #pragma unroll
         for (int y = 0; y < 8; ++y) { col[y] = idct_clamp(idct_round(Xv[y]), mid, maxv); }
-    }
-    __syncthreads();   // every thread is done reading sB
-    if (j < nb) {
+        // thread (j, x) holds column x of block j: per row a wave's 64 lanes
+        // store 128 contiguous bytes (8 blocks x 8 columns)
+        const uint32_t px = (bx0 + j) * 8u + x;
+        if (px < P.vis_w) {
+            __attribute__((address_space(1))) uint16_t* dst =
+                (__attribute__((address_space(1))) uint16_t*)(uintptr_t)P.dst + (size_t)y0 * P.stride + px;
 #pragma unroll
-        for (int y = 0; y < 8; ++y) { L.out[y][j * 8 + x] = (uint16_t)col[y]; }
-    }
-    __syncthreads();
-    // thread t stores row y = t / 32 of block jj = t % 32: 16 contiguous bytes,
-    // a half-wave one 512-byte row span of the tile
-    {
-        const uint32_t y = t >> 5, jj = t & 31u;
-        const uint32_t py = y0 + y, px = (bx0 + jj) * 8u;
-        if (jj < nb && py < P.vis_h) {
-            uint16_t* dst = out + (size_t)py * P.stride + px;
-            const uint16_t* src = &L.out[y][jj * 8];
-            if (px + 8u <= P.vis_w && ((uintptr_t)dst & 15u) == 0u) {
-                *(uint4*)dst = *(const uint4*)src;
-            } else {
-                for (uint32_t k = 0; k < 8u && px + k < P.vis_w; ++k) { dst[k] = src[k]; }
+            for (int y = 0; y < 8; ++y) {
+                if (y0 + (uint32_t)y < P.vis_h) { dst[(size_t)y * P.stride] = (uint16_t)col[y]; }
             }
         }
     }
