@@ -121,22 +121,27 @@ struct RowV { uint32_t cum, f; };
 struct LaneSym { uint32_t w, topb, pb, step; };
 
 // Decodes one run symbol: every lane forms its symbol's successor state
-// f (x >> 12) + slot - cum in one 24-bit multiply-add, and that state's
-// renormalisation bit count, while the wave compares the slot against the
-// cumulative frequencies; the lowest matching lane is the symbol, and
-// readlanes fetch its state, bit count and step (the scalar unit does only the
-// slot, the search bit scan and the reservoir shift).
+// f (x >> 12) + slot - cum in one 24-bit multiply-add while the wave compares
+// the slot against the cumulative frequencies; the lowest matching lane is the
+// symbol, readlanes fetch its state and step, and the scalar unit
+// renormalises.
 __device__ static inline uint32_t drun(RansRd& D, const RowV& r, uint32_t step)
 {
     const uint32_t slot = D.x & 4095u;
     const uint32_t cand = __umul24(r.f, D.x >> 12) + (slot - r.cum);
-    const uint32_t cand_n8 = norm_bits(cand);   // every successor is >= 2^11 > 0
     const uint32_t L = (uint32_t)__builtin_ctzll(__ballot(r.cum <= slot));
     D.x = __builtin_amdgcn_readlane(cand, L);
-    const uint32_t n8 = __builtin_amdgcn_readlane(cand_n8, L);
     const uint32_t e = __builtin_amdgcn_readlane(step, L);
-    rr_shift(D, n8);
+    rr_norm(D);   // every successor is >= 2^11 > 0
     return e;
+}
+
+// lane idx of v becomes m (a scalar): one lane write
+__device__ static inline uint32_t set_lane(uint32_t v, uint32_t m, uint32_t idx)
+{
+    // the lane select goes through M0 (one scalar operand per vector instruction)
+    asm("s_mov_b32 m0, %2\n\ts_nop 0\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(m), "s"(idx) : "m0");
+    return v;
 }
 
 __device__ static inline uint32_t rr_bits(RansRd& D, uint32_t w)
@@ -193,7 +198,7 @@ __device__ static inline void ac_phase(RansRd& D, const RowV& lvl, const RowV& r
     while (idx < LIM) {
         uint32_t L;
         const uint32_t m = dlevel(D, lvl, ls, lane, L);
-        val = lane == idx ? m : val;
+        val = set_lane(val, m, idx);
         rr_need(D, 32u, lane);
         e = drun(D, run, ls.step);
         idx += e;
@@ -260,7 +265,7 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
                     uint32_t L;
                     const uint32_t m = dlevel(D, ROW(lrow), ls, lane, L);
                     lrow = LVL0 + __builtin_amdgcn_readlane(ls.pb, L);
-                    val = lane == idx ? m : val;
+                    val = set_lane(val, m, idx);
                     rr_need(D, 32u, lane);
                     e = drun(D, run, ls.step);
                     idx += e;

@@ -285,22 +285,33 @@ __device__ static inline Chunks make_chunks(uint32_t w, uint32_t m)
 
 // The serial chain over one block's active positions (bits of m, highest
 // first). Every lane evaluates position p's operations with its own
-// constants; lane p's results are the real ones: it logs the states before
-// its chunks and its level symbol and the state after (for the byte pass) and
-// its final state is broadcast to the next position. WIDE: a first chunk of
+// constants (the run symbol's renormalisation, which only depends on the
+// broadcast state and lane p's bound, runs on the scalar unit); lane p's
+// results are the real ones: it logs the states before and after its level
+// symbol (and, WIDE, before its second chunk) for the byte pass, and its final
+// state is broadcast to the next position. WIDE: a first chunk of
 // some lane is not empty.
 template <bool WIDE>
 __device__ static inline uint32_t enc_positions(uint32_t x, uint64_t m, const SymK& rk, const Chunks& ch,
                                                 const SymK& bk, uint32_t* sLog, uint32_t lane)
 {
+    uint32_t* const mylog = sLog + 4u * lane;   // lane p writes entry p
     do {
         const uint32_t p = 63u - (uint32_t)__clzll(m);
         m &= ~(1ull << p);
-        const uint32_t xa = enc_sym(x, rk);
+        // the run symbol: its renormalisation on the scalar unit with lane
+        // p's bound (the state is wave-uniform), the update in every lane
+        const uint32_t X = __builtin_amdgcn_readlane(rk.X, p);
+        const uint32_t x1 = x >= X ? x >> 8 : x;
+        const uint32_t xr = x1 >= X ? x1 >> 8 : x1;
+        const uint32_t xa = __umul24(__umulhi(xr << 1, rk.A) >> rk.l, rk.nf) + xr + rk.cum;
         const uint32_t xa2 = WIDE ? enc_chunk(xa, ch.wp8a, ch.wa, ch.ca) : xa;
         const uint32_t xb = enc_chunk(xa2, ch.wp8b, ch.wb, ch.cb);
         const uint32_t xo = enc_sym(xb, bk);
-        if (lane == p) { *(uint4*)(sLog + 4u * p) = make_uint4(xa, xa2, xb, xo); }
+        if (lane == p) {
+            if (WIDE) { *(uint4*)mylog = make_uint4(xb, xo, xa2, 0u); }
+            else { *(uint2*)mylog = make_uint2(xb, xo); }
+        }
         x = __builtin_amdgcn_readlane(xo, p);
     } while (m != 0ull);
     return x;
@@ -315,7 +326,8 @@ __device__ static inline uint32_t wave_u32(uint32_t v) { return __builtin_amdgcn
 #define ENC_FIXED_WORDS ((12 + (TCG_PREFIX_MAX + 3) / 4 + 3) & ~3)
 
 // encoder constants of the backward pass (three tables of 11 x 64 words) and
-// its per-position state log (64 x 4 words, 16-byte aligned)
+// its per-position state log (64 x 4 words, 16-byte aligned: the states
+// before and after the level symbol and before a second chunk)
 #define ENC_TAB_WORDS (3u * 11u * 64u)
 #define ENC_LOG_WORDS (4u * 64u)
 
@@ -584,7 +596,8 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
         const uint32_t ccat = lane == 0u ? dsym : cat;
         const Chunks ch = make_chunks(ccat > 1u ? ccat - 1u : 0u, lane == 0u ? dcm : lm);
         const uint32_t x_in = x;
-        if (__ballot(is_act && ch.wa != 0u) == 0ull) {
+        const bool wide = __ballot(is_act && ch.wa != 0u) != 0ull;
+        if (!wide) {
             x = enc_positions<false>(x, act, rk, ch, bk, sLog, lane);
         } else {
             x = enc_positions<true>(x, act, rk, ch, bk, sLog, lane);
@@ -595,8 +608,10 @@ __device__ static void slice_encode_one(const TcgConsts* __restrict__ K, const t
         uint32_t xr = 0u, xa = 0u, xa2 = 0u, xb = 0u;
         if (is_act) {
             const uint4 e = *(const uint4*)(sLog + 4u * lane);
-            xa = e.x; xa2 = e.y; xb = e.z;
-            xr = nx == 64u ? x_in : sLog[4u * nx + 3u];
+            xb = e.x;
+            xr = nx == 64u ? x_in : sLog[4u * nx + 1u];
+            xa = enc_sym(xr, rk);
+            xa2 = wide ? e.z : xa;
             kR = enc_sym_shift(xr, rk) >> 3;
             kRA = kR + (enc_chunk_shift(xa, ch.wp8a) >> 3);
             kRA2 = kRA + (enc_chunk_shift(xa2, ch.wp8b) >> 3);
