@@ -1139,7 +1139,8 @@ def main():
         "kernel_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": top["kernel"], "achieved": top["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": top["frac"], "traffic": top["traffic"],
-                     "bytes_per_launch": top["bytes_per_launch"], "kernels": kern_rl},
+                     "bytes_per_launch": top["bytes_per_launch"], "kernels": kern_rl,
+                     "traffic_calibration": traffic.get("calibration")},
         "entropy": ent,
         "cpu_baseline": cpu,
         "other_configs": configs,
