@@ -150,6 +150,44 @@ def test_device_request_padded_strides(pkg, abi, syn, oracle, codec):
                 assert np.array_equal(got, w), (req.mode, req.scale, i)
 
 
+@pytest.mark.parametrize("pad,shift", [(8, 0), (4, 8), (1, 2)])
+def test_deblock_plane_alignment(pkg, abi, syn, oracle, codec, pad, shift):
+    """Full-size decode of a deblocking stream into device planes of each
+    alignment class of the in-place filter: base and row pitch 16-byte aligned
+    (aligned row chunks), 8-byte aligned (half-row vectors) and 2-byte aligned
+    (sample by sample)."""
+    pkt, cfg = _packet(pkg, abi, syn, oracle, PACKETS[0])
+    req = abi.make_request(abi.DECODE_MODE_FULL)
+    _, want, _ = oracle_request(oracle, abi, pkt, cfg, req)
+    a = np.frombuffer(pkt, np.uint8)
+    b = codec.alloc(len(pkt))
+    b.upload(a)
+    host = (ctypes.c_void_p * 1)(a.ctypes.data)
+    sizes = (ctypes.c_size_t * 1)(len(pkt))
+    dpk = (ctypes.c_uint64 * 1)(b.addr)
+    planes = (ctypes.c_uint64 * 4)()
+    strides = (ctypes.c_size_t * 4)()
+    infos = (abi.FrameOutput * 1)()
+    rcs = (ctypes.c_int32 * 1)()
+    bufs = []
+    for p, w in enumerate(want):
+        st = w.shape[1] + pad
+        d = codec.alloc(st * w.shape[0] * 2 + 16)
+        d.upload(np.full(st * w.shape[0] + 8, 0x5A5A, np.uint16))
+        bufs.append((d, st))
+        planes[p] = d.addr + shift
+        strides[p] = st
+    st = codec._lib.tcx_decode_batch_dev_request(1, host, sizes, dpk, planes, strides, infos, rcs,
+                                                 ctypes.byref(req))
+    assert st == 0 and rcs[0] == 0, codec.last_error()
+    for (d, stp), w in zip(bufs, want):
+        raw = d.download(stp * w.shape[0] * 2 + 16)
+        got = raw[shift: shift + stp * w.shape[0] * 2].view(np.uint16).reshape(w.shape[0], stp)
+        assert np.array_equal(got[:, : w.shape[1]], w), (pad, shift)
+        # the row padding and the bytes before the plane keep their contents
+        assert (got[:, w.shape[1]:] == 0x5A5A).all() and (raw[:shift] == 0x5A).all(), (pad, shift)
+
+
 def test_image_preview(pkg, abi, syn, oracle, codec):
     pkt, cfg = _packet(pkg, abi, syn, oracle, PACKETS[0])
     data = toos_wrap(pkt, 0)

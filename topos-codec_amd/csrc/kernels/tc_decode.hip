@@ -637,11 +637,13 @@ __global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict
 // segments read: filtering the vertical segment of a tile and then its
 // horizontal segment gives the result of the whole-plane vertical pass
 // followed by the whole-plane horizontal pass (deblock.c:163-212), and tiles
-// share no sample, so they run independently in place. One thread per tile:
-// its g rows of g samples arrive with 8-byte (g = 8) or 4-byte (g = 4) vector
-// loads when the plane's base and row pitch allow it; a wave covers 64
-// consecutive tiles of a tile row, so each of its row loads is one contiguous
-// span of 64 g samples.
+// share no sample, so they run independently in place. One thread per tile,
+// a wave per run of consecutive tiles of a tile row. With g = 8 on a plane
+// whose base and row pitch are 16-byte aligned the wave reads and writes each
+// row as aligned 16-byte chunks and passes half-tiles between neighbouring
+// lanes (deblock_chunks); otherwise each thread reads its tile's half-rows
+// with 8-byte (g = 8) or 4-byte (g = 4) vector loads where the alignment
+// allows, or sample by sample (deblock_tile).
 // ---------------------------------------------------------------------------
 __device__ static inline uint32_t db_absu(int32_t v) { return (uint32_t)(v < 0 ? -v : v); }
 __device__ static inline int32_t db_clampv(int32_t v, int32_t m) { return v < 0 ? 0 : (v > m ? m : v); }
@@ -666,8 +668,8 @@ __device__ static inline void db_edge(int32_t& p1, int32_t& p0, int32_t& q0, int
     p0 = np0; q0 = nq0; p1 = np1; q1 = nq1;
 }
 
-#define DB_TX TCG_DB_TX   // tiles per tile row of a workgroup (one wave)
-#define DB_TY TCG_DB_TY   // tile rows per workgroup
+#define DB_TX TCG_DB_TX         // lanes of a wave (tiles of one tile row)
+#define DB_WAVES TCG_DB_WAVES   // waves of a workgroup, side by side along a tile row
 
 typedef __attribute__((address_space(1))) uint8_t gbyte;
 
@@ -786,20 +788,118 @@ __device__ static inline void deblock_tile(const tcg_dplane& P, int32_t k, int32
     }
 }
 
-// grid: x = group of DB_TX tiles along a tile row, y = group of DB_TY tile
-// rows, z = plane of the batch
+typedef uint32_t db_u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t db_u32x2 __attribute__((ext_vector_type(2)));
+
+// a lane's value as seen by the lane above it (lane 0 reads 0) and by the
+// lane below it (lane 63 reads 0): DPP whole-wave shifts by one lane
+__device__ static inline uint32_t db_from_below(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+__device__ static inline uint32_t db_from_above(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+}
+
+// g = 8, plane base and row pitch 16-byte aligned: the wave covers tiles
+// t0 .. t0 + 62 of tile row r. Lane l reads the aligned chunk c = t0 + l - 1
+// of each of the tile row's 8 rows (samples 8c .. 8c + 7: the right half of
+// tile c and the left half of tile c + 1), so each row load of the wave is
+// one contiguous 1 KiB span. Lane l >= 1 owns tile c: its left half arrives
+// from lane l - 1 and, once filtered, goes back to lane l - 1, which writes
+// the whole chunk. Lane 0 only supplies and writes back tile t0's left half,
+// and lane 63 writes only its own tile's right half (the upper half of its
+// chunk is the next wave's first tile). Rows and chunks outside the plane are
+// read from a clamped in-plane address and reach no filtered or stored
+// sample: no edge is filtered there and no such chunk is written. Called
+// with every lane of the wave active (the DPP shifts read all 64 lanes).
+__device__ static inline void deblock_chunks(const tcg_dplane& P, int32_t t0, int32_t r, int32_t lane)
+{
+    constexpr int32_t G = 8, HG = 4;
+    const int32_t w = (int32_t)P.vis_w, h = (int32_t)P.vis_h;
+    const int32_t c = t0 + lane - 1, x0 = c * G - HG, y0 = r * G - HG;
+    gbyte* const base = (gbyte*)(uintptr_t)P.dst;
+    const uint32_t pitch = P.stride * 2u;
+    const bool cin = c >= 0 && c < w / G;
+    const uint32_t cb = cin ? (uint32_t)c * 16u : 0u;
+    db_u32x4 raw[G];
+#pragma unroll
+    for (int32_t i = 0; i < G; ++i) {
+        const int32_t y = y0 + i;
+        const uint32_t ro = (y >= 0 && y < h) ? (uint32_t)y * pitch : 0u;
+        raw[i] = *(const __attribute__((address_space(1))) db_u32x4*)(base + ro + cb);
+    }
+    int32_t v[G][G];
+#pragma unroll
+    for (int32_t i = 0; i < G; ++i) {
+        const uint32_t a = db_from_below(raw[i].z), b = db_from_below(raw[i].w);
+        v[i][0] = (int32_t)(a & 0xFFFFu);
+        v[i][1] = (int32_t)(a >> 16);
+        v[i][2] = (int32_t)(b & 0xFFFFu);
+        v[i][3] = (int32_t)(b >> 16);
+        v[i][4] = (int32_t)(raw[i].x & 0xFFFFu);
+        v[i][5] = (int32_t)(raw[i].x >> 16);
+        v[i][6] = (int32_t)(raw[i].y & 0xFFFFu);
+        v[i][7] = (int32_t)(raw[i].y >> 16);
+    }
+    const int32_t alpha = (int32_t)P.db_alpha, beta = alpha / 2 + 2, clip = alpha >> 1;
+    const int32_t maxv = (int32_t)P.maxv;
+    if (c >= 1 && (c + 1) * G <= w) {
+#pragma unroll
+        for (int32_t i = 0; i < G; ++i) {
+            if (y0 + i >= 0 && y0 + i < h) {
+                db_edge(v[i][HG - 2], v[i][HG - 1], v[i][HG], v[i][HG + 1], alpha, beta, clip, maxv);
+            }
+        }
+    }
+    if (r >= 1 && (r + 1) * G <= h) {
+#pragma unroll
+        for (int32_t j = 0; j < G; ++j) {
+            if (x0 + j >= 0 && x0 + j < w) {
+                db_edge(v[HG - 2][j], v[HG - 1][j], v[HG][j], v[HG + 1][j], alpha, beta, clip, maxv);
+            }
+        }
+    }
+#pragma unroll
+    for (int32_t i = 0; i < G; ++i) {
+        const int32_t y = y0 + i;
+        const uint32_t lo0 = (uint32_t)v[i][4] | (uint32_t)v[i][5] << 16;
+        const uint32_t lo1 = (uint32_t)v[i][6] | (uint32_t)v[i][7] << 16;
+        const uint32_t hi0 = db_from_above((uint32_t)v[i][0] | (uint32_t)v[i][1] << 16);
+        const uint32_t hi1 = db_from_above((uint32_t)v[i][2] | (uint32_t)v[i][3] << 16);
+        if (y < 0 || y >= h || !cin) { continue; }
+        gbyte* const row = base + (uint32_t)y * pitch + cb;
+        if (lane >= 1 && lane <= 62) { *(__attribute__((address_space(1))) db_u32x4*)row = db_u32x4{lo0, lo1, hi0, hi1}; }
+        else if (lane == 0) { *(__attribute__((address_space(1))) db_u32x2*)(row + 8) = db_u32x2{hi0, hi1}; }
+        else { *(__attribute__((address_space(1))) db_u32x2*)row = db_u32x2{lo0, lo1}; }
+    }
+}
+
+// grid: x = group of DB_WAVES runs of DB_TX - 1 tiles along a tile row
+// (DB_TX tiles a run on the per-thread tile path, whose extra groups exit),
+// y = tile row, z = plane of the batch
 template <uint32_t G>
-__global__ __launch_bounds__(DB_TX * DB_TY) void k_deblock(const tcg_dplane* __restrict__ planes)
+__global__ __launch_bounds__(DB_TX * DB_WAVES) void k_deblock(const tcg_dplane* __restrict__ planes)
 {
     const tcg_dplane P = planes[blockIdx.z];
     if (!P.deblock || (P.db_unit ? P.db_unit : 8u) != G) { return; }
-    const int32_t k = (int32_t)(blockIdx.x * DB_TX + (threadIdx.x % DB_TX));
-    const int32_t r = (int32_t)(blockIdx.y * DB_TY + (threadIdx.x / DB_TX));
-    if (k > (int32_t)P.vis_w / (int32_t)G || r > (int32_t)P.vis_h / (int32_t)G) { return; }
+    const int32_t lane = (int32_t)(threadIdx.x % DB_TX);
+    const int32_t run = (int32_t)(blockIdx.x * DB_WAVES + threadIdx.x / DB_TX);
+    const int32_t r = (int32_t)blockIdx.y;
+    if (r > (int32_t)P.vis_h / (int32_t)G) { return; }
+    const uint64_t align = P.dst | ((uint64_t)P.stride * 2u);
+    if (G == 8u && (align & 15u) == 0u) {
+        const int32_t t0 = run * (DB_TX - 1);
+        if (t0 > (int32_t)P.vis_w / 8) { return; }   // whole wave
+        deblock_chunks(P, t0, r, lane);
+        return;
+    }
+    const int32_t k = run * DB_TX + lane;
+    if (k > (int32_t)P.vis_w / (int32_t)G) { return; }
     // vector access needs the tile's half-rows (g/2 samples) aligned: the
     // plane's base and pitch at g bytes (w and h are multiples of g)
-    const bool vec = ((P.dst | ((uint64_t)P.stride * 2u)) & (uint64_t)(G - 1u)) == 0u;
-    if (vec) { deblock_tile<G, true>(P, k, r); }
+    if ((align & (uint64_t)(G - 1u)) == 0u) { deblock_tile<G, true>(P, k, r); }
     else { deblock_tile<G, false>(P, k, r); }
 }
 
@@ -878,7 +978,7 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         tcg_mark(c, 2);
         if (max_db_tiles > 0u) {
             hipLaunchKernelGGL(k_deblock<8>, dim3(max_db_tiles >> 16, max_db_tiles & 0xFFFFu, nplanes),
-                               dim3(DB_TX * DB_TY), 0, st, planes_dev);
+                               dim3(DB_TX * DB_WAVES), 0, st, planes_dev);
         }
         tcg_mark(c, 3);
         unsigned long long* pairs = (info->kinds & TCG_HAS_RANS2) ? tcg_pairs_counter(c) : nullptr;
@@ -915,7 +1015,7 @@ extern "C" int32_t tcg_decode_sample(tcg_ctx* c, const tcg_dplane* src_dev, cons
     hipLaunchKernelGGL(k_sample, dim3(gx, nplanes), dim3(256), 0, c->stream, src_dev, dst_dev);
     if (max_db_tiles > 0u) {
         hipLaunchKernelGGL(k_deblock<4>, dim3(max_db_tiles >> 16, max_db_tiles & 0xFFFFu, nplanes),
-                           dim3(DB_TX * DB_TY), 0, c->stream, dst_dev);
+                           dim3(DB_TX * DB_WAVES), 0, c->stream, dst_dev);
     }
     return tcg_check("target-size sample launch");
 }

@@ -271,15 +271,20 @@ typedef struct tcg_batch_info {
 uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols);
 
 
-/* Deblocking runs one thread per g x g tile centred on a block corner, in
- * workgroups of TCG_DB_TX tiles along a tile row x TCG_DB_TY tile rows. The
- * grid extent for a w x h plane, packed as (x groups << 16 | y groups); a
- * batch launches the component-wise maximum over its planes. */
+/* Deblocking runs one thread per g x g tile centred on a block corner. A
+ * workgroup is TCG_DB_WAVES waves side by side along one tile row; a wave
+ * covers TCG_DB_TX - 1 consecutive tiles of the row (its first lane only
+ * supplies half of the first tile) or, where the plane's alignment rules out
+ * 16-byte row chunks, TCG_DB_TX tiles. Neighbouring waves of a row share the
+ * cache line at their boundary, which keeps it in one L2. The grid extent for
+ * a w x h plane, packed as (x groups << 16 | tile rows); a batch launches the
+ * component-wise maximum over its planes. */
 #define TCG_DB_TX 64u
-#define TCG_DB_TY 4u
+#define TCG_DB_WAVES 8u
 static inline uint32_t tcg_db_grid(uint32_t w, uint32_t h, uint32_t g)
 {
-    const uint32_t gx = (w / g + 1u + TCG_DB_TX - 1u) / TCG_DB_TX, gy = (h / g + 1u + TCG_DB_TY - 1u) / TCG_DB_TY;
+    const uint32_t per = TCG_DB_WAVES * (TCG_DB_TX - 1u);
+    const uint32_t gx = (w / g + 1u + per - 1u) / per, gy = h / g + 1u;
     return gx << 16 | gy;
 }
 static inline uint32_t tcg_db_grid_max(uint32_t a, uint32_t b)
