@@ -28,7 +28,12 @@ HIP_SRC = ["tc_runtime.hip", "tc_encode.hip", "tc_decode.hip", "tc_alpha.hip", "
 CFLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wextra",
           "-Werror=implicit-function-declaration",
           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "host")]
-HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+# -falign-loops=32: loop heads on 32-byte boundaries. The serial entropy
+# chains are latency-bound loops of a few hundred bytes of code; without the
+# alignment their speed moves by a few percent with unrelated code changes
+# (measured on the 8-frame 4K batch: rANS2 decode 39.4 ms unaligned, 37.9 ms
+# aligned).
+HIPFLAGS = ["--offload-arch=" + ARCH, "-O3", "-falign-loops=32", "-std=c++17", "-fPIC", "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(CSRC, "kernels")]
 

@@ -678,14 +678,18 @@ int32_t tc_qp_eff(const tc_fhdr* fh, const topos_frame_config* cfg, uint32_t p)
     return qe > ceil ? ceil : qe;
 }
 
+static const uint32_t tc_tw[64] = TC_TW_INIT;
+
 /* Q = max(1, (qm * qp_scale + 128) >> 8), half = Q >> 1, dead zone Q/4 on AC,
- * coefficient clamp 2^25 (scaled up for bit depth > 12) */
+ * coefficient clamp 2^25 (scaled up for bit depth > 12); qwt = Q W 2^-32,
+ * transposed, is exact in f64 (Q W < 2^40, and the scale is a power of two) */
 void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bd, uint32_t tbl)
 {
     for (int i = 0; i < 64; ++i) {
         uint32_t Q = (uint32_t)(((uint64_t)qm[i] * tc_qp_scale_tab[tbl & 1u][qp] + 128u) >> 8);
         if (Q == 0u) { Q = 1u; }
         q->Q[i] = Q;
+        q->qwt[(i & 7) * 8 + (i >> 3)] = (double)((uint64_t)Q * tc_tw[i]) * 0x1p-32;
         q->half[i] = Q >> 1;
         q->dz[i] = i == 0 ? 0u : Q / 4u;
         tc_qdiv32_consts(Q, &q->qmul[i], &q->qsh[i]);
@@ -695,7 +699,11 @@ void tc_qctx_build(tcg_qctx* q, const uint16_t* qm, uint32_t qp, uint32_t bd, ui
     q->maxv = (int32_t)((1u << bd) - 1u);
     q->fclamp = bd <= 12u ? (1 << 25)
                           : (int32_t)((((int64_t)1 << 25) * (int64_t)((1u << (bd - 1u)) - 1u)) / 2047);
-    for (int i = 0; i < 64; ++i) { q->qlim[i] = (uint32_t)q->fclamp / q->Q[i]; }
+    q->qlim_min = 0xFFFFFFFFu;
+    for (int i = 0; i < 64; ++i) {
+        q->qlim[i] = (uint32_t)q->fclamp / q->Q[i];
+        if (q->qlim[i] < q->qlim_min) { q->qlim_min = q->qlim[i]; }
+    }
 }
 
 /* alpha = clamp(2 * max(1, round(scale(qp) * 3 / 173056)), 3, 48)

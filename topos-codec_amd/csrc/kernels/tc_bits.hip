@@ -470,7 +470,8 @@ __device__ static int32_t bits_decode_body(BitRd& D, const tcg_dslice& S, uint32
     const uint32_t dc_max = 0x08000000u, lv_max = 0x04000000u;
     const int64_t dlim = (int64_t)1 << 25;
     const uint32_t myscan = tc_zigzag_pos(lane);
-    int32_t* out = qbuf + (size_t)S.blk0 * 64u + lane;
+    // decoded blocks are stored column-major: natural index 8 r + c at 8 c + r
+    int32_t* out = qbuf + (size_t)S.blk0 * 64u + ((lane & 7u) << 3 | lane >> 3);
     int32_t left = 0;
     for (uint32_t by = 0; by < S.h; ++by) {
         for (uint32_t bx = 0; bx < cols; ++bx) {

@@ -219,9 +219,11 @@ __device__ static int32_t slice_decode_body(RansRd& D, const uint32_t (&cumv)[11
     const uint32_t dc_max = S.dc_max, lv_max = S.lv_max;
     const int32_t dlim = dc_max > 0x08000000u ? (1 << 27) : (1 << 25);
     const RowV run = ROW(RUN_ROW);
-    // lane = scan position; its coefficient's natural index in the block
-    // (blocks of the band are consecutive)
-    int32_t* out = qbuf + (size_t)S.blk0 * 64u + kZigzag[lane];
+    // lane = scan position; its coefficient's place in the block, stored
+    // column-major (natural index 8 r + c at 8 c + r; blocks of the band are
+    // consecutive)
+    const uint32_t nat = kZigzag[lane];
+    int32_t* out = qbuf + (size_t)S.blk0 * 64u + ((nat & 7u) << 3 | nat >> 3);
     uint32_t pdc_b = 0u;
     int32_t left = 0;
     for (uint32_t by = 0; by < S.h; ++by) {
@@ -405,28 +407,34 @@ __global__ __launch_bounds__(256) void k_slice_decode(const TcgConsts* __restric
 
 // ---------------------------------------------------------------------------
 // K6: dequantisation + inverse transform + level shift + clamp + clip-store.
-// Tile = 32 blocks of one block row, 256 threads. Pass 1: thread (j, u)
-// dequantises row u of block j and forms B[u][x] = sum_v G[u][v] M[v][x];
-// pass 2: thread (j, x) forms x[y][x] = sum_u M[u][y] B[u][x] and writes
-// column x of the block. For bit depth <= 12 every intermediate is an integer
-// below 2^53 in magnitude, so f64 FMA is exact; deeper samples use int64.
+// The entropy decoders store each block's coefficients column-major (natural
+// index 8 r + c at 8 c + r). Tile = 32 blocks of one block row, 256 threads;
+// each wave owns 8 blocks of the tile and a private LDS region, so no
+// workgroup barrier is needed. Pass 1: lane (j, c) dequantises column c of
+// block j and forms T[y][c] = sum_u M[u][y] G[u][c]; pass 2 (after a
+// wave-level LDS transpose): lane (j, y) forms x[y][x] = sum_v T[y][v] M[v][x]
+// and writes row y of the block as one 16-byte store. Every intermediate is
+// an exact integer (f64 below 2^53 for bit depth <= 12, int64 above), so the
+// order of the two passes does not change a sample. The dequantisation
+// weights Q W 2^-32 come precomputed, transposed, with the quantiser context.
 // ---------------------------------------------------------------------------
 #define IQ_BLOCKS TCG_IQ_BLOCKS
+#define IQ_WAVE_BLOCKS 8u
 
 // 8-point inverse transform (exact even/odd form of out[x] = sum_v in[v] M[v][x])
 __device__ static inline void idct8(const double* in, double* out) { tc_idct8_f64(in, out); }
 __device__ static inline void idct8(const int64_t* in, int64_t* out) { tc_idct8_i64(in, out); }
 
-// coefficients of row u of block j of a tile (thread t = 8 j + u), loaded
-// ahead of the transform
+// coefficients of column c of block j of a tile (thread t = 8 j + c: 32
+// contiguous bytes), loaded ahead of the transform
 struct IqRow { int4 r0, r1; };
 
 __device__ static inline void iq_load(const tcg_dplane& P, uint32_t brow, uint32_t bx0, uint32_t nb,
                                       const int32_t* __restrict__ qbuf, IqRow& q)
 {
-    const uint32_t j = threadIdx.x >> 3, u = threadIdx.x & 7u;
+    const uint32_t j = threadIdx.x >> 3, c = threadIdx.x & 7u;
     if (j < nb) {
-        const int32_t* src = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0 + j) * 64u + u * 8u;
+        const int32_t* src = qbuf + ((size_t)P.blk0 + (size_t)brow * P.bcols + bx0 + j) * 64u + c * 8u;
         q.r0 = *(const int4*)src;
         q.r1 = *(const int4*)(src + 4);
     }
@@ -457,172 +465,176 @@ __device__ static inline int32_t idct_round(int64_t v)
     return v >= 0 ? (int32_t)((v + ((int64_t)1 << 31)) >> 32) : -(int32_t)((-v + ((int64_t)1 << 31)) >> 32);
 }
 
-// LDS of one IDCT tile: the first pass's output (f64 or int64, 65-word pitch
-// per block)
-struct IqLds {
-    uint64_t B[IQ_BLOCKS * 65];
+// LDS of one wave: the first pass's output of its 8 blocks (f64 or int64,
+// 65-word pitch per block)
+struct IqWaveLds {
+    uint64_t B[IQ_WAVE_BLOCKS * 65];
 };
 
+// The 8 blocks bx0 .. bx0 + nb - 1 (nb <= 8) of block row brow, one wave.
+// vec16: the plane's base and row pitch are 16-byte aligned.
 template <bool WIDE>
-__device__ static void dequant_idct_tile(const tcg_dplane& P, const tcg_qctx* __restrict__ qc,
-                                         uint32_t brow, uint32_t bx0, uint32_t nb, bool conceal,
-                                         const IqRow& pre, IqLds& L, uint32_t* sQ, double* sQW, int32_t* sQlim,
-                                         const double* sW)
+__device__ static void dequant_idct_wave(const tcg_dplane& P, const tcg_qctx* __restrict__ qc,
+                                         uint32_t brow, uint32_t bx0, uint32_t nb, bool conceal, bool vec16,
+                                         const IqRow& pre, IqWaveLds& L, uint32_t lane)
 {
     typedef typename std::conditional<WIDE, int64_t, double>::type acc_t;
     acc_t* sB = (acc_t*)L.B;
-    const uint32_t t = threadIdx.x;
     const int32_t mid = qc->mid, maxv = qc->maxv;
     const int64_t fclamp = qc->fclamp;
-    if (t < 64u) {
-        const uint32_t Q = qc->Q[t];
-        sQ[t] = Q;
-        if (!WIDE) {
-            // Q W 2^-32 is exact (Q W < 2^40); |q| <= floor(fclamp / Q) keeps
-            // |q Q| within the clamp, so such a coefficient dequantises as
-            // q (Q W 2^-32) exactly
-            sQW[t] = (double)Q * sW[t];
-            sQlim[t] = (int32_t)qc->qlim[t];
-        }
-    }
-    uint16_t* out = (uint16_t*)P.dst;   // deblocking, if any, runs in place afterwards
     const uint32_t y0 = brow * 8u;
     if (conceal) {
-        // fill the visible part of the tile with the neutral value
-        const uint32_t xs = bx0 * 8u, xe = min(P.vis_w, (bx0 + nb) * 8u);
-        for (uint32_t i = t; i < 8u * 256u; i += 256u) {
-            const uint32_t y = y0 + i / 256u, x = xs + i % 256u;
-            if (y < P.vis_h && x < xe) { out[(size_t)y * P.stride + x] = (uint16_t)P.fill; }
+        // fill the visible part of the wave's blocks with the neutral value
+        const uint32_t x = bx0 * 8u + lane, xe = min(P.vis_w, (bx0 + nb) * 8u);
+        uint16_t* out = (uint16_t*)P.dst;
+        if (x < xe) {
+            for (uint32_t y = y0; y < min(y0 + 8u, P.vis_h); ++y) { out[(size_t)y * P.stride + x] = (uint16_t)P.fill; }
         }
         return;
     }
-    __syncthreads();
+    const uint32_t j = lane >> 3;
     {
-        const uint32_t j = t >> 3, u = t & 7u;
+        const uint32_t c = lane & 7u;   // pass 1: column c of block j
         const int4 r0 = pre.r0, r1 = pre.r1;
-        int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };
+        int32_t qv[8] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w };   // rows u = 0..7
         if (P.limit != 0u && P.limit < 63u) {
             // reduced decode: scan positions above the limit are dropped
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                if ((uint32_t)kZigzagInv[u * 8 + v] > P.limit) { qv[v] = 0; }
+            for (int u = 0; u < 8; ++u) {
+                if ((uint32_t)kZigzagInv[u * 8 + c] > P.limit) { qv[u] = 0; }
             }
         }
         if (j < nb) {
-            // lanes holding a coefficient above its clamp limit:
-            // |q| > lim  <=>  q + lim (unsigned) > 2 lim
-            uint64_t big = 0ull;
-            if (!WIDE) {
-#pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const uint32_t lim = (uint32_t)sQlim[u * 8 + v];
-                    big |= __ballot((uint32_t)qv[v] + lim > 2u * lim);
-                }
-            }
             acc_t G[8];
             if (WIDE) {
 #pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    int64_t d = (int64_t)qv[v] * (int64_t)sQ[u * 8 + v];
+                for (int u = 0; u < 8; ++u) {
+                    int64_t d = (int64_t)qv[u] * (int64_t)qc->Q[u * 8 + c];
                     d = d > fclamp ? fclamp : (d < -fclamp ? -fclamp : d);
                     d = d > (1 << 30) ? (1 << 30) : (d < -(1 << 30) ? -(1 << 30) : d);
-                    G[v] = (acc_t)d * (acc_t)kTW[u * 8 + v];
+                    G[u] = (acc_t)d * (acc_t)kTW[u * 8 + c];
                 }
-            } else if (__builtin_expect(big == 0ull, 1)) {
-#pragma unroll
-                for (int v = 0; v < 8; ++v) { G[v] = (acc_t)((double)qv[v] * sQW[u * 8 + v]); }
             } else {
-                // |q| < 2^27 and Q < 2^26, so q Q is an integer below 2^53 and
-                // exact in f64; the clamp bounds it by 2^25 and the weight by
-                // 2^14, so the product with the (2^-32-scaled) weight is exact
-                // as well
-                const double fc = (double)fclamp;
+                // lanes holding a coefficient that may reach the clamp: |q|
+                // above the smallest per-position limit floor(fclamp / Q)
+                // (accepted levels are below 2^27, so |q| does not wrap)
+                uint32_t amax = 0u;
 #pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    double d = (double)qv[v] * (double)sQ[u * 8 + v];
-                    d = fmin(fmax(d, -fc), fc);
-                    G[v] = (acc_t)(d * sW[u * 8 + v]);
+                for (int u = 0; u < 8; ++u) { amax = max(amax, (uint32_t)abs(qv[u])); }
+                if (__builtin_expect(__ballot(amax > qc->qlim_min) == 0ull, 1)) {
+                    // |q| <= floor(fclamp / Q) keeps |q Q| within the clamp,
+                    // so the coefficient dequantises as q (Q W 2^-32) exactly
+                    const double2* qw = (const double2*)(qc->qwt + c * 8u);
+#pragma unroll
+                    for (int u = 0; u < 8; u += 2) {
+                        const double2 w2 = qw[u / 2];
+                        G[u] = (acc_t)((double)qv[u] * w2.x);
+                        G[u + 1] = (acc_t)((double)qv[u + 1] * w2.y);
+                    }
+                } else {
+                    // |q| < 2^27 and Q < 2^26, so q Q is an integer below 2^53
+                    // and exact in f64; the clamp bounds it by 2^25 and the
+                    // weight by 2^14, so the product with the (2^-32-scaled)
+                    // weight is exact as well
+                    const double fc = (double)fclamp;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        double d = (double)qv[u] * (double)qc->Q[u * 8 + c];
+                        d = fmin(fmax(d, -fc), fc);
+                        G[u] = (acc_t)(d * ((double)kTW[u * 8 + c] * 0x1p-32));
+                    }
                 }
             }
-            acc_t Bv[8];
-            idct8(G, Bv);
+            acc_t Tc[8];
+            idct8(G, Tc);
 #pragma unroll
-            for (int x = 0; x < 8; ++x) { sB[j * 65 + u * 8 + x] = Bv[x]; }
+            for (int y = 0; y < 8; ++y) { sB[j * 65 + y * 8 + c] = Tc[y]; }
         }
     }
-    __syncthreads();
-    int32_t col[8];
-    const uint32_t j = t >> 3, x = t & 7u;
+    wave_sync();   // the wave's own LDS region: its lanes exchange columns for rows
     if (j < nb) {
-        acc_t B[8];
+        const uint32_t y = lane & 7u;   // pass 2: row y of block j
+        acc_t Tr[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) { B[u] = sB[j * 65 + u * 8 + x]; }
+        for (int v = 0; v < 8; ++v) { Tr[v] = sB[j * 65 + y * 8 + v]; }
         acc_t Xv[8];
-        idct8(B, Xv);
+        idct8(Tr, Xv);
+        uint32_t px[4];
 #pragma unroll
-        for (int y = 0; y < 8; ++y) { col[y] = idct_clamp(idct_round(Xv[y]), mid, maxv); }
-        // thread (j, x) holds column x of block j: per row a wave's 64 lanes
-        // store 128 contiguous bytes (8 blocks x 8 columns)
-        const uint32_t px = (bx0 + j) * 8u + x;
-        if (px < P.vis_w) {
+        for (int k = 0; k < 4; ++k) {
+            px[k] = (uint32_t)idct_clamp(idct_round(Xv[2 * k]), mid, maxv) |
+                    (uint32_t)idct_clamp(idct_round(Xv[2 * k + 1]), mid, maxv) << 16;
+        }
+        const uint32_t x0 = (bx0 + j) * 8u;
+        if (y0 + y < P.vis_h) {
             __attribute__((address_space(1))) uint16_t* dst =
-                (__attribute__((address_space(1))) uint16_t*)(uintptr_t)P.dst + (size_t)y0 * P.stride + px;
+                (__attribute__((address_space(1))) uint16_t*)(uintptr_t)P.dst + (size_t)(y0 + y) * P.stride + x0;
+            if (vec16 && x0 + 8u <= P.vis_w) {
+                typedef uint32_t iq_u32x4 __attribute__((ext_vector_type(4)));
+                *(__attribute__((address_space(1))) iq_u32x4*)dst = iq_u32x4{ px[0], px[1], px[2], px[3] };
+            } else {
 #pragma unroll
-            for (int y = 0; y < 8; ++y) {
-                if (y0 + (uint32_t)y < P.vis_h) { dst[(size_t)y * P.stride] = (uint16_t)col[y]; }
+                for (int x = 0; x < 8; ++x) {
+                    if (x0 + (uint32_t)x < P.vis_w) { dst[x] = (uint16_t)(px[x >> 1] >> (16 * (x & 1))); }
+                }
             }
         }
     }
+    wave_sync();   // pass-2 reads done before the region is written again
 }
 
 // grid: x = group of TPW tiles (32 blocks of a block row each) within the
-// plane, y = plane of the batch. The coefficients of all TPW tiles are
-// loaded first (TPW x 8 KiB in flight per workgroup); each tile's row record
-// (quantiser context, slice) then selects its context and concealment.
+// plane, y = plane of the batch. Every tile's row record (quantiser context,
+// slice) and coefficients are loaded first (TPW x 8 KiB in flight per
+// workgroup); then each wave transforms its 8 blocks of each tile on its own.
+// At most 80 VGPRs (six waves per SIMD, no spills): the wave count, not the
+// registers the compiler would otherwise keep live, sets how much of the HBM
+// latency the kernel hides.
 template <uint32_t TPW>
-__global__ __launch_bounds__(256) void k_dequant_idct(const tcg_qctx* __restrict__ qctxs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dequant_idct(const tcg_qctx* __restrict__ qctxs,
                                                        const tcg_dplane* __restrict__ planes,
                                                        const tcg_rowrec* __restrict__ rowrec,
                                                        uint32_t max_rows,
                                                        const int32_t* __restrict__ status,
                                                        const int32_t* __restrict__ qbuf)
 {
-    __shared__ IqLds L;
-    __shared__ uint32_t sQ[64];
-    __shared__ double sW[64], sQW[64];
-    __shared__ int32_t sQlim[64];
+    __shared__ IqWaveLds L[256u / 64u];
     const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (P.kind != 0u) { return; }
-    // weights scaled by 2^-32 (exact): the f64 path then ends in the value to round
-    if (threadIdx.x < 64u) { sW[threadIdx.x] = (double)kTW[threadIdx.x] * 0x1p-32; }  // read after each tile's barrier
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t tiles_per_row = (P.bcols + IQ_BLOCKS - 1u) / IQ_BLOCKS;
     const uint32_t ntiles = P.brows * tiles_per_row;
+    if (blockIdx.x * TPW >= ntiles) { return; }
+    const bool vec16 = ((P.dst | ((uint64_t)P.stride * 2u)) & 15u) == 0u;
+    uint32_t brow[TPW], bx0[TPW];
+    tcg_rowrec rr[TPW];
+#pragma unroll
+    for (uint32_t k = 0; k < TPW; ++k) {
+        const uint32_t local = min(blockIdx.x * TPW + k, ntiles - 1u);
+        brow[k] = tc_qdiv32(local, P.tile_m, P.tile_l);
+        bx0[k] = (local - brow[k] * tiles_per_row) * IQ_BLOCKS;
+        rr[k] = rowrec[(size_t)pi * max_rows + brow[k]];
+    }
     IqRow pre[TPW];
 #pragma unroll
     for (uint32_t k = 0; k < TPW; ++k) {
-        const uint32_t local = blockIdx.x * TPW + k;
-        if (local < ntiles) {
-            const uint32_t brow = tc_qdiv32(local, P.tile_m, P.tile_l);
-            const uint32_t bx0 = (local - brow * tiles_per_row) * IQ_BLOCKS;
-            iq_load(P, brow, bx0, min((uint32_t)IQ_BLOCKS, P.bcols - bx0), qbuf, pre[k]);
+        if (blockIdx.x * TPW + k < ntiles) {
+            iq_load(P, brow[k], bx0[k], min((uint32_t)IQ_BLOCKS, P.bcols - bx0[k]), qbuf, pre[k]);
         }
     }
 #pragma unroll
     for (uint32_t k = 0; k < TPW; ++k) {
-        const uint32_t local = blockIdx.x * TPW + k;
-        if (local >= ntiles) { break; }
-        const uint32_t brow = tc_qdiv32(local, P.tile_m, P.tile_l);
-        const uint32_t bx0 = (local - brow * tiles_per_row) * IQ_BLOCKS;
-        const uint32_t nb = min((uint32_t)IQ_BLOCKS, P.bcols - bx0);
-        const tcg_rowrec rr = rowrec[(size_t)pi * max_rows + brow];
-        const tcg_qctx* qc = &qctxs[rr.qctx];
-        const bool conceal = status[rr.slice] != 0;
-        if (k > 0u) { __syncthreads(); }  // the previous tile is done with sQ and the tile LDS
+        if (blockIdx.x * TPW + k >= ntiles) { break; }
+        const uint32_t wb0 = bx0[k] + wave * IQ_WAVE_BLOCKS;   // the wave's first block
+        if (wb0 >= P.bcols) { continue; }
+        const uint32_t nb = min(IQ_WAVE_BLOCKS, P.bcols - wb0);
+        const tcg_qctx* qc = &qctxs[rr[k].qctx];
+        const bool conceal = status[rr[k].slice] != 0;
         if (qc->bit_depth > 12u) {
-            dequant_idct_tile<true>(P, qc, brow, bx0, nb, conceal, pre[k], L, sQ, sQW, sQlim, sW);
+            dequant_idct_wave<true>(P, qc, brow[k], wb0, nb, conceal, vec16, pre[k], L[wave], lane);
         } else {
-            dequant_idct_tile<false>(P, qc, brow, bx0, nb, conceal, pre[k], L, sQ, sQW, sQlim, sW);
+            dequant_idct_wave<false>(P, qc, brow[k], wb0, nb, conceal, vec16, pre[k], L[wave], lane);
         }
     }
 }

@@ -18,12 +18,7 @@ static __device__ const int8_t kTM[64] = {
     5, -12, 12, -5, -5, 12, -12, 5,   1, -7, 11, -13, 13, -11, 7, -1,
 };
 static __device__ const uint32_t kTE[8] = { 1352, 680, 676, 740, 648, 740, 676, 680 };
-static __device__ const uint32_t kTW[64] = {
-    2350, 4672, 4699, 4293, 4902, 4293, 4699, 4672,  4672, 9288, 9343, 8535, 9747, 8535, 9343, 9288,
-    4699, 9343, 9399, 8586, 9805, 8586, 9399, 9343,  4293, 8535, 8586, 7843, 8957, 7843, 8586, 8535,
-    4902, 9747, 9805, 8957, 10228, 8957, 9805, 9747, 4293, 8535, 8586, 7843, 8957, 7843, 8586, 8535,
-    4699, 9343, 9399, 8586, 9805, 8586, 9399, 9343,  4672, 9288, 9343, 8535, 9747, 8535, 9343, 9288,
-};
+static __device__ const uint32_t kTW[64] = TC_TW_INIT;
 static __device__ const uint8_t kZigzag[64] = {
     0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5,
     12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6, 7, 14, 21, 28,

@@ -295,4 +295,15 @@ TC_HD int32_t tc_dc_predict(uint32_t has_left, int32_t left, uint32_t has_top, i
     return 0;
 }
 
+/* Inverse-transform weights W[u][v] = round(2^32 / (E_u E_v)) of the codec,
+ * row-major (native/src/transform/transform_tables.h:10-36): a bitstream
+ * constant, shared by the device table and the host's dequantisation weights. */
+#define TC_TW_INIT                                                                                   \
+    {                                                                                                \
+        2350, 4672, 4699, 4293, 4902, 4293, 4699, 4672,  4672, 9288, 9343, 8535, 9747, 8535, 9343, 9288, \
+        4699, 9343, 9399, 8586, 9805, 8586, 9399, 9343,  4293, 8535, 8586, 7843, 8957, 7843, 8586, 8535, \
+        4902, 9747, 9805, 8957, 10228, 8957, 9805, 9747, 4293, 8535, 8586, 7843, 8957, 7843, 8586, 8535, \
+        4699, 9343, 9399, 8586, 9805, 8586, 9399, 9343,  4672, 9288, 9343, 8535, 9747, 8535, 9343, 9288, \
+    }
+
 #endif

@@ -55,12 +55,16 @@ typedef struct tcg_vlc_books {
 #define TCG_IQ_BLOCKS 32u
 
 typedef struct tcg_qctx {
+    double qwt[64];      /* the f64 dequantisation weight Q W 2^-32 of natural position 8 u + v,
+                            stored at 8 v + u (column-major, as the decoded coefficients);
+                            exact: Q W < 2^40 */
     uint32_t qmul[64];   /* the quantiser divide floor(n / Q) = tc_qdiv32(n, qmul, qsh) */
     uint32_t qsh[64];
     uint32_t Q[64];
     uint32_t half[64];
     uint32_t dz[64];
     uint32_t qlim[64];   /* floor(fclamp / Q): larger |q| dequantise to the clamp */
+    uint32_t qlim_min;   /* min over the 64 positions of qlim: |q| <= qlim_min never clamps */
     int32_t fclamp;
     uint32_t bit_depth;
     int32_t mid;
