@@ -509,7 +509,7 @@ def mov_bench(pkg, abi, codec, host_pkts, cfg, dec0, reps=5):
             "equal_to_resident_decode": bool(all(np.array_equal(a, b) for a, b in zip(got, dec0)))}
 
 
-def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
+def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds, gather_us):
     """The mixed serving load: 60 % decodes / 40 % encodes over 1080p LT, 4K HQ,
     4K 4444 XQ + alpha and RAW 4096x3072 requests (shares 30/40/20/10 %), max
     rate from a saturated run, then p50 / p99 under Poisson arrivals."""
@@ -517,7 +517,7 @@ def serve_mix_bench(pkg, abi, syn, codec, hbm_gib, max_batch, lanes, seconds):
     frames = [resident_frame(pkg, abi, syn, codec, kind, w, h, 200 + i, kw)
               for i, (_, kind, w, h, kw) in enumerate(SERVE_MIX)]
     shares = np.array([m[0] for m in SERVE_MIX])
-    srv = pkg.Server(codec, hbm_gib << 30, max_batch=max_batch, max_inflight=lanes)
+    srv = pkg.Server(codec, hbm_gib << 30, max_batch=max_batch, max_inflight=lanes, gather_us=gather_us)
 
     def submit(k, d):
         f = frames[k]
@@ -773,6 +773,8 @@ def main():
     ap.add_argument("--hbm-budget", type=int, default=24, help="serving loop HBM budget (GiB)")
     ap.add_argument("--max-batch", type=int, default=64, help="serving loop: most frames per launch sequence")
     ap.add_argument("--lanes", type=int, default=2, help="serving loop: launch sequences in flight")
+    ap.add_argument("--gather-us", type=int, default=200,
+                    help="serving loop: arrival coalescing gap in microseconds (0: launch at once)")
     ap.add_argument("--load-seconds", type=float, default=4.0, help="length of each Poisson load run")
     ap.add_argument("--no-tune", action="store_true", help="skip the entropy launch-shape autotuning")
     ap.add_argument("--no-configs", action="store_true",
@@ -892,7 +894,7 @@ def main():
             print(f"# autotune ({t['batch_frames']}-frame batch): {t}", file=sys.stderr, flush=True)
 
     # served-batched: the serving loop over requests on resident frames and packets
-    srv = pkg.Server(codec, a.hbm_budget << 30, max_batch=a.max_batch, max_inflight=a.lanes)
+    srv = pkg.Server(codec, a.hbm_budget << 30, max_batch=a.max_batch, max_inflight=a.lanes, gather_us=a.gather_us)
 
     def enc_req(i):
         return lambda: srv.submit_encode(cfg, dev_planes=in_addr[i])
@@ -1029,7 +1031,7 @@ def main():
                    "convert_bgra32_4k": convert_bench(pkg, abi, codec, orc, 2),
                    "convert_rgba64_4k": convert_bench(pkg, abi, codec, orc, 7),
                    "serving_mix": serve_mix_bench(pkg, abi, syn, codec, a.hbm_budget, a.max_batch, a.lanes,
-                                                  max(1.0, a.load_seconds / 2))}
+                                                  max(1.0, a.load_seconds / 2), a.gather_us)}
         print(f"# other configurations ({time.perf_counter() - t0:.1f}s): {configs}", file=sys.stderr, flush=True)
 
     cpu = None

@@ -144,6 +144,29 @@ def test_server_results_equal_alone_and_oracle(pkg, abi, codec, cases):
 
 
 @pytest.mark.gpu
+def test_server_gathers_bursts_into_equal_batches(pkg, abi, codec, cases):
+    """Arrival coalescing and equal batches: a burst submitted faster than
+    the coalescing gap is not started as a one-request batch; once arrivals
+    stop, nine waiting requests with max_batch 4 run as 3 + 3 + 3 (not
+    4 + 4 + 1). Every result equals the request alone."""
+    c = cases[4]
+    srv = pkg.Server(codec, 1 << 30, max_batch=4, max_inflight=1, gather_us=20000)
+    try:
+        for k in range(9):
+            st, _ = srv.submit_encode(c["cfg"], dev_planes=[b.addr for b in c["dev_in"]], user=k)
+            assert st == 0, codec.last_error()
+        comps = _drain(srv, 9)
+        assert sorted(comp.batch_frames for comp in comps) == [3] * 9
+        for comp in comps:
+            assert comp.status == 0
+            assert _download_result(codec, pkg, abi, comp, c["cfg"]) == c["pkt"]
+            assert srv.release(comp.id) == 0
+        assert srv.stats().batches == 3
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
 def test_server_admission_holds_budget(pkg, abi, codec, cases):
     # room for the output planes and staged packet of two 1280x720 decodes only
     c = cases[3]

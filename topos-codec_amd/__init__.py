@@ -890,10 +890,10 @@ class Server:
     arrive, poll completions, release results. Request buffers handed in as
     numpy arrays or bytes are kept alive here until the request is released."""
 
-    def __init__(self, codec, hbm_budget, max_batch=8, max_inflight=4):
+    def __init__(self, codec, hbm_budget, max_batch=8, max_inflight=4, gather_us=0):
         self._lib = codec._lib
         self._codec = codec
-        cfg = abi.TcxServerConfig(hbm_budget, max_batch, max_inflight)
+        cfg = abi.TcxServerConfig(hbm_budget, max_batch, max_inflight, gather_us)
         self._h = ctypes.c_void_p()
         st = self._lib.tcx_server_create(ctypes.byref(cfg), ctypes.byref(self._h))
         if st != 0:

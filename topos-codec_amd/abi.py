@@ -351,7 +351,7 @@ TCX_REQ_DECODE = 2
 
 class TcxServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch", ctypes.c_uint32),
-                ("max_inflight", ctypes.c_uint32)]
+                ("max_inflight", ctypes.c_uint32), ("gather_us", ctypes.c_uint32)]
 
 
 class TcxRequest(ctypes.Structure):

@@ -481,7 +481,7 @@ int32_t tc_engine_encode_submit(tc_ectx* c, tc_enc_req* reqs, uint32_t n)
                 info.max_cols = fh->bcols[p] > info.max_cols ? fh->bcols[p] : info.max_cols;
                 tc_qctx_build(&hq[iq], tc_qm_tab[fh->qmatrix_id][tc_is_chroma(fh, p) ? 1 : 0],
                               (uint32_t)P->qp_eff[p], fh->bit_depth, tbl);
-                tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + TCG_FQ_BLOCKS - 1u) / TCG_FQ_BLOCKS);
             }
             iq++;
             for (uint32_t y0 = 0; y0 < fh->brows[p]; y0 += P->slice_rows) {
@@ -654,7 +654,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
             nblocks += (uint32_t)fh->bcols[p] * fh->brows[p];
             maxrows = fh->brows[p] > maxrows ? fh->brows[p] : maxrows;
             if (!tc_is_alpha(fh, p)) {
-                const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                const uint32_t pt = (uint32_t)fh->brows[p] * ((fh->bcols[p] + TCG_IQ_BLOCKS - 1u) / TCG_IQ_BLOCKS);
                 ntiles = pt > ntiles ? pt : ntiles;
             }
             const int dropped = r->skip_alpha && tc_is_alpha(fh, p);
@@ -778,7 +778,7 @@ int32_t tc_engine_decode_submit(tc_ectx* c, tc_dec_req* reqs, uint32_t n)
                 info.max_cols = fh->bcols[p] > info.max_cols ? fh->bcols[p] : info.max_cols;
                 D->fill = 1u << (fh->bit_depth - 1u);
                 D->maxv = (1u << fh->bit_depth) - 1u;
-                tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + 31u) / 32u);
+                tile += (uint32_t)fh->brows[p] * ((fh->bcols[p] + TCG_IQ_BLOCKS - 1u) / TCG_IQ_BLOCKS);
                 if (db && p < 3u && !scaled && vw % 8u == 0u && vh % 8u == 0u && vw >= 16u && vh >= 16u &&
                     fh->qp_base <= 95u && fh->bit_depth >= 8u && fh->bit_depth <= 16u) {
                     D->deblock = 1u;

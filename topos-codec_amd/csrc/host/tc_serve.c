@@ -7,8 +7,15 @@
  *     encode = worst-case packet + staged host inputs, decode = output
  *     planes + staged packet;
  *  2. starts a launch sequence on every free lane: the direction whose oldest
- *     admitted request arrived first, up to max_batch of its admitted
- *     requests in arrival order, mixed sizes and profiles in one batch;
+ *     admitted request arrived first, its admitted requests in arrival order,
+ *     mixed sizes and profiles in one batch. A queue longer than max_batch is
+ *     cut into equal batches (160 waiting with max_batch 64: 54, 53, 53), so
+ *     the last one is not a remainder that holds a lane for a whole serial
+ *     entropy chain with a fraction of the device busy. While requests keep
+ *     arriving less than gather_us apart and the batch is not full, the
+ *     launch waits for them (at most 20 x gather_us after its oldest request
+ *     arrived): a burst then forms full batches instead of a first batch of
+ *     one frame, and isolated arrivals are not delayed;
  *  3. polls the running lanes without blocking; a finished lane's requests
  *     complete, their reservations shrink to what the result holds (packet
  *     bytes / output planes) and the staged inputs return to the budget.
@@ -75,6 +82,9 @@ struct tcx_server {
     pthread_t thr;
     int started, stop;
     uint32_t max_batch, max_inflight;
+    int64_t gather_ns;          /* arrival coalescing gap (0: off) */
+    int64_t last_submit;        /* time of the latest submission */
+    int64_t wake;               /* a deferred launch is due by then (0: none) */
     uint64_t base;              /* the budget's device allocation */
     tc_arena arena;
     squeue q_wait, q_enc, q_dec, q_done;
@@ -330,10 +340,22 @@ static int launch_locked(tcx_server* s)
         if (s->q_enc.head == NULL) { src = &s->q_dec; }
         else if (s->q_dec.head == NULL) { src = &s->q_enc; }
         else { src = s->q_enc.head->t_submit <= s->q_dec.head->t_submit ? &s->q_enc : &s->q_dec; }
+        const int64_t now = tcx_now_ns();
+        if (s->gather_ns > 0 && !s->stop && src->n < s->max_batch && now - s->last_submit < s->gather_ns &&
+            now - src->head->t_submit < 20 * s->gather_ns) {
+            /* arrivals are still streaming in: wait for them */
+            const int64_t due_gap = s->last_submit + s->gather_ns, due_max = src->head->t_submit + 20 * s->gather_ns;
+            s->wake = due_gap < due_max ? due_gap : due_max;
+            if (arc == TC_OK) { tc_engine_release(c); }
+            break;
+        }
+        s->wake = 0;
+        /* equal batches: ceil(n / ceil(n / max_batch)) */
+        const uint32_t nbat = (src->n + s->max_batch - 1u) / s->max_batch;
+        const uint32_t take = (src->n + nbat - 1u) / nbat;
         L->kind = src == &s->q_enc ? TCX_REQ_ENCODE : TCX_REQ_DECODE;
         L->n = 0;
-        const int64_t now = tcx_now_ns();
-        while (L->n < s->max_batch && src->head != NULL) {
+        while (L->n < take && src->head != NULL) {
             sreq* q = q_pop(src);
             q->state = RQ_RUNNING;
             q->t_launch = now;
@@ -413,10 +435,15 @@ static void* srv_main(void* arg)
             }
         }
         if (!progress) {
-            if (s->nbusy > 0u) {
-                /* lanes running: poll again shortly */
+            if (s->nbusy > 0u || s->wake != 0) {
+                /* lanes running or a launch deferred: look again shortly */
+                int64_t ns = 20000;
+                if (s->nbusy == 0u) {
+                    const int64_t left = s->wake - tcx_now_ns();
+                    ns = left < 1000 ? 1000 : (left < 50000 ? left : 50000);
+                }
                 pthread_mutex_unlock(&s->mu);
-                const struct timespec ts = { 0, 20000 };
+                const struct timespec ts = { 0, (long)ns };
                 nanosleep(&ts, NULL);
                 pthread_mutex_lock(&s->mu);
             } else {
@@ -449,6 +476,7 @@ int32_t tcx_server_create(const tcx_server_config* cfg, tcx_server** out)
     if (s == NULL) { return tc_fail(TC_ERR_OUT_OF_MEMORY, "server"); }
     s->max_batch = mb;
     s->max_inflight = mi;
+    s->gather_ns = (int64_t)cfg->gather_us * 1000;
     if (tc_arena_init(&s->arena, cfg->hbm_budget) != 0) { free(s); return tc_fail(TC_ERR_OUT_OF_MEMORY, "arena"); }
     if (tcg_malloc(&s->base, s->arena.size + SRV_SLACK) != 0) {
         tc_arena_fini(&s->arena);
@@ -524,6 +552,7 @@ int32_t tcx_server_submit(tcx_server* s, const tcx_request* req, uint64_t* id)
     q->live_next = s->live;
     s->live = q;
     q_push(&s->q_wait, q);
+    s->last_submit = tcx_now_ns();
     s->st.submitted++;
     if (id) { *id = q->id; }
     pthread_cond_signal(&s->cv_work);

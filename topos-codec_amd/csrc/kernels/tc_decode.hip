@@ -420,6 +420,7 @@ __global__ __launch_bounds__(256) void k_slice_decode(const TcgConsts* __restric
 // ---------------------------------------------------------------------------
 #define IQ_BLOCKS TCG_IQ_BLOCKS
 #define IQ_WAVE_BLOCKS 8u
+#define IQ_WAVES (IQ_BLOCKS / IQ_WAVE_BLOCKS)   // waves of a workgroup (one tile)
 
 // 8-point inverse transform (exact even/odd form of out[x] = sum_v in[v] M[v][x])
 __device__ static inline void idct8(const double* in, double* out) { tc_idct8_f64(in, out); }
@@ -591,14 +592,14 @@ __device__ static void dequant_idct_wave(const tcg_dplane& P, const tcg_qctx* __
 // registers the compiler would otherwise keep live, sets how much of the HBM
 // latency the kernel hides.
 template <uint32_t TPW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_dequant_idct(const tcg_qctx* __restrict__ qctxs,
+__global__ __launch_bounds__(64u * IQ_WAVES) __attribute__((amdgpu_waves_per_eu(6))) void k_dequant_idct(const tcg_qctx* __restrict__ qctxs,
                                                        const tcg_dplane* __restrict__ planes,
                                                        const tcg_rowrec* __restrict__ rowrec,
                                                        uint32_t max_rows,
                                                        const int32_t* __restrict__ status,
                                                        const int32_t* __restrict__ qbuf)
 {
-    __shared__ IqWaveLds L[256u / 64u];
+    __shared__ IqWaveLds L[IQ_WAVES];
     const uint32_t pi = blockIdx.y;
     const tcg_dplane P = planes[pi];
     if (P.kind != 0u) { return; }
@@ -650,7 +651,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // horizontal segment gives the result of the whole-plane vertical pass
 // followed by the whole-plane horizontal pass (deblock.c:163-212), and tiles
 // share no sample, so they run independently in place. One thread per tile,
-// a wave per run of consecutive tiles of a tile row. With g = 8 on a plane
+// one wave per tile row, taking the row in runs of consecutive tiles. With g = 8 on a plane
 // whose base and row pitch are 16-byte aligned the wave reads and writes each
 // row as aligned 16-byte chunks and passes half-tiles between neighbouring
 // lanes (deblock_chunks); otherwise each thread reads its tile's half-rows
@@ -681,7 +682,7 @@ __device__ static inline void db_edge(int32_t& p1, int32_t& p0, int32_t& q0, int
 }
 
 #define DB_TX TCG_DB_TX         // lanes of a wave (tiles of one tile row)
-#define DB_WAVES TCG_DB_WAVES   // waves of a workgroup, side by side along a tile row
+#define DB_WAVES TCG_DB_WAVES   // waves of a workgroup, along one tile row
 
 typedef __attribute__((address_space(1))) uint8_t gbyte;
 
@@ -888,31 +889,35 @@ __device__ static inline void deblock_chunks(const tcg_dplane& P, int32_t t0, in
     }
 }
 
-// grid: x = group of DB_WAVES runs of DB_TX - 1 tiles along a tile row
-// (DB_TX tiles a run on the per-thread tile path, whose extra groups exit),
-// y = tile row, z = plane of the batch
+// grid: y = tile row, z = plane of the batch; a workgroup of DB_WAVES waves
+// per tile row, wave w taking runs w, w + DB_WAVES, ... of DB_TX - 1 tiles
+// along the row (DB_TX on the per-thread tile path). Neighbouring runs share
+// the cache line at their boundary, which stays in the workgroup's L2.
 template <uint32_t G>
 __global__ __launch_bounds__(DB_TX * DB_WAVES) void k_deblock(const tcg_dplane* __restrict__ planes)
 {
     const tcg_dplane P = planes[blockIdx.z];
     if (!P.deblock || (P.db_unit ? P.db_unit : 8u) != G) { return; }
-    const int32_t lane = (int32_t)(threadIdx.x % DB_TX);
-    const int32_t run = (int32_t)(blockIdx.x * DB_WAVES + threadIdx.x / DB_TX);
+    const int32_t lane = (int32_t)(threadIdx.x % DB_TX), wave = (int32_t)(threadIdx.x / DB_TX);
     const int32_t r = (int32_t)blockIdx.y;
     if (r > (int32_t)P.vis_h / (int32_t)G) { return; }
+    const int32_t last = (int32_t)P.vis_w / (int32_t)G;   // tiles 0 .. last of the row
     const uint64_t align = P.dst | ((uint64_t)P.stride * 2u);
     if (G == 8u && (align & 15u) == 0u) {
-        const int32_t t0 = run * (DB_TX - 1);
-        if (t0 > (int32_t)P.vis_w / 8) { return; }   // whole wave
-        deblock_chunks(P, t0, r, lane);
+        for (int32_t t0 = wave * (DB_TX - 1); t0 <= last; t0 += DB_WAVES * (DB_TX - 1)) {
+            deblock_chunks(P, t0, r, lane);
+        }
         return;
     }
-    const int32_t k = run * DB_TX + lane;
-    if (k > (int32_t)P.vis_w / (int32_t)G) { return; }
     // vector access needs the tile's half-rows (g/2 samples) aligned: the
     // plane's base and pitch at g bytes (w and h are multiples of g)
-    if ((align & (uint64_t)(G - 1u)) == 0u) { deblock_tile<G, true>(P, k, r); }
-    else { deblock_tile<G, false>(P, k, r); }
+    const bool vec = (align & (uint64_t)(G - 1u)) == 0u;
+    for (int32_t k0 = wave * DB_TX; k0 <= last; k0 += DB_WAVES * DB_TX) {
+        const int32_t k = k0 + lane;
+        if (k > last) { break; }
+        if (vec) { deblock_tile<G, true>(P, k, r); }
+        else { deblock_tile<G, false>(P, k, r); }
+    }
 }
 
 // Target-size decode: target sample (tx, ty) of a plane takes the source
@@ -977,20 +982,20 @@ extern "C" int32_t tcg_decode(tcg_ctx* c, const tcg_qctx* qctx_dev, uint32_t npl
         tcg_mark(c, 1);
         if (max_plane_tiles > 0u) {
             if (iq_tpw == 4u) {
-                hipLaunchKernelGGL(k_dequant_idct<4>, dim3((max_plane_tiles + 3u) / 4u, nplanes), dim3(256), 0, st,
+                hipLaunchKernelGGL(k_dequant_idct<4>, dim3((max_plane_tiles + 3u) / 4u, nplanes), dim3(64u * IQ_WAVES), 0, st,
                                    qctx_dev, planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
             } else if (iq_tpw == 2u) {
-                hipLaunchKernelGGL(k_dequant_idct<2>, dim3((max_plane_tiles + 1u) / 2u, nplanes), dim3(256), 0, st,
+                hipLaunchKernelGGL(k_dequant_idct<2>, dim3((max_plane_tiles + 1u) / 2u, nplanes), dim3(64u * IQ_WAVES), 0, st,
                                    qctx_dev, planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
             } else {
-                hipLaunchKernelGGL(k_dequant_idct<1>, dim3(max_plane_tiles, nplanes), dim3(256), 0, st, qctx_dev,
+                hipLaunchKernelGGL(k_dequant_idct<1>, dim3(max_plane_tiles, nplanes), dim3(64u * IQ_WAVES), 0, st, qctx_dev,
                                    planes_dev, rowrec_dev, max_rows, slice_status_dev, (const int32_t*)qbuf);
             }
         }
         tcg_mark(c, 2);
         if (max_db_tiles > 0u) {
-            hipLaunchKernelGGL(k_deblock<8>, dim3(max_db_tiles >> 16, max_db_tiles & 0xFFFFu, nplanes),
-                               dim3(DB_TX * DB_WAVES), 0, st, planes_dev);
+            hipLaunchKernelGGL(k_deblock<8>, dim3(1, max_db_tiles & 0xFFFFu, nplanes), dim3(DB_TX * DB_WAVES), 0,
+                               st, planes_dev);
         }
         tcg_mark(c, 3);
         unsigned long long* pairs = (info->kinds & TCG_HAS_RANS2) ? tcg_pairs_counter(c) : nullptr;
@@ -1026,8 +1031,8 @@ extern "C" int32_t tcg_decode_sample(tcg_ctx* c, const tcg_dplane* src_dev, cons
     const uint32_t gx = min((max_px + 255u) / 256u, 8192u);
     hipLaunchKernelGGL(k_sample, dim3(gx, nplanes), dim3(256), 0, c->stream, src_dev, dst_dev);
     if (max_db_tiles > 0u) {
-        hipLaunchKernelGGL(k_deblock<4>, dim3(max_db_tiles >> 16, max_db_tiles & 0xFFFFu, nplanes),
-                           dim3(DB_TX * DB_WAVES), 0, c->stream, dst_dev);
+        hipLaunchKernelGGL(k_deblock<4>, dim3(1, max_db_tiles & 0xFFFFu, nplanes), dim3(DB_TX * DB_WAVES), 0,
+                           c->stream, dst_dev);
     }
     return tcg_check("target-size sample launch");
 }

@@ -51,8 +51,9 @@ typedef struct tcg_vlc_books {
 #ifndef TCG_FQ_BLOCKS
 #define TCG_FQ_BLOCKS 32u
 #endif
-/* decode transform tile: blocks of one block row per IDCT tile */
-#define TCG_IQ_BLOCKS 32u
+/* decode transform tile: blocks of one block row per IDCT tile (8: one wave,
+ * a one-wave workgroup per tile) */
+#define TCG_IQ_BLOCKS 8u
 
 typedef struct tcg_qctx {
     double qwt[64];      /* the f64 dequantisation weight Q W 2^-32 of natural position 8 u + v,
@@ -275,21 +276,21 @@ typedef struct tcg_batch_info {
 uint32_t tcg_entropy_capacity(int encode, uint32_t wpg, uint32_t max_cols);
 
 
-/* Deblocking runs one thread per g x g tile centred on a block corner. A
- * workgroup is TCG_DB_WAVES waves side by side along one tile row; a wave
- * covers TCG_DB_TX - 1 consecutive tiles of the row (its first lane only
- * supplies half of the first tile) or, where the plane's alignment rules out
- * 16-byte row chunks, TCG_DB_TX tiles. Neighbouring waves of a row share the
- * cache line at their boundary, which keeps it in one L2. The grid extent for
- * a w x h plane, packed as (x groups << 16 | tile rows); a batch launches the
- * component-wise maximum over its planes. */
+/* Deblocking runs one thread per g x g tile centred on a block corner, one
+ * workgroup of TCG_DB_WAVES waves per tile row: the waves take the row in runs
+ * of TCG_DB_TX - 1 consecutive tiles (a run's first lane only supplies half of
+ * its first tile) or, where the plane's alignment rules out 16-byte row
+ * chunks, runs of TCG_DB_TX tiles. The grid extent for a w x h plane, packed
+ * as (1 << 16 | tile rows); a batch launches the component-wise maximum over
+ * its planes. */
 #define TCG_DB_TX 64u
-#define TCG_DB_WAVES 8u
+#ifndef TCG_DB_WAVES
+#define TCG_DB_WAVES 4u
+#endif
 static inline uint32_t tcg_db_grid(uint32_t w, uint32_t h, uint32_t g)
 {
-    const uint32_t per = TCG_DB_WAVES * (TCG_DB_TX - 1u);
-    const uint32_t gx = (w / g + 1u + per - 1u) / per, gy = h / g + 1u;
-    return gx << 16 | gy;
+    (void)w;
+    return 1u << 16 | (h / g + 1u);
 }
 static inline uint32_t tcg_db_grid_max(uint32_t a, uint32_t b)
 {
