@@ -1064,9 +1064,11 @@ def main():
     for key, (kname, bps) in algo.items():
         ms = float(np.mean(kt[key]))
         ach = samples * bps / (ms * 1e-3) / 1e9
-        tb = traffic.get(key, {}).get("bytes")
+        tr = traffic.get(key, {})
+        tb = tr.get("bytes")
         kern_rl[key] = {"kernel": kname, "ms": round(ms, 4), "bytes_per_launch": samples * bps,
                         "achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tb,
+                        "traffic_fetch": tr.get("fetch_bytes"), "traffic_write": tr.get("write_bytes"),
                         "traffic_ratio": round(tb / (samples * bps), 4) if tb else None}
     for key, iss in ((counters or {}).get("transform_issue") or {}).items():
         kern_rl[key]["issue"] = iss
