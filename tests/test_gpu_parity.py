@@ -108,3 +108,49 @@ def test_dc_context_model_variants(pkg, abi, syn, oracle, codec, kind, w, h, see
     assert d_rc == o_rc == 0
     for a, b in zip(d_planes, o_planes):
         assert (a == b).all()
+
+
+def _extreme_plane(M, w, h, maxv, invert):
+    """Blocks whose samples are 0 or maxv in the sign pattern of one 2-D basis
+    function (u, v) of the transform matrix: the forward transform's sums and
+    products reach their worst case for that output frequency."""
+    import numpy as np
+    p = np.zeros((h, w), np.uint16)
+    k = 0
+    for by in range(h // 8):
+        for bx in range(w // 8):
+            u, v = k % 8, (k // 8) % 8
+            k += 1
+            pos = (M[u][:, None] * M[v][None, :]) >= 0
+            if invert ^ (k % 2 == 0):
+                pos = ~pos
+            p[by * 8:(by + 1) * 8, bx * 8:(bx + 1) * 8] = np.where(pos, maxv, 0)
+    return p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt,bd,qp", [("422", 10, 0), ("422", 10, 62), ("444", 12, 0), ("444", 12, 30)])
+def test_extreme_contrast_transforms(pkg, abi, oracle, codec, fmt, bd, qp):
+    """Worst-case transform ranges (the 24-bit forward-transform multiplies,
+    the exact f64 inverse with clamped coefficients): samples at 0 and the
+    largest value in the sign patterns of every basis function. GPU packet
+    equals the oracle's; the decode of it equals the oracle's decode."""
+    import ctypes
+    import numpy as np
+    M = np.ctypeslib.as_array((ctypes.c_int16 * 64).in_dll(oracle, "orc_M")).astype(np.int64).reshape(8, 8)
+    w, h = 256, 64
+    maxv = (1 << bd) - 1
+    pf = 0 if fmt == "422" else 1
+    dims = abi.plane_dims(w, h, pf, 0)
+    planes = [_extreme_plane(M, pw, ph, maxv, i == 1) for i, (pw, ph) in enumerate(dims)]
+    cfg = abi.make_config(w, h, qp, profile=3, pixel_format=pf, bit_depth=bd, slice_rows=2, entropy_mode=8)
+    orc_rc, orc_pkt = orc_encode(oracle, cfg, planes)
+    assert orc_rc == 0
+    st, pkt, _ = codec.encode_status(cfg, planes)
+    assert st == 0, codec.last_error()
+    assert pkt == orc_pkt
+    o_rc, o_planes, _, _ = orc_decode(oracle, orc_pkt, dims)
+    d_rc, d_planes, _ = codec.decode(orc_pkt, strict=False)
+    assert d_rc == o_rc == 0
+    for a, b in zip(d_planes, o_planes):
+        assert (a == b).all()

@@ -265,9 +265,10 @@ def _ref_round(v):
 def test_f64_inverse_transform_worst_case(oracle):
     """Dequantised coefficients at the clamp (|G| = f_clamp W, f_clamp = 2^25
     for bit depths <= 12) with the signs that maximise each output sample:
-    every f64 step of the kernel's dequant / IDCT / rounding stays an exact
-    integer below 2^53, so it equals the exact integer transform and the
-    reference rounding (the kernel's one-FMA rounding is checked too)."""
+    every f64 step of the kernel's dequant / IDCT / rounding (columns first,
+    then rows) stays an exact integer below 2^53, so it equals the exact
+    integer transform and the reference rounding (the kernel's one-FMA
+    rounding is checked too)."""
     import math
     M = np.ctypeslib.as_array((ctypes.c_int16 * 64).in_dll(oracle, "orc_M")).astype(np.int64).reshape(8, 8)
     W = np.ctypeslib.as_array((ctypes.c_uint32 * 64).in_dll(oracle, "orc_W")).astype(np.int64).reshape(8, 8)
@@ -278,16 +279,56 @@ def test_f64_inverse_transform_worst_case(oracle):
             su = np.sign(M[:, y]).astype(np.int64)
             sv = np.sign(M[:, x]).astype(np.int64)
             G = (su[:, None] * sv[None, :]) * fc * W          # exact integers (< 2^39)
-            # pass 1 (rows u): B[u][x'] = sum_v G[u][v] M[v][x']
-            B = [_idct8_f64([float(g) for g in G[u]]) for u in range(8)]
-            cols = [_idct8_f64([B[u][xx] for u in range(8)]) for xx in range(8)]
+            # the kernel's order: pass 1 (columns v): T[y'][v] = sum_u M[u][y'] G[u][v],
+            # pass 2 (rows y'): X[y'][x'] = sum_v T[y'][v] M[v][x']
+            Tc = [_idct8_f64([float(G[u][v]) for u in range(8)]) for v in range(8)]   # Tc[v][y']
+            B = [[Tc[v][yy] for v in range(8)] for yy in range(8)]                    # B[y'][v]
+            rows = [_idct8_f64(B[yy]) for yy in range(8)]                             # rows[y'][x']
             exact = M.T @ G @ M                                   # X[y][x] = sum_uv M[u][y] G[u][v] M[v][x]
             for yy in range(8):
                 for xx in range(8):
-                    v = cols[xx][yy]
+                    v = rows[yy][xx]
                     worst = max(worst, abs(v))
                     assert v == float(exact[yy, xx]) and abs(v) < 2.0 ** 53
                     r = int(math.floor(abs(v) * 2.0 ** -32 + 0.5))  # the kernel's FMA form (exact, unfused too)
                     assert (r if v >= 0 else -r) == _ref_round(int(exact[yy, xx]))
             assert all(abs(b) < 2.0 ** 53 for row in B for b in row)
     assert worst > 2.0 ** 45    # the test reaches the range the argument is about
+
+
+def _fdct8_multiplicands(v):
+    """The butterfly form of the forward 8-point transform (k_fdct_quant's
+    fdct8_i24, tc_fdct8_i32) over symbolic inputs: each value is its integer
+    coefficient vector over the 8 inputs. Returns (multiplicands, outputs)."""
+    s = [v[k] + v[7 - k] for k in range(4)]
+    d = [v[k] - v[7 - k] for k in range(4)]
+    e0, e1, f0, f1 = s[0] + s[3], s[1] + s[2], s[0] - s[3], s[1] - s[2]
+    mult = [e0 + e1, e0 - e1, f0, f1, d[0], d[1], d[2], d[3]]
+    out = [13 * (e0 + e1), 13 * d[0] + 11 * d[1] + 7 * d[2] + d[3], 12 * f0 + 5 * f1,
+           11 * d[0] - 4 * d[1] - 13 * d[2] - 8 * d[3], 9 * (e0 - e1), 8 * d[0] - 13 * d[1] + 4 * d[2] + 11 * d[3],
+           5 * f0 - 12 * f1, d[0] - 7 * d[1] + 11 * d[2] - 13 * d[3]]
+    return mult, out
+
+
+def test_fdct_24bit_multiplies_exact(oracle):
+    """k_fdct_quant multiplies with 24-bit operands (v_mul_i32_i24) for bit
+    depths <= 12: every multiplicand must stay below 2^23 in magnitude and
+    every product and output within 32 bits, for the worst-case inputs. Level-
+    shifted 12-bit samples lie in [-2048, 2047]; a linear form's largest
+    magnitude over that box is sum |coef| * 2048. The column pass's inputs are
+    the row pass's outputs of one frequency v, each bounded alike."""
+    eye = [np.eye(8, dtype=np.int64)[k] for k in range(8)]
+    mult, out = _fdct8_multiplicands(eye)
+    M = np.ctypeslib.as_array((ctypes.c_int16 * 64).in_dll(oracle, "orc_M")).astype(np.int64).reshape(8, 8)
+    assert all((np.array(out) == M).ravel())          # the butterfly is the codec's matrix M
+    xmax = 2048
+    row_mult = max(int(np.abs(m).sum()) * xmax for m in mult)
+    row_out = [int(np.abs(o).sum()) * xmax for o in out]          # bound of the row output of frequency v
+    assert row_mult < 2 ** 23 and max(row_out) < 2 ** 23
+    col_mult = max(int(np.abs(m).sum()) * b for m in mult for b in row_out)
+    col_prod = max(13 * int(np.abs(m).sum()) * b for m in mult for b in row_out)
+    col_out = max(int(np.abs(o).sum()) * b for o in out for b in row_out)
+    assert col_mult < 2 ** 23, col_mult
+    assert col_prod < 2 ** 31 and col_out < 2 ** 31
+    # 16-bit samples (the int32 path) would not fit: the bound is tight enough to matter
+    assert max(int(np.abs(m).sum()) * b * 16 for m in mult for b in row_out) >= 2 ** 23

@@ -79,6 +79,35 @@ __device__ static inline void fq_load(const tcg_eplane& P, const FqTile& T, int3
     }
 }
 
+// The 8-point forward transform of tc_fdct8_i32 with 24-bit multiplies
+// (v_mul_i32_i24, full rate, where a 32-bit multiply issues at quarter rate).
+// Exact when every multiplicand is below 2^23 in magnitude: for samples of at
+// most 12 bits (level-shifted, |x| <= 2048) the row pass's sums and
+// differences stay below 2^15 and its outputs below 13 * 2^14 < 2^18; the
+// column pass's sums of eight such values stay below 2^21, and every product
+// (|13 * sum| < 2^25) fits the 32 bits the instruction returns.
+__device__ static inline void fdct8_i24(const int32_t* in, int32_t* out)
+{
+    const int32_t s0 = in[0] + in[7], s1 = in[1] + in[6], s2 = in[2] + in[5], s3 = in[3] + in[4];
+    const int32_t d0 = in[0] - in[7], d1 = in[1] - in[6], d2 = in[2] - in[5], d3 = in[3] - in[4];
+    const int32_t e0 = s0 + s3, e1 = s1 + s2, f0 = s0 - s3, f1 = s1 - s2;
+    out[0] = __mul24(13, e0 + e1);
+    out[4] = __mul24(9, e0 - e1);
+    out[2] = __mul24(12, f0) + __mul24(5, f1);
+    out[6] = __mul24(5, f0) - __mul24(12, f1);
+    out[1] = __mul24(13, d0) + __mul24(11, d1) + __mul24(7, d2) + d3;
+    out[3] = __mul24(11, d0) - __mul24(4, d1) - __mul24(13, d2) - __mul24(8, d3);
+    out[5] = __mul24(8, d0) - __mul24(13, d1) + __mul24(4, d2) + __mul24(11, d3);
+    out[7] = d0 - __mul24(7, d1) + __mul24(11, d2) - __mul24(13, d3);
+}
+
+template <bool WIDE>
+__device__ static inline void fdct8_pass(const int32_t* in, int32_t* out)
+{
+    if (WIDE) { tc_fdct8_i32(in, out); }
+    else { fdct8_i24(in, out); }
+}
+
 template <bool WIDE>
 __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int32_t fclamp, int32_t* x,
                                        int32_t* __restrict__ qbuf, int32_t* sA, const uint32_t* sQm,
@@ -95,7 +124,7 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int
                 for (int k = 0; k < 8; ++k) { x[k] = (int32_t)(int16_t)x[k]; }
             }
             int32_t A[8];
-            tc_fdct8_i32(x, A);
+            fdct8_pass<WIDE>(x, A);
 #pragma unroll
             for (int v = 0; v < 8; ++v) { sA[j * FQ_PITCH + r * 8 + v] = A[v]; }
         }
@@ -109,7 +138,7 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int
 #pragma unroll
         for (int y = 0; y < 8; ++y) { a[y] = sA[j * FQ_PITCH + y * 8 + v]; }
         int32_t Fc[8];
-        tc_fdct8_i32(a, Fc);
+        fdct8_pass<WIDE>(a, Fc);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             int32_t F = Fc[u];
