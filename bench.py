@@ -9,7 +9,7 @@ slice) at fixed qp 62. Inputs are resident in HBM before the timed region.
 A step is one pass of the batch through the hot path: 8 frame encodes and 8
 frame decodes, all data in HBM.
  - served-batched path (the reported value): the serving loop (tcx_server_*,
-   continuous batching over two engine lanes, batches of up to 64 frames, HBM-budget admission) is given
+   continuous batching over two engine lanes, batches of up to 96 frames, HBM-budget admission) is given
    K steps' worth of requests (8K encodes of the resident frames and 8K decodes
    of their resident packets, interleaved) and timed until the last completes;
    warm-up: one pass of 8 + 8 checked requests, then the same K-step request
@@ -770,8 +770,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true")
-    ap.add_argument("--hbm-budget", type=int, default=24, help="serving loop HBM budget (GiB)")
-    ap.add_argument("--max-batch", type=int, default=64, help="serving loop: most frames per launch sequence")
+    ap.add_argument("--hbm-budget", type=int, default=48, help="serving loop HBM budget (GiB)")
+    ap.add_argument("--max-batch", type=int, default=96, help="serving loop: most frames per launch sequence")
     ap.add_argument("--lanes", type=int, default=2, help="serving loop: launch sequences in flight")
     ap.add_argument("--gather-us", type=int, default=200,
                     help="serving loop: arrival coalescing gap in microseconds (0: launch at once)")

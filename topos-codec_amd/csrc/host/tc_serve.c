@@ -34,7 +34,7 @@
 #include "topos_gpu.h"
 
 #define SRV_MAX_LANES 4u
-#define SRV_MAX_BATCH 64u
+#define SRV_MAX_BATCH 128u
 #define SRV_SLACK 1024u /* tail slack of a packet region: the decoder reads through aligned windows */
 
 enum { RQ_WAIT = 0, RQ_ADMITTED, RQ_RUNNING, RQ_DONE, RQ_POLLED };
