@@ -168,12 +168,12 @@ typedef struct tcx_server tcx_server;
 
 typedef struct tcx_server_config {
     size_t hbm_budget;      /* bytes of HBM for admitted requests (staged inputs, packets, planes) */
-    uint32_t max_batch;     /* most frames per launch sequence, 1..128 (0: 8) */
+    uint32_t max_batch;     /* most frames per launch sequence, 1..256 (0: 8) */
     uint32_t max_inflight;  /* launch sequences in flight at once, 1..4 (0: 4) */
     uint32_t gather_us;     /* arrival coalescing: while requests keep arriving less than
-                               gather_us apart and a batch is not full, its launch waits for
-                               them, at most 20 x gather_us after its oldest request arrived
-                               (0: launch at once) */
+                               gather_us apart, a launch waits for them, at most 20 x gather_us
+                               after its oldest request arrived, then splits the queue into
+                               equal batches (0: launch at once) */
 } tcx_server_config;
 
 #define TCX_REQ_ENCODE 1u

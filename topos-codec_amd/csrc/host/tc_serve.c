@@ -9,13 +9,13 @@
  *  2. starts a launch sequence on every free lane: the direction whose oldest
  *     admitted request arrived first, its admitted requests in arrival order,
  *     mixed sizes and profiles in one batch. A queue longer than max_batch is
- *     cut into equal batches (160 waiting with max_batch 64: 54, 53, 53), so
+ *     cut into equal batches (160 waiting with max_batch 96: 80 and 80), so
  *     the last one is not a remainder that holds a lane for a whole serial
  *     entropy chain with a fraction of the device busy. While requests keep
- *     arriving less than gather_us apart and the batch is not full, the
- *     launch waits for them (at most 20 x gather_us after its oldest request
- *     arrived): a burst then forms full batches instead of a first batch of
- *     one frame, and isolated arrivals are not delayed;
+ *     arriving less than gather_us apart, the launch waits for them (at most
+ *     20 x gather_us after its oldest request arrived): a burst then forms
+ *     equal full batches instead of a first batch of one frame, and isolated
+ *     arrivals are not delayed;
  *  3. polls the running lanes without blocking; a finished lane's requests
  *     complete, their reservations shrink to what the result holds (packet
  *     bytes / output planes) and the staged inputs return to the budget.
@@ -34,7 +34,7 @@
 #include "topos_gpu.h"
 
 #define SRV_MAX_LANES 4u
-#define SRV_MAX_BATCH 128u
+#define SRV_MAX_BATCH 256u
 #define SRV_SLACK 1024u /* tail slack of a packet region: the decoder reads through aligned windows */
 
 enum { RQ_WAIT = 0, RQ_ADMITTED, RQ_RUNNING, RQ_DONE, RQ_POLLED };
@@ -341,7 +341,7 @@ static int launch_locked(tcx_server* s)
         else if (s->q_dec.head == NULL) { src = &s->q_enc; }
         else { src = s->q_enc.head->t_submit <= s->q_dec.head->t_submit ? &s->q_enc : &s->q_dec; }
         const int64_t now = tcx_now_ns();
-        if (s->gather_ns > 0 && !s->stop && src->n < s->max_batch && now - s->last_submit < s->gather_ns &&
+        if (s->gather_ns > 0 && !s->stop && now - s->last_submit < s->gather_ns &&
             now - src->head->t_submit < 20 * s->gather_ns) {
             /* arrivals are still streaming in: wait for them */
             const int64_t due_gap = s->last_submit + s->gather_ns, due_max = src->head->t_submit + 20 * s->gather_ns;
