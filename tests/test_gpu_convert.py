@@ -176,3 +176,40 @@ def test_converted_frame_encodes_like_oracle(oracle, abi, codec):
     st, sizes, rcs, _ = codec.encode_batch_dev([cfg], [[b.addr for b in planes]], [out])
     assert st == 0 and rcs == [0], codec.last_error()
     assert sizes[0] == len(pkt) and out.download(sizes[0]).tobytes() == pkt
+
+
+@pytest.mark.parametrize("fmt", [0, 2, 5, 7])
+@pytest.mark.parametrize("w", [1001, 1002, 1003, 1004])
+def test_convert_device_quads_and_row_tails(oracle, abi, codec, fmt, w):
+    """Device-resident buffers that take four pixels per thread (16-byte
+    aligned source rows, output rows padded to 8 elements): 3- and 4-channel,
+    8- and 16-bit sources, widths ending in a full quad and in a 1-, 2- or
+    3-pixel tail, every output mode, against the oracle."""
+    rng = np.random.default_rng(70 + fmt + w)
+    h = 5
+    src, _ = cvt_source(rng, fmt, abi, w, h)
+    nch, smp = abi.cvt_channels(fmt), abi.cvt_sample_bytes(fmt)
+    pitch = (w * nch * smp + 15) // 16 * 16 + 16
+    padded = np.zeros((h, pitch), np.uint8)
+    padded[:, : w * nch * smp] = src
+    dsrc = codec.alloc(pitch * h)
+    dsrc.upload(padded)
+    for mode in (0, 1, 2):
+        rc, _, fo = cvt_call(oracle.orc_convert_packed_rgb, abi, src, w, h, fmt, mode, with_alpha=True)
+        assert rc == 0
+        shapes = abi.cvt_plane_shapes(w, h, mode, True)
+        strides = [(s[1] + 7) // 8 * 8 + 8 for s in shapes]
+        outs = [codec.alloc(st * s[0] * 2) for st, s in zip(strides, shapes)]
+        for b, st, s in zip(outs, strides, shapes):
+            b.upload(np.full(st * s[0], 0xBEEF, np.uint16))
+        p = abi.CvtParams()
+        p.format, p.width, p.height, p.out_mode, p.bit_depth = fmt, w, h, mode, 10
+        p.kr, p.kb = 0.2126, 0.0722
+        p.src, p.src_stride = dsrc.addr, pitch
+        p.y_out, p.u_out, p.v_out, p.a_out = (b.addr for b in outs)
+        p.y_stride, p.u_stride, p.v_stride, p.a_stride = strides
+        assert codec.convert_packed_rgb_dev(p) == 0, codec.last_error()
+        for b, st, sh, want in zip(outs, strides, shapes, fo):
+            got = b.download(st * sh[0] * 2, np.uint16).reshape(sh[0], st)
+            assert np.array_equal(got[:, : sh[1]], want), (fmt, w, mode)
+            assert (got[:, sh[1]:] == 0xBEEF).all(), (fmt, w, mode)

@@ -226,7 +226,7 @@ def _round_f32(x):
 
 @pytest.mark.parametrize("d", [255, 65535])
 def test_conversion_normalisation_exact(d):
-    """The conversion kernel's c / d for integer samples (tc_convert.hip cvt_px):
+    """The conversion kernel's c / d for integer samples (tc_convert.hip cvt_norm):
     q0 = c * r, res = fma(-q0, d, c), q = fma(res, r, q0) with r = 1 / d
     correctly rounded, each fma rounded once, equals the IEEE quotient for every
     c in 0..d."""
@@ -242,6 +242,44 @@ def test_conversion_normalisation_exact(d):
         res = _round_f32(ci - q * fd)
         got = _round_f32(Fraction(float(res)) * fr + q)
         assert got == want[i], (d, i)
+
+
+@pytest.mark.parametrize("kr,kb", [(0.2126, 0.0722), (0.299, 0.114), (0.2627, 0.0593)])
+def test_conversion_chroma_division_exact(kr, kb):
+    """The conversion kernel's chroma division (tc_convert.hip cvt_div): the
+    binary64 product a * RN64(1 / d), rounded once to binary32, equals the IEEE
+    binary32 quotient a / d for d = 2 (1 - kb) and 2 (1 - kr) as the host
+    rounds them. Numerators: random binary32 values over every binade from
+    2^-40 to 1 (both signs), the (b - y) and (r - y) of random 8- and 16-bit
+    colours in the kernel's operation order, and the binary32 neighbours of
+    m d for quotients m just below, at and above binary32 midpoints (the
+    hardest cases). The binary32 reciprocal product fails on this set."""
+    rng = np.random.default_rng(5)
+    n = 1 << 20
+    nums = [np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-40, 0, n)) * rng.choice([-1.0, 1.0], n)]
+    for norm in (255.0, 65535.0):
+        c = (rng.integers(0, int(norm) + 1, (3, n)).astype(np.float32) / np.float32(norm)).astype(np.float32)
+        rf, gf, bf = c
+        kf = np.float32(1.0 - kr - kb)
+        y = (np.float32(kr) * rf + kf * gf).astype(np.float32)
+        y = (y + np.float32(kb) * bf).astype(np.float32)
+        nums += [bf - y, rf - y]
+    for dd in (2.0 * (1.0 - kb), 2.0 * (1.0 - kr)):
+        d = np.float32(dd)
+        # quotients near midpoints: m = (2j + 1) 2^(e - 24) in [2^e, 2^(e+1))
+        e = rng.integers(-20, 0, n)
+        j = rng.integers(1 << 22, 1 << 23, n)
+        m = np.ldexp((2 * j + 1).astype(np.float64), e - 24)
+        a = (m * np.float64(d)).astype(np.float32)
+        nums += [a, np.nextafter(a, np.float32(-2)), np.nextafter(a, np.float32(2))]
+    a_all = np.concatenate([np.asarray(v, np.float32) for v in nums])
+    for dd in (2.0 * (1.0 - kb), 2.0 * (1.0 - kr)):
+        d = np.float32(dd)
+        want = a_all / d
+        rd = 1.0 / np.float64(d)
+        got = (a_all.astype(np.float64) * rd).astype(np.float32)
+        assert np.array_equal(got, want), int(np.count_nonzero(got != want))
+        assert np.count_nonzero(a_all * (np.float32(1.0) / d) != want) > 1000
 
 
 def _idct8_f64(v):

@@ -75,6 +75,8 @@ static int32_t cvt_plan(const topos_cvt_params* p, tcg_cvt* k)
         k->kf = (float)(1.0 - krd - kbd);
         k->du = (float)(2.0 * (1.0 - kbd));
         k->dv = (float)(2.0 * (1.0 - krd));
+        k->rdu = 1.0 / (double)k->du;
+        k->rdv = 1.0 / (double)k->dv;
         if (p->full_range) {
             k->y_mult = k->scale; k->y_off = 0.0f;
             k->c_mult = k->scale; k->c_off = 0.0f;
@@ -97,7 +99,9 @@ int32_t tc_convert_packed_rgb(const topos_cvt_params* params)
     const size_t row = (size_t)k.width * k.nch * k.smp;
     const size_t in_pitch = al256(row);
     const uint32_t cwid = k.mode == TOPOS_CVT_OUT_YUV422 ? k.cw : k.width;
-    const size_t yb = al256((size_t)k.width * k.height * 2u), cb = al256((size_t)cwid * k.height * 2u);
+    /* staged output rows start on 256-byte boundaries (the kernel's wide stores) */
+    const uint32_t yp = (uint32_t)(al256((size_t)k.width * 2u) / 2u), cp = (uint32_t)(al256((size_t)cwid * 2u) / 2u);
+    const size_t yb = (size_t)yp * k.height * 2u, cb = (size_t)cp * k.height * 2u;
     const int has_a = params->a_out != NULL;
     tc_ectx* c = NULL;
     if ((rc = tc_engine_acquire(&c)) != TC_OK) { return rc; }
@@ -115,16 +119,16 @@ int32_t tc_convert_packed_rgb(const topos_cvt_params* params)
         d.u = dout + yb;
         d.v = dout + yb + cb;
         d.a = has_a ? dout + yb + 2u * cb : 0u;
-        d.y_st = k.width;
-        d.u_st = cwid;
-        d.v_st = cwid;
-        d.a_st = k.width;
+        d.y_st = yp;
+        d.u_st = cp;
+        d.v_st = cp;
+        d.a_st = yp;
         if (tcg_convert_rgb(g, &d) != 0) { rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "cvt: %s", tcg_last_error()); }
-        struct { uint16_t* dst; uint32_t st; uint64_t src; uint32_t w; } outs[4] = {
-            { params->y_out, k.y_st, d.y, k.width }, { params->u_out, k.u_st, d.u, cwid },
-            { params->v_out, k.v_st, d.v, cwid }, { params->a_out, k.a_st, d.a, k.width } };
+        struct { uint16_t* dst; uint32_t st; uint64_t src; uint32_t pitch; uint32_t w; } outs[4] = {
+            { params->y_out, k.y_st, d.y, yp, k.width }, { params->u_out, k.u_st, d.u, cp, cwid },
+            { params->v_out, k.v_st, d.v, cp, cwid }, { params->a_out, k.a_st, d.a, yp, k.width } };
         for (int i = 0; i < (has_a ? 4 : 3) && rc == TC_OK; ++i) {
-            if (tcg_d2h_2d(g, outs[i].dst, (size_t)outs[i].st * 2u, outs[i].src, (size_t)outs[i].w * 2u,
+            if (tcg_d2h_2d(g, outs[i].dst, (size_t)outs[i].st * 2u, outs[i].src, (size_t)outs[i].pitch * 2u,
                            (size_t)outs[i].w * 2u, k.height) != 0) {
                 rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "cvt: plane download: %s", tcg_last_error());
             }

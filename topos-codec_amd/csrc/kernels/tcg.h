@@ -185,6 +185,7 @@ typedef struct tcg_cvt {
     uint32_t vec;               /* set by the launcher: aligned 4-channel rows */
     float scale, kr, kb, kf, du, dv, y_mult, y_off, c_mult, c_off, a_nmax, norm_div;
     float norm_rcp;             /* 1 / norm_div, correctly rounded */
+    double rdu, rdv;            /* 1 / du and 1 / dv in binary64, correctly rounded */
 } tcg_cvt;
 
 /* 4:4:4 -> 4:2:2 chroma (device addresses, strides in elements) */
