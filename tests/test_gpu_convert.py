@@ -135,7 +135,7 @@ def test_convert_device_strides(oracle, abi, codec, fmt, w, pad):
 
 def test_planar_444_to_422(oracle, abi, codec):
     rng = np.random.default_rng(13)
-    for w, h in ((1, 3), (3, 2), (257, 5), (3840, 2160)):
+    for w, h in ((1, 3), (3, 2), (7, 2), (8, 2), (9, 2), (15, 2), (16, 2), (17, 2), (257, 5), (3840, 2160)):
         u = rng.integers(0, 65536, size=(h, w + 7), dtype=np.uint16)
         v = rng.integers(0, 65536, size=(h, w + 7), dtype=np.uint16)
         rc_o, uo, vo = p422_call(oracle.orc_convert_444_422, abi, u, v, w, h, bit_depth=16, in_stride=w + 7,
@@ -213,3 +213,35 @@ def test_convert_device_quads_and_row_tails(oracle, abi, codec, fmt, w):
             got = b.download(st * sh[0] * 2, np.uint16).reshape(sh[0], st)
             assert np.array_equal(got[:, : sh[1]], want), (fmt, w, mode)
             assert (got[:, sh[1]:] == 0xBEEF).all(), (fmt, w, mode)
+
+
+@pytest.mark.parametrize("w", [1, 9, 16, 17, 1000, 1001, 1003])
+@pytest.mark.parametrize("aligned", [True, False])
+def test_planar_444_to_422_device(oracle, abi, codec, w, aligned):
+    """Device-resident planes: rows 16-byte aligned in and 8-byte aligned out
+    (four output samples per thread, row tails sample by sample) and odd
+    strides (one thread per sample), against the oracle; padding untouched."""
+    rng = np.random.default_rng(90 + w)
+    h = 4
+    cw = (w + 1) // 2
+    ist = (w + 7) // 8 * 8 + 8 if aligned else w + 3
+    ost = (cw + 3) // 4 * 4 + 4 if aligned else cw + 1
+    u = rng.integers(0, 65536, size=(h, ist), dtype=np.uint16)
+    v = rng.integers(0, 65536, size=(h, ist), dtype=np.uint16)
+    rc, uo, vo = p422_call(oracle.orc_convert_444_422, abi, u, v, w, h, bit_depth=16, in_stride=ist)
+    assert rc == 0
+    bufs = [codec.alloc(a.nbytes) for a in (u, v)]
+    for b, a in zip(bufs, (u, v)):
+        b.upload(a)
+    outs = [codec.alloc(ost * h * 2) for _ in range(2)]
+    for b in outs:
+        b.upload(np.full(ost * h, 0xBEEF, np.uint16))
+    p = abi.Planar422Params()
+    p.width, p.height, p.bit_depth = w, h, 16
+    p.u_in, p.u_in_stride, p.v_in, p.v_in_stride = bufs[0].addr, ist, bufs[1].addr, ist
+    p.u_out, p.u_out_stride, p.v_out, p.v_out_stride = outs[0].addr, ost, outs[1].addr, ost
+    assert codec.convert_yuv444_to_422_dev(p) == 0, codec.last_error()
+    for b, want in zip(outs, (uo, vo)):
+        got = b.download(ost * h * 2, np.uint16).reshape(h, ost)
+        assert np.array_equal(got[:, :cw], want), (w, aligned)
+        assert (got[:, cw:] == 0xBEEF).all(), (w, aligned)

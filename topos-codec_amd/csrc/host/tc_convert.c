@@ -188,7 +188,9 @@ int32_t tc_convert_yuv444_to_422(const topos_planar_444_to_422_params* params)
     tcg_p422 k;
     int32_t rc = p422_plan(params, &k);
     if (rc != TC_OK) { return rc; }
-    const size_t ib = al256((size_t)k.width * k.height * 2u), ob = al256((size_t)k.cw * k.height * 2u);
+    /* staged rows start on 256-byte boundaries (the kernel's vector accesses) */
+    const size_t ip = al256((size_t)k.width * 2u), op = al256((size_t)k.cw * 2u);
+    const size_t ib = ip * k.height, ob = op * k.height;
     tc_ectx* c = NULL;
     if ((rc = tc_engine_acquire(&c)) != TC_OK) { return rc; }
     uint64_t din = 0, dout = 0;
@@ -196,19 +198,19 @@ int32_t tc_convert_yuv444_to_422(const topos_planar_444_to_422_params* params)
     tcg_ctx* g = tc_engine_stream(c);
     tcg_p422 d = k;
     d.u_in = din; d.v_in = din + ib; d.u_out = dout; d.v_out = dout + ob;
-    d.uis = k.width; d.vis = k.width; d.uos = k.cw; d.vos = k.cw;
-    if (rc == TC_OK && (tcg_h2d_2d(g, d.u_in, (size_t)k.width * 2u, params->u_in, (size_t)k.uis * 2u,
+    d.uis = ip / 2u; d.vis = ip / 2u; d.uos = op / 2u; d.vos = op / 2u;
+    if (rc == TC_OK && (tcg_h2d_2d(g, d.u_in, ip, params->u_in, (size_t)k.uis * 2u,
                                    (size_t)k.width * 2u, k.height) != 0 ||
-                        tcg_h2d_2d(g, d.v_in, (size_t)k.width * 2u, params->v_in, (size_t)k.vis * 2u,
+                        tcg_h2d_2d(g, d.v_in, ip, params->v_in, (size_t)k.vis * 2u,
                                    (size_t)k.width * 2u, k.height) != 0)) {
         rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "planar 444->422: upload: %s", tcg_last_error());
     }
     if (rc == TC_OK && tcg_convert_444_422(g, &d) != 0) {
         rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "planar 444->422: %s", tcg_last_error());
     }
-    if (rc == TC_OK && (tcg_d2h_2d(g, params->u_out, (size_t)k.uos * 2u, d.u_out, (size_t)k.cw * 2u,
+    if (rc == TC_OK && (tcg_d2h_2d(g, params->u_out, (size_t)k.uos * 2u, d.u_out, op,
                                    (size_t)k.cw * 2u, k.height) != 0 ||
-                        tcg_d2h_2d(g, params->v_out, (size_t)k.vos * 2u, d.v_out, (size_t)k.cw * 2u,
+                        tcg_d2h_2d(g, params->v_out, (size_t)k.vos * 2u, d.v_out, op,
                                    (size_t)k.cw * 2u, k.height) != 0)) {
         rc = tc_fail(TC_ERR_OUT_OF_MEMORY, "planar 444->422: download: %s", tcg_last_error());
     }

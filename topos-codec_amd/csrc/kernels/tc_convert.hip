@@ -348,17 +348,63 @@ __global__ __launch_bounds__(256) void k_convert_rgb4(const tcg_cvt k)
     }
 }
 
+// rounded mean of horizontally adjacent samples 2x and 2x + 1 (the last
+// sample repeated at an odd width)
+__device__ inline uint16_t p422_px(const uint16_t* in, uint32_t x, uint32_t width)
+{
+    const uint32_t sx = x * 2u;
+    const uint32_t sx1 = sx + 1u < width ? sx + 1u : sx;
+    return (uint16_t)(((uint32_t)in[sx] + (uint32_t)in[sx1] + 1u) >> 1);
+}
+
+// one thread per output sample of row blockIdx.y: any base address and stride
 __global__ __launch_bounds__(256) void k_convert_444_422(const tcg_p422 k)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t y = blockIdx.y;
     if (x >= k.cw || y >= k.height) { return; }
-    const uint32_t sx = x * 2u;
-    const uint32_t sx1 = sx + 1u < k.width ? sx + 1u : sx;
     const uint16_t* us = (const uint16_t*)k.u_in + (size_t)y * k.uis;
     const uint16_t* vs = (const uint16_t*)k.v_in + (size_t)y * k.vis;
-    ((uint16_t*)k.u_out)[(size_t)y * k.uos + x] = (uint16_t)(((uint32_t)us[sx] + (uint32_t)us[sx1] + 1u) >> 1);
-    ((uint16_t*)k.v_out)[(size_t)y * k.vos + x] = (uint16_t)(((uint32_t)vs[sx] + (uint32_t)vs[sx1] + 1u) >> 1);
+    ((uint16_t*)k.u_out)[(size_t)y * k.uos + x] = p422_px(us, x, k.width);
+    ((uint16_t*)k.v_out)[(size_t)y * k.vos + x] = p422_px(vs, x, k.width);
+}
+
+// four output samples of one plane row from one 16-byte load of eight input
+// samples: (a + b + 1) >> 1 of each 16-bit pair, per dword
+__device__ inline cvt_u32x2 p422_four(const cvt_u32x4 w)
+{
+    const uint32_t m[4] = { w.x, w.y, w.z, w.w };
+    uint32_t r[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) { r[i] = ((m[i] & 0xFFFFu) + (m[i] >> 16) + 1u) >> 1; }
+    return cvt_u32x2{ r[0] | r[1] << 16, r[2] | r[3] << 16 };
+}
+
+// One thread per four output samples (x = 4 t .. 4 t + 3) of row blockIdx.y
+// in both chroma planes, for input rows whose base and stride are 16-byte
+// aligned and output rows that take 8-byte stores: one 16-byte load and one
+// 8-byte non-temporal store per plane. The last group of a row, when it is
+// short of eight input samples, goes sample by sample.
+__global__ __launch_bounds__(256) void k_convert_444_422_4(const tcg_p422 k)
+{
+    const uint32_t x0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
+    const uint32_t y = blockIdx.y;
+    if (x0 >= k.cw || y >= k.height) { return; }
+    const uint16_t* us = (const uint16_t*)k.u_in + (size_t)y * k.uis;
+    const uint16_t* vs = (const uint16_t*)k.v_in + (size_t)y * k.vis;
+    uint16_t* uo = (uint16_t*)k.u_out + (size_t)y * k.uos;
+    uint16_t* vo = (uint16_t*)k.v_out + (size_t)y * k.vos;
+    if (2u * x0 + 8u > k.width) {
+        for (uint32_t x = x0; x < k.cw; ++x) {
+            uo[x] = p422_px(us, x, k.width);
+            vo[x] = p422_px(vs, x, k.width);
+        }
+        return;
+    }
+    const cvt_u32x4 wu = *(cvt_gsrc4*)(us + 2u * x0);
+    const cvt_u32x4 wv = *(cvt_gsrc4*)(vs + 2u * x0);
+    __builtin_nontemporal_store(p422_four(wu), (cvt_gdst2*)(uo + x0));
+    __builtin_nontemporal_store(p422_four(wv), (cvt_gdst2*)(vo + x0));
 }
 
 }  // namespace
@@ -404,7 +450,13 @@ extern "C" int32_t tcg_convert_444_422(tcg_ctx* c, const tcg_p422* k)
 {
     c->kind = 3;
     tcg_mark(c, 0);
-    hipLaunchKernelGGL(k_convert_444_422, dim3((k->cw + 255u) / 256u, k->height), dim3(256), 0, c->stream, *k);
+    const bool in16 = ((k->u_in | k->v_in | k->uis * 2u | k->vis * 2u) & 15u) == 0u;
+    const bool out8 = ((k->u_out | k->v_out | k->uos * 2u | k->vos * 2u) & 7u) == 0u;
+    if (in16 && out8) {
+        hipLaunchKernelGGL(k_convert_444_422_4, dim3((k->cw + 1023u) / 1024u, k->height), dim3(256), 0, c->stream, *k);
+    } else {
+        hipLaunchKernelGGL(k_convert_444_422, dim3((k->cw + 255u) / 256u, k->height), dim3(256), 0, c->stream, *k);
+    }
     tcg_mark(c, 1);
     return tcg_check("444 to 422 launch");
 }
