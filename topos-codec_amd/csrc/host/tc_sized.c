@@ -177,12 +177,22 @@ static int32_t sz_run(tc_ectx* ec, sz_item* items, uint32_t n, int mode)
         }
         if (missing == 0u) { break; }
         if (round > 4u * SZ_BUDGET) { rc = tc_fail(TC_ERR_STATE, "sized encode: search does not converge"); break; }
-        /* speculation width: up to 9 qps for a lone search, fewer per frame when many share a launch */
-        uint32_t want = 64u / missing;
-        want = want < 3u ? 3u : want > 9u ? 9u : want;
+        /* speculation width: up to 9 qps for a lone search, fewer per frame
+         * when many share a launch. The first round of a lone search without
+         * a seed state takes 17: its second probe lands anywhere within the
+         * log-slope jump, so a wide first round usually saves a launch (a
+         * launch of 17 candidates of a 4K 422 HQ frame takes ~40 ms against
+         * ~31 ms for 9; cold searches to 3 / 6 / 9 MB took 77 / 63 / 36 ms
+         * with 9 and 61 / 42 / 43 ms with 17). Searches sharing a launch
+         * stay narrow: 8 frames x 16 candidates made each round slower than
+         * the launch it saved. */
+        const uint32_t share = 64u / missing;
         uint32_t nc = 0;
         for (uint32_t i = 0; i < n; ++i) {
             if (items[i].done) { continue; }
+            const int wide = round == 0u && missing == 1u && !(items[i].seed_on && items[i].seed.valid);
+            uint32_t want = wide ? 17u : share;
+            want = want < 3u ? 3u : want > (wide ? 17u : 9u) ? (wide ? 17u : 9u) : want;
             uint32_t qs[32];
             const uint32_t k = sz_candidates(&items[i], want, qs);
             for (uint32_t j = 0; j < k; ++j) { cands[nc++] = (sz_cand){ i, qs[j] }; }
