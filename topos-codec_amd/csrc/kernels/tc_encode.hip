@@ -144,7 +144,7 @@ __device__ static void fdct_quant_tile(const tcg_eplane& P, const FqTile& T, int
             int32_t F = Fc[u];
             const uint32_t nat = (uint32_t)u * 8u + v;
             if (WIDE) { F = predistort(F, nat); }
-            int32_t c = F > fclamp ? fclamp : (F < -fclamp ? -fclamp : F);
+            const int32_t c = min(max(F, -fclamp), fclamp);   // one v_med3_i32
             const uint32_t mag = (uint32_t)(c < 0 ? -c : c);
             // Q/2 - dz >= 0 (dz = Q/4 for AC, 0 for DC), so |F| + Q/2 - dz never
             // goes below zero and needs no saturation
